@@ -1,0 +1,978 @@
+// knc_capi.hip -- host side of the library: the C ABI declared in
+// include/knc_gpu.h and the context that keeps a KNC State resident in HBM.
+//
+// The context mirrors the pieces of the reference's scheme that surround
+// the stage kernel: the block set and its radial chain (State,
+// src/state.rs:26-43, with neighbours at index +-1 as in src/scheme.rs:70-75),
+// the geometry cache (src/mesh.rs:246-264, kept as 1-D factors), the RK
+// driver (godunov-core RungeKuttaOrder, weights of src/state.rs:179-212), the
+// CFL time step (src/state.rs:143-162) and scheme::advance's fold loop
+// (src/scheme.rs:212-240).  Errors follow HydroError (src/physics/mod.rs:
+// 82-117) with the lowest (step, block, i, j) reported.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/knc_gpu.h"
+#include "knc_device.hpp"
+#include "knc_kernels.hpp"
+#include "knc_zone.hpp"
+
+using namespace knc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(KNC_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+
+int check_hydro(const knc_hydro* h)
+{
+    if (!h) return fail(KNC_ERR_INVALID_ARGUMENT, "hydro is NULL");
+    if (h->system != KNC_EULER && h->system != KNC_SRHD) return fail(KNC_ERR_INVALID_ARGUMENT, "unknown system");
+    if (h->riemann_solver != KNC_HLLE && h->riemann_solver != KNC_HLLC)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "unknown riemann solver");
+    if (h->runge_kutta_order < 1 || h->runge_kutta_order > 3)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "runge_kutta_order must be 1, 2 or 3");
+    // Hydrodynamics::validate (relativistic_hydro.rs:46-54, newtonian_hydro.rs:38-46)
+    if (h->plm_theta < 1.0 || h->plm_theta > 2.0)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "plm_theta must be in the range [1, 2]");
+    if (h->cfl_number < 0.0 || h->cfl_number > 0.7)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "cfl_number must be in the range [0.0, 0.7]");
+    if (!(h->gamma_law_index > 1.0)) return fail(KNC_ERR_INVALID_ARGUMENT, "gamma_law_index must exceed 1");
+    if (!(h->light_speed > 0.0)) return fail(KNC_ERR_INVALID_ARGUMENT, "light_speed must be positive");
+    return KNC_OK;
+}
+
+ZoneParams zone_params(const knc_hydro& h)
+{
+    ZoneParams zp;
+    zp.gamma = h.gamma_law_index;
+    zp.gm1 = h.gamma_law_index - 1.0;
+    zp.gfac = h.gamma_law_index / (h.gamma_law_index - 1.0);
+    zp.plm_theta = h.plm_theta;
+    zp.c = h.light_speed;
+    return zp;
+}
+
+// Radial table of one block, built with the operation order of the oracle /
+// the reference formulas (src/mesh.rs:119-123,157-161).
+std::vector<double> radial_table(int nr, const double* rv)
+{
+    std::vector<double> t((size_t)rgeo_size(nr));
+    double* v = t.data();
+    double* volA = v + (nr + 1);
+    double* rc = volA + nr;
+    double* dr = rc + nr;
+    for (int i = 0; i <= nr; ++i) v[i] = rv[i];
+    for (int i = 0; i < nr; ++i) {
+        const double r0 = rv[i], r1 = rv[i + 1];
+        volA[i] = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
+        rc[i] = std::sqrt(r0 * r1);
+        dr[i] = r1 - r0;
+    }
+    return t;
+}
+
+struct HostBlock {
+    int32_t index = 0;
+    int nr = 0, nq = 0;
+    double* mem = nullptr;      // NSLOT * 5 * nr * nq doubles
+    double* rgeo = nullptr;     // device radial table
+    std::vector<double> rv;
+};
+
+}  // namespace
+
+struct knc_context {
+    knc_hydro h{};
+    knc_options o{};
+    ZoneParams zp{};
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+
+    std::map<int32_t, HostBlock> blocks;  // ordered by radial index
+    int cur = 0;                          // slot holding U^n
+    double time = 0.0;
+    int64_t iteration = 0;
+
+    // polar table
+    int nq = 0;
+    std::vector<double> qv;
+    double* d_theta = nullptr;
+    ThetaTable th{};
+
+    // boundary rows
+    int inner_mode = BND_OUTFLOW, outer_mode = BND_OUTFLOW;
+    double* d_bnd = nullptr;   // 2 x (5 planes x 2nq)
+
+    // launch tables
+    bool dirty = true;
+    int rows_per_item = 16;
+    BlockDesc* d_blocks = nullptr;
+    int cap_blocks = 0;
+    WorkItem* d_items = nullptr;
+    int cap_items = 0;
+    int32_t* d_nitems = nullptr;
+    int nitems = 0;
+    std::vector<std::pair<int, int>> block_items;  // (offset, count) per table entry
+    int threads = 64;
+    int persistent_grid = 0;
+
+    // per-call device state
+    ErrLog* d_err = nullptr;
+    C2PStats* d_stats = nullptr;
+    unsigned long long* d_min_bits = nullptr;
+    double* d_dt = nullptr;
+
+    // graphs keyed by (starting slot, fold)
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+
+    // timing
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_stage_ms = 0.0;
+    int last_launches = 0;
+};
+
+namespace {
+
+void drop_graphs(knc_context* c)
+{
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+}
+
+int upload_theta(knc_context* c, int nq, const double* qv)
+{
+    c->nq = nq;
+    c->qv.assign(qv, qv + nq + 1);
+    std::vector<double> t((size_t)6 * (nq + 1), 0.0);
+    double* sinq = t.data();
+    double* cosq = sinq + (nq + 1);
+    double* dcost = cosq + (nq + 1);
+    double* cot = dcost + (nq + 1);
+    double* dq = cot + (nq + 1);
+    double* qc = dq + (nq + 1);
+    for (int j = 0; j <= nq; ++j) {
+        sinq[j] = std::sin(qv[j]);
+        cosq[j] = std::cos(qv[j]);
+    }
+    for (int j = 0; j < nq; ++j) {
+        dcost[j] = -(std::cos(qv[j + 1]) - std::cos(qv[j]));
+        const double q = 0.5 * (qv[j] + qv[j + 1]);
+        qc[j] = q;
+        cot[j] = std::cos(q) / std::sin(q);
+        dq[j] = qv[j + 1] - qv[j];
+    }
+    if (c->d_theta) HIP_TRY(hipFree(c->d_theta));
+    HIP_TRY(hipMalloc(&c->d_theta, t.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(c->d_theta, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->th.sinq = c->d_theta;
+    c->th.cosq = c->d_theta + (nq + 1);
+    c->th.dcost = c->d_theta + 2 * (nq + 1);
+    c->th.cot = c->d_theta + 3 * (nq + 1);
+    c->th.dq = c->d_theta + 4 * (nq + 1);
+    c->th.qc = c->d_theta + 5 * (nq + 1);
+    c->th.nq = nq;
+    if (c->d_bnd) HIP_TRY(hipFree(c->d_bnd));
+    HIP_TRY(hipMalloc(&c->d_bnd, (size_t)2 * 5 * 2 * nq * sizeof(double)));
+    HIP_TRY(hipMemset(c->d_bnd, 0, (size_t)2 * 5 * 2 * nq * sizeof(double)));
+    int threads = (nq + 63) / 64 * 64;
+    c->threads = threads;
+    return KNC_OK;
+}
+
+int choose_rows(knc_context* c)
+{
+    if (c->o.rows_per_item > 0) return c->o.rows_per_item;
+    // Enough work items to give every CU several workgroups, while keeping the
+    // 4 recomputed halo rows per item a small fraction of the item.
+    int64_t rows = 0;
+    for (auto& kv : c->blocks) rows += kv.second.nr;
+    const int64_t want = (int64_t)c->num_cu * (c->threads <= 256 ? 4 : 1);
+    int r = (int)std::max<int64_t>(4, std::min<int64_t>(32, rows / std::max<int64_t>(1, want)));
+    return r;
+}
+
+// Rebuilds the device block table and work list after the block set changed.
+int rebuild_tables(knc_context* c)
+{
+    if (!c->dirty) return KNC_OK;
+    const int nb = (int)c->blocks.size();
+    c->rows_per_item = choose_rows(c);
+    std::vector<BlockDesc> desc;
+    std::vector<WorkItem> items;
+    c->block_items.clear();
+    int order = 0;
+    int32_t prev_index = 0;
+    for (auto& kv : c->blocks) {
+        HostBlock& b = kv.second;
+        if (order > 0 && b.index != prev_index + 1)
+            return fail(KNC_ERR_INVALID_ARGUMENT, "block indices must be contiguous (gap after %d)", prev_index);
+        prev_index = b.index;
+        BlockDesc d{};
+        const size_t plane = (size_t)5 * b.nr * b.nq;
+        for (int s = 0; s < NSLOT; ++s) d.slot[s] = b.mem + s * plane;
+        d.rgeo = b.rgeo;
+        d.nr = b.nr;
+        d.nq = b.nq;
+        d.left = order > 0 ? order - 1 : -1;
+        d.right = order + 1 < nb ? order + 1 : -1;
+        d.order = order;
+        d.index = b.index;
+        desc.push_back(d);
+        const int off = (int)items.size();
+        for (int i0 = 0; i0 < b.nr; i0 += c->rows_per_item)
+            items.push_back(WorkItem{order, i0, std::min(b.nr, i0 + c->rows_per_item), 0});
+        c->block_items.push_back({off, (int)items.size() - off});
+        ++order;
+    }
+    if (nb > c->cap_blocks) {
+        drop_graphs(c);
+        if (c->d_blocks) HIP_TRY(hipFree(c->d_blocks));
+        c->cap_blocks = std::max(nb, 2 * c->cap_blocks);
+        HIP_TRY(hipMalloc(&c->d_blocks, (size_t)c->cap_blocks * sizeof(BlockDesc)));
+    }
+    if ((int)items.size() > c->cap_items) {
+        drop_graphs(c);
+        if (c->d_items) HIP_TRY(hipFree(c->d_items));
+        c->cap_items = std::max((int)items.size(), 2 * c->cap_items);
+        HIP_TRY(hipMalloc(&c->d_items, (size_t)c->cap_items * sizeof(WorkItem)));
+    }
+    if (nb) HIP_TRY(hipMemcpyAsync(c->d_blocks, desc.data(), desc.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                                   c->stream));
+    if (!items.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                               c->stream));
+    c->nitems = (int)items.size();
+    int32_t n = c->nitems;
+    HIP_TRY(hipMemcpyAsync(c->d_nitems, &n, sizeof n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // Persistent grid: every CU holds as many stage workgroups as fit.
+    int per_cu = 1;
+    if (c->threads <= 256) per_cu = std::max(1, 512 / c->threads);  // ~2 waves/SIMD at <=256 VGPRs
+    c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
+    c->dirty = false;
+    return KNC_OK;
+}
+
+StageArgs base_args(knc_context* c)
+{
+    StageArgs a{};
+    a.blocks = c->d_blocks;
+    a.items = c->d_items;
+    a.nitems_ptr = nullptr;
+    a.nitems = c->nitems;
+    a.in_slot = c->cur;
+    a.out_slot = c->cur;
+    a.avg_slot = -1;
+    a.bweight = 0.0;
+    a.dt_dev = c->d_dt;
+    a.dt = 0.0;
+    a.zp = c->zp;
+    a.th = c->th;
+    a.bnd.inner_mode = c->inner_mode;
+    a.bnd.outer_mode = c->outer_mode;
+    a.bnd.inner_rows = c->d_bnd;
+    a.bnd.outer_rows = c->d_bnd + (size_t)5 * 2 * c->nq;
+    a.err = c->d_err;
+    a.stats = c->d_stats;
+    a.step = 0;
+    return a;
+}
+
+__global__ void k_reset_err(ErrLog* e)
+{
+    if (threadIdx.x == 0) {
+        e->min_key = ~0ull;
+        e->count = 0u;
+    }
+}
+
+__global__ void k_set_double(double* p, double v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+// Launches one stage over every block: per-block launches or one batched one.
+int launch_one_stage(knc_context* c, StageArgs a)
+{
+    const bool stats = c->o.collect_stats != 0;
+    if (c->o.launch_mode == KNC_LAUNCH_BATCHED) {
+        a.items = c->d_items;
+        a.nitems_ptr = c->d_nitems;
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, c->persistent_grid, c->threads, c->stream));
+        c->last_launches += 1;
+    } else {
+        for (auto& bi : c->block_items) {
+            a.items = c->d_items + bi.first;
+            a.nitems_ptr = nullptr;
+            a.nitems = bi.second;
+            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, bi.second, c->threads, c->stream));
+            c->last_launches += 1;
+        }
+    }
+    return KNC_OK;
+}
+
+// One RK step of RungeKuttaOrder (godunov-core) with the weighted averages of
+// src/state.rs:179-212 fused into the last stage.  Rotates c->cur.
+int enqueue_rk_step(knc_context* c, int step)
+{
+    const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
+    StageArgs a = base_args(c);
+    a.step = step;
+    int rc;
+    switch (c->h.runge_kutta_order) {
+    case 1:
+        a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        if ((rc = launch_one_stage(c, a))) return rc;
+        c->cur = f1;
+        break;
+    case 2:
+        a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        if ((rc = launch_one_stage(c, a))) return rc;
+        a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.5;
+        if ((rc = launch_one_stage(c, a))) return rc;
+        c->cur = f2;
+        break;
+    default:
+        a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        if ((rc = launch_one_stage(c, a))) return rc;
+        a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.75;
+        if ((rc = launch_one_stage(c, a))) return rc;
+        a.in_slot = f2; a.out_slot = f1; a.avg_slot = c->cur; a.bweight = 1.0 / 3.0;
+        if ((rc = launch_one_stage(c, a))) return rc;
+        c->cur = f1;
+        break;
+    }
+    return KNC_OK;
+}
+
+// Host time after one RK step, in the reference's arithmetic
+// (WeightedAverage on State::time, src/state.rs:204-208).
+double advance_time(int order, double t0, double dt)
+{
+    if (order == 1) return t0 + dt;
+    if (order == 2) {
+        const double t2 = (t0 + dt) + dt;
+        return t2 * (-0.5 + 1.0) + t0 * 0.5;
+    }
+    const double t1 = t0 + dt;
+    const double t2 = (t1 + dt) * (-0.75 + 1.0) + t0 * 0.75;
+    const double b3 = 1.0 / 3.0;
+    return (t2 + dt) * (-b3 + 1.0) + t0 * b3;
+}
+
+// Enqueues the CFL reduction into d_dt (adaptive) or sets the fixed dt.
+int enqueue_time_step(knc_context* c)
+{
+    if (c->h.system == KNC_SRHD && !c->h.adaptive_time_step) {
+        const HostBlock& b = c->blocks.begin()->second;
+        const double dt = c->h.cfl_number * (b.rv[1] - b.rv[0]) / c->h.light_speed;
+        hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->d_dt, dt);
+        HIP_TRY(hipGetLastError());
+        return KNC_OK;
+    }
+    StageArgs a = base_args(c);
+    a.in_slot = c->cur;
+    a.step = 0;
+    a.nitems_ptr = c->d_nitems;
+    const int grid = std::min(c->nitems, c->num_cu * 4);
+    HIP_TRY(launch_dt(c->h.system, a, c->d_min_bits, std::max(1, grid), c->stream));
+    HIP_TRY(launch_dt_finish(c->d_min_bits, c->h.cfl_number, c->d_dt, c->stream));
+    return KNC_OK;
+}
+
+// Reads the error log after a synchronised call; fills err and returns the code.
+int collect_error(knc_context* c, knc_error* err)
+{
+    unsigned long long key = 0;
+    HIP_TRY(hipMemcpy(&key, &c->d_err->min_key, sizeof key, hipMemcpyDeviceToHost));
+    if (err) std::memset(err, 0, sizeof *err);
+    if (key == ~0ull) return KNC_OK;
+    ErrLog log;
+    HIP_TRY(hipMemcpy(&log, c->d_err, sizeof log, hipMemcpyDeviceToHost));
+    const int order = (int)((key >> 32) & 0xffff), row = (int)((key >> 16) & 0xffff), col = (int)(key & 0xffff);
+    int code = KNC_HYDRO_ROOT_FINDER_FAILED;
+    knc_error e{};
+    e.i = row;
+    e.j = col;
+    int o = 0;
+    for (auto& kv : c->blocks) {
+        if (o++ == order) {
+            const HostBlock& b = kv.second;
+            e.block = b.index;
+            if (row < b.nr && col < c->nq) {
+                e.r = std::sqrt(b.rv[row] * b.rv[row + 1]);
+                e.theta = 0.5 * (c->qv[col] + c->qv[col + 1]);
+            }
+        }
+    }
+    const unsigned int n = std::min(log.count, (unsigned int)ERR_LOG_CAP);
+    for (unsigned int k = 0; k < n; ++k) {
+        if (log.entry[k].key == key) {
+            code = log.entry[k].code;
+            e.value = log.entry[k].value;
+            for (int q = 0; q < 4; ++q) e.conserved[q] = log.entry[k].cons[q];
+            break;
+        }
+    }
+    e.code = code;
+    if (err) *err = e;
+    return fail(code, "hydro error %d at block %d zone (%d,%d) (r, theta) = (%.4e, %.4f)", code, e.block, e.i, e.j,
+                e.r, e.theta);
+}
+
+int require_blocks(knc_context* c)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (c->blocks.empty()) return fail(KNC_ERR_NO_BLOCK, "no blocks");
+    return rebuild_tables(c);
+}
+
+// Runs `fold` RK steps after an optional dt pass; returns after the stream
+// drained and the error log was read.
+int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* dt_used, knc_error* err)
+{
+    int rc;
+    if ((rc = require_blocks(c))) return rc;
+    c->last_launches = 0;
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
+    HIP_TRY(hipGetLastError());
+    if (with_dt) {
+        if ((rc = enqueue_time_step(c))) return rc;
+    } else {
+        hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->d_dt, dt_fixed);
+        HIP_TRY(hipGetLastError());
+    }
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    const int start_slot = c->cur;
+    const bool graph = c->o.launch_mode == KNC_LAUNCH_BATCHED && c->o.use_graph && fold > 1;
+    if (graph) {
+        auto key = std::make_pair(start_slot, fold);
+        auto it = c->graphs.find(key);
+        if (it == c->graphs.end()) {
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int crc = KNC_OK;
+            for (int s = 0; s < fold && crc == KNC_OK; ++s) crc = enqueue_rk_step(c, s + 1);
+            hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            c->cur = start_slot;
+            if (crc) return crc;
+            HIP_TRY(ce);
+            hipGraphExec_t ex;
+            HIP_TRY(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+            HIP_TRY(hipGraphDestroy(g));
+            it = c->graphs.emplace(key, ex).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, c->stream));
+        // advance the slot bookkeeping as the captured steps did
+        for (int s = 0; s < fold; ++s) {
+            const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
+            c->cur = c->h.runge_kutta_order == 2 ? f2 : f1;
+        }
+        c->last_launches = fold * c->h.runge_kutta_order;
+    } else {
+        for (int s = 0; s < fold; ++s)
+            if ((rc = enqueue_rk_step(c, s + 1))) return rc;
+    }
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double dt = dt_fixed;
+    HIP_TRY(hipMemcpy(&dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost));
+    if (c->timing) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        const int nst = fold * c->h.runge_kutta_order;
+        c->last_stage_ms = nst ? ms / nst : 0.0;
+    }
+    if (dt_used) *dt_used = dt;
+    if ((rc = collect_error(c, err))) return rc;
+    for (int s = 0; s < fold; ++s) {
+        c->time = advance_time(c->h.runge_kutta_order, c->time, dt);
+        c->iteration += 1;
+    }
+    return KNC_OK;
+}
+
+// Temporary device buffers for the per-zone entry points.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+int zone_call_prep(const knc_hydro* h)
+{
+    int rc = check_hydro(h);
+    if (rc) return rc;
+    return KNC_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* knc_last_error_message(void) { return g_last_error.c_str(); }
+int knc_abi_version(void) { return KNC_ABI_VERSION; }
+
+int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_context** out)
+{
+    if (!out) return fail(KNC_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int rc = check_hydro(hydro);
+    if (rc) return rc;
+    knc_context* c = new knc_context();
+    c->h = *hydro;
+    if (options) c->o = *options;
+    else {
+        c->o.launch_mode = KNC_LAUNCH_PER_BLOCK;
+        c->o.use_graph = 0;
+    }
+    if (c->o.c2p_guess != KNC_GUESS_PLAIN) {
+        delete c;
+        return fail(KNC_ERR_UNSUPPORTED, "learned c2p guess requires trained weights (knc_mlp_*)");
+    }
+    c->zp = zone_params(c->h);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        delete c;
+        return fail(KNC_ERR_HIP, "no HIP device available");
+    }
+    if (c->o.device < 0 || c->o.device >= ndev) {
+        delete c;
+        return fail(KNC_ERR_INVALID_ARGUMENT, "device %d out of range", c->o.device);
+    }
+    auto cleanup = [&](int code) { knc_context_destroy(c); return code; };
+    if (hipSetDevice(c->o.device) != hipSuccess) return cleanup(fail(KNC_ERR_HIP, "hipSetDevice failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->o.device) != hipSuccess)
+        return cleanup(fail(KNC_ERR_HIP, "hipGetDeviceProperties failed"));
+    c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(KNC_ERR_HIP, "hipStreamCreate failed"));
+    if (hipMalloc(&c->d_err, sizeof(ErrLog)) != hipSuccess || hipMalloc(&c->d_stats, sizeof(C2PStats)) != hipSuccess ||
+        hipMalloc(&c->d_min_bits, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_dt, sizeof(double)) != hipSuccess || hipMalloc(&c->d_nitems, sizeof(int32_t)) != hipSuccess)
+        return cleanup(fail(KNC_ERR_HIP, "hipMalloc failed"));
+    (void)hipMemset(c->d_stats, 0, sizeof(C2PStats));
+    (void)hipMemset(c->d_dt, 0, sizeof(double));
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
+    if (launch_dt_reset(c->d_min_bits, c->stream) != hipSuccess) return cleanup(fail(KNC_ERR_HIP, "reset failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return cleanup(fail(KNC_ERR_HIP, "hipEventCreate failed"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(KNC_ERR_HIP, "init failed"));
+    *out = c;
+    return KNC_OK;
+}
+
+void knc_context_destroy(knc_context* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    for (auto& kv : c->blocks) {
+        (void)hipFree(kv.second.mem);
+        (void)hipFree(kv.second.rgeo);
+    }
+    void* ptrs[] = {c->d_theta, c->d_bnd, c->d_blocks, c->d_items, c->d_nitems, c->d_err, c->d_stats,
+                    c->d_min_bits, c->d_dt};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int knc_block_insert(knc_context* c, int32_t index, int32_t nr, int32_t nq, const double* rv, const double* qv,
+                     const double* conserved, const double* scalar_mass)
+{
+    if (!c || !rv || !qv || !conserved || !scalar_mass) return fail(KNC_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (nr < 2) return fail(KNC_ERR_INVALID_ARGUMENT, "must have at least 2 radial zones per block");
+    if (nr >= 65536) return fail(KNC_ERR_INVALID_ARGUMENT, "nr must be below 65536");
+    if (nq < 1 || nq > MAX_NQ) return fail(KNC_ERR_INVALID_ARGUMENT, "nq must be in [1, %d]", MAX_NQ);
+    if (c->blocks.count(index)) return fail(KNC_ERR_INVALID_ARGUMENT, "block %d already present", index);
+    if (c->blocks.size() >= 65535) return fail(KNC_ERR_INVALID_ARGUMENT, "too many blocks");
+    if (c->blocks.empty()) {
+        int rc = upload_theta(c, nq, qv);
+        if (rc) return rc;
+    } else if (nq != c->nq || std::memcmp(qv, c->qv.data(), sizeof(double) * (nq + 1)) != 0) {
+        return fail(KNC_ERR_INVALID_ARGUMENT, "every block must share the polar grid");
+    } else {
+        const int32_t lo = c->blocks.begin()->first, hi = c->blocks.rbegin()->first;
+        if (index != lo - 1 && index != hi + 1)
+            return fail(KNC_ERR_INVALID_ARGUMENT, "block %d is not adjacent to the chain [%d, %d]", index, lo, hi);
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HostBlock b;
+    b.index = index;
+    b.nr = nr;
+    b.nq = nq;
+    b.rv.assign(rv, rv + nr + 1);
+    const size_t n = (size_t)nr * nq;
+    HIP_TRY(hipMalloc(&b.mem, (size_t)NSLOT * 5 * n * sizeof(double)));
+    std::vector<double> rt = radial_table(nr, rv);
+    if (hipMalloc(&b.rgeo, rt.size() * sizeof(double)) != hipSuccess) {
+        (void)hipFree(b.mem);
+        return fail(KNC_ERR_HIP, "hipMalloc failed");
+    }
+    std::vector<double> soa(5 * n);
+    for (size_t k = 0; k < n; ++k) {
+        soa[k] = conserved[4 * k];
+        soa[n + k] = conserved[4 * k + 1];
+        soa[2 * n + k] = conserved[4 * k + 2];
+        soa[3 * n + k] = conserved[4 * k + 3];
+        soa[4 * n + k] = scalar_mass[k];
+    }
+    HIP_TRY(hipMemcpy(b.rgeo, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(b.mem, 0, (size_t)NSLOT * 5 * n * sizeof(double)));
+    HIP_TRY(hipMemcpy(b.mem + (size_t)c->cur * 5 * n, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->blocks.emplace(index, b);
+    c->dirty = true;
+    return KNC_OK;
+}
+
+int knc_block_remove(knc_context* c, int32_t index)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    auto it = c->blocks.find(index);
+    if (it == c->blocks.end()) return fail(KNC_ERR_NO_BLOCK, "no block %d", index);
+    if (it != c->blocks.begin() && std::next(it) != c->blocks.end())
+        return fail(KNC_ERR_INVALID_ARGUMENT, "only the innermost or outermost block can be removed");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(it->second.mem));
+    HIP_TRY(hipFree(it->second.rgeo));
+    c->blocks.erase(it);
+    c->dirty = true;
+    return KNC_OK;
+}
+
+int knc_block_download(knc_context* c, int32_t index, double* conserved, double* scalar_mass)
+{
+    if (!c || !conserved || !scalar_mass) return fail(KNC_ERR_INVALID_ARGUMENT, "NULL argument");
+    auto it = c->blocks.find(index);
+    if (it == c->blocks.end()) return fail(KNC_ERR_NO_BLOCK, "no block %d", index);
+    const HostBlock& b = it->second;
+    const size_t n = (size_t)b.nr * b.nq;
+    std::vector<double> soa(5 * n);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(soa.data(), b.mem + (size_t)c->cur * 5 * n, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) {
+        conserved[4 * k] = soa[k];
+        conserved[4 * k + 1] = soa[n + k];
+        conserved[4 * k + 2] = soa[2 * n + k];
+        conserved[4 * k + 3] = soa[3 * n + k];
+        scalar_mass[k] = soa[4 * n + k];
+    }
+    return KNC_OK;
+}
+
+int knc_block_count(knc_context* c, int32_t* count, int32_t* inner_index, int32_t* outer_index)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (count) *count = (int32_t)c->blocks.size();
+    if (!c->blocks.empty()) {
+        if (inner_index) *inner_index = c->blocks.begin()->first;
+        if (outer_index) *outer_index = c->blocks.rbegin()->first;
+    }
+    return KNC_OK;
+}
+
+int knc_set_boundary(knc_context* c, int32_t side, int32_t mode, const double* primitive, const double* scalar)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (side != KNC_SIDE_INNER && side != KNC_SIDE_OUTER) return fail(KNC_ERR_INVALID_ARGUMENT, "bad side");
+    if (mode != KNC_BOUNDARY_FIXED && mode != KNC_BOUNDARY_OUTFLOW) return fail(KNC_ERR_INVALID_ARGUMENT, "bad mode");
+    if (c->nq == 0) return fail(KNC_ERR_NO_BLOCK, "insert a block before setting boundaries");
+    if (mode == KNC_BOUNDARY_FIXED) {
+        if (!primitive || !scalar) return fail(KNC_ERR_INVALID_ARGUMENT, "fixed rows need primitive and scalar");
+        const int n2 = 2 * c->nq;
+        std::vector<double> soa((size_t)5 * n2);
+        for (int k = 0; k < n2; ++k) {
+            for (int q = 0; q < 4; ++q) soa[(size_t)q * n2 + k] = primitive[4 * k + q];
+            soa[(size_t)4 * n2 + k] = scalar[k];
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(c->d_bnd + (size_t)side * 5 * n2, soa.data(), soa.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+    }
+    if (side == KNC_SIDE_INNER) c->inner_mode = mode; else c->outer_mode = mode;
+    drop_graphs(c);
+    return KNC_OK;
+}
+
+int knc_set_time(knc_context* c, double time, int64_t iteration)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    c->time = time;
+    c->iteration = iteration;
+    return KNC_OK;
+}
+
+int knc_get_time(knc_context* c, double* time, int64_t* iteration)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (time) *time = c->time;
+    if (iteration) *iteration = c->iteration;
+    return KNC_OK;
+}
+
+int knc_time_step(knc_context* c, double* dt, knc_error* err)
+{
+    int rc;
+    if ((rc = require_blocks(c))) return rc;
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
+    HIP_TRY(hipGetLastError());
+    if ((rc = enqueue_time_step(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double v = 0.0;
+    HIP_TRY(hipMemcpy(&v, c->d_dt, sizeof v, hipMemcpyDeviceToHost));
+    if ((rc = collect_error(c, err))) return rc;
+    if (dt) *dt = v;
+    return KNC_OK;
+}
+
+int knc_advance_steps(knc_context* c, double dt, int32_t nsteps, knc_error* err)
+{
+    if (nsteps < 0) return fail(KNC_ERR_INVALID_ARGUMENT, "nsteps < 0");
+    if (nsteps == 0) return KNC_OK;
+    return run_steps(c, false, dt, nsteps, nullptr, err);
+}
+
+int knc_advance(knc_context* c, int32_t fold, double* dt_used, knc_error* err)
+{
+    if (fold < 1) return fail(KNC_ERR_INVALID_ARGUMENT, "fold must be >= 1");
+    return run_steps(c, true, 0.0, fold, dt_used, err);
+}
+
+int knc_get_c2p_stats(knc_context* c, knc_c2p_stats* out)
+{
+    if (!c || !out) return fail(KNC_ERR_INVALID_ARGUMENT, "NULL argument");
+    C2PStats s;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(&s, c->d_stats, sizeof s, hipMemcpyDeviceToHost));
+    for (int k = 0; k < HIST_BINS; ++k) out->hist[k] = s.hist[k];
+    uint64_t z = 0;
+    for (int k = 0; k < HIST_BINS; ++k) z += s.hist[k];
+    out->zones = z;
+    out->fallbacks = s.fallbacks;
+    out->floors = s.floors;
+    out->learned_iters = s.learned_iters;
+    return KNC_OK;
+}
+
+int knc_reset_c2p_stats(knc_context* c)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemset(c->d_stats, 0, sizeof(C2PStats)));
+    return KNC_OK;
+}
+
+int knc_synchronize(knc_context* c)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return KNC_OK;
+}
+
+int knc_set_timing(knc_context* c, int32_t enable)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    c->timing = enable != 0;
+    return KNC_OK;
+}
+
+int knc_last_stage_time(knc_context* c, double* ms, int32_t* launches)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (ms) *ms = c->last_stage_ms;
+    if (launches) *launches = c->last_launches;
+    return KNC_OK;
+}
+
+// ---- per-zone entry points -------------------------------------------------
+#define ZONE_ALLOC(buf, bytes) HIP_TRY(hipMalloc(&(buf).p, (bytes) ? (bytes) : 8))
+
+int knc_hydro_try_to_primitive(const knc_hydro* h, int64_t n, const double* cons, const double* guess, double* prim,
+                               int32_t* code, int32_t* iters)
+{
+    int rc = zone_call_prep(h);
+    if (rc) return rc;
+    if (n < 0 || (n && (!cons || !prim || !code))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return KNC_OK;
+    DevBuf dc, dg, dp, dcode, dit;
+    ZONE_ALLOC(dc, n * 32);
+    ZONE_ALLOC(dp, n * 32);
+    ZONE_ALLOC(dcode, n * 4);
+    if (guess) ZONE_ALLOC(dg, n * 8);
+    if (iters) ZONE_ALLOC(dit, n * 4);
+    HIP_TRY(hipMemcpy(dc.p, cons, n * 32, hipMemcpyHostToDevice));
+    if (guess) HIP_TRY(hipMemcpy(dg.p, guess, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(launch_try_to_primitive(h->system, zone_params(*h), n, (const double*)dc.p, (const double*)dg.p,
+                                    (double*)dp.p, (int32_t*)dcode.p, (int32_t*)dit.p, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(prim, dp.p, n * 32, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(code, dcode.p, n * 4, hipMemcpyDeviceToHost));
+    if (iters) HIP_TRY(hipMemcpy(iters, dit.p, n * 4, hipMemcpyDeviceToHost));
+    return KNC_OK;
+}
+
+int knc_hydro_to_conserved(const knc_hydro* h, int64_t n, const double* prim, double* cons)
+{
+    int rc = zone_call_prep(h);
+    if (rc) return rc;
+    if (n < 0 || (n && (!cons || !prim))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return KNC_OK;
+    DevBuf dp, dc;
+    ZONE_ALLOC(dp, n * 32);
+    ZONE_ALLOC(dc, n * 32);
+    HIP_TRY(hipMemcpy(dp.p, prim, n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(launch_to_conserved(h->system, zone_params(*h), n, (const double*)dp.p, (double*)dc.p, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(cons, dc.p, n * 32, hipMemcpyDeviceToHost));
+    return KNC_OK;
+}
+
+int knc_hydro_intercell_flux(const knc_hydro* h, int64_t n, const double* pl, const double* pr, const double* sl,
+                             const double* sr, int32_t direction, double* flux, double* sflux)
+{
+    int rc = zone_call_prep(h);
+    if (rc) return rc;
+    if (direction != KNC_DIRECTION_RADIAL && direction != KNC_DIRECTION_POLAR)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "bad direction");
+    if (n < 0 || (n && (!pl || !pr || !sl || !sr || !flux || !sflux))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return KNC_OK;
+    DevBuf a, b, c, d, f, g;
+    ZONE_ALLOC(a, n * 32);
+    ZONE_ALLOC(b, n * 32);
+    ZONE_ALLOC(c, n * 8);
+    ZONE_ALLOC(d, n * 8);
+    ZONE_ALLOC(f, n * 32);
+    ZONE_ALLOC(g, n * 8);
+    HIP_TRY(hipMemcpy(a.p, pl, n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.p, pr, n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c.p, sl, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.p, sr, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(launch_intercell_flux(h->system, h->riemann_solver, direction, zone_params(*h), n, (const double*)a.p,
+                                  (const double*)b.p, (const double*)c.p, (const double*)d.p, (double*)f.p,
+                                  (double*)g.p, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(flux, f.p, n * 32, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sflux, g.p, n * 8, hipMemcpyDeviceToHost));
+    return KNC_OK;
+}
+
+int knc_hydro_geometrical_source_terms(const knc_hydro* h, int64_t n, const double* prim, const double* r,
+                                       const double* theta, double* out)
+{
+    int rc = zone_call_prep(h);
+    if (rc) return rc;
+    if (n < 0 || (n && (!prim || !r || !theta || !out))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return KNC_OK;
+    std::vector<double> cot((size_t)n);
+    for (int64_t k = 0; k < n; ++k) cot[k] = std::cos(theta[k]) / std::sin(theta[k]);
+    DevBuf dp, dr, dc, dout;
+    ZONE_ALLOC(dp, n * 32);
+    ZONE_ALLOC(dr, n * 8);
+    ZONE_ALLOC(dc, n * 8);
+    ZONE_ALLOC(dout, n * 32);
+    HIP_TRY(hipMemcpy(dp.p, prim, n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dr.p, r, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc.p, cot.data(), n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(launch_sources(h->system, zone_params(*h), n, (const double*)dp.p, (const double*)dr.p,
+                           (const double*)dc.p, (double*)dout.p, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, dout.p, n * 32, hipMemcpyDeviceToHost));
+    return KNC_OK;
+}
+
+int knc_hydro_max_signal_speed(const knc_hydro* h, int64_t n, const double* prim, double* out)
+{
+    int rc = zone_call_prep(h);
+    if (rc) return rc;
+    if (n < 0 || (n && (!prim || !out))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return KNC_OK;
+    DevBuf dp, dout;
+    ZONE_ALLOC(dp, n * 32);
+    ZONE_ALLOC(dout, n * 8);
+    HIP_TRY(hipMemcpy(dp.p, prim, n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(launch_max_signal_speed(h->system, zone_params(*h), n, (const double*)dp.p, (double*)dout.p, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, dout.p, n * 8, hipMemcpyDeviceToHost));
+    return KNC_OK;
+}
+
+int knc_hydro_plm_gradient(double theta, int64_t n, const double* a, const double* b, const double* c, double* out)
+{
+    if (n < 0 || (n && (!a || !b || !c || !out))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return KNC_OK;
+    DevBuf da, db, dc, dout;
+    ZONE_ALLOC(da, n * 8);
+    ZONE_ALLOC(db, n * 8);
+    ZONE_ALLOC(dc, n * 8);
+    ZONE_ALLOC(dout, n * 8);
+    HIP_TRY(hipMemcpy(da.p, a, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(db.p, b, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc.p, c, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(launch_plm(theta, n, (const double*)da.p, (const double*)db.p, (const double*)dc.p, (double*)dout.p,
+                       nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, dout.p, n * 8, hipMemcpyDeviceToHost));
+    return KNC_OK;
+}
+
+int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
+                   double* mad)
+{
+    if (n < 0 || (n && (!a || !b || !c || !div || !sq || !mad))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return KNC_OK;
+    DevBuf da, db, dc, d1, d2, d3;
+    ZONE_ALLOC(da, n * 8);
+    ZONE_ALLOC(db, n * 8);
+    ZONE_ALLOC(dc, n * 8);
+    ZONE_ALLOC(d1, n * 8);
+    ZONE_ALLOC(d2, n * 8);
+    ZONE_ALLOC(d3, n * 8);
+    HIP_TRY(hipMemcpy(da.p, a, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(db.p, b, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc.p, c, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(launch_fp64_probe(n, (const double*)da.p, (const double*)db.p, (const double*)dc.p, (double*)d1.p,
+                              (double*)d2.p, (double*)d3.p, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(div, d1.p, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sq, d2.p, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(mad, d3.p, n * 8, hipMemcpyDeviceToHost));
+    return KNC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// knc_device.hpp -- data layout in HBM shared by the kernels and the host
+// side of the library.
+//
+// State of one radial block (nr x nq zones, theta fastest) is five FP64
+// planes, D | S_r | S_q | tau | M (volume-integrated conserved quantities and
+// scalar mass, the SoA form of BlockState, src/state.rs:26-30), each
+// nr*nq doubles, contiguous.  Each block owns three such state slots that the
+// RK stages rotate through.  Geometry is kept as 1-D factors: per-block radial
+// tables and one theta table shared by every block (the reference gives all
+// blocks the same polar grid, src/mesh.rs:358-379); the 2-D face areas and
+// volumes of GridGeometry (src/mesh.rs:21-29) are rebuilt from them in
+// registers with the same floating-point operations.
+#pragma once
+
+#include <stdint.h>
+
+namespace knc {
+
+constexpr int NSLOT = 3;
+constexpr int MAX_NQ = 1024;
+
+// Radial geometry table of one block, offsets into BlockDesc::rgeo.
+//   rv[nr+1]   vertices
+//   volA[nr]   ((2 pi)(r1^3 - r0^3))/3   so dV = volA[i]*dcost[j]
+//   rc[nr]     sqrt(r0 r1)               zone centre radius (mesh.rs:157-161)
+//   dr[nr]     r1 - r0
+struct BlockDesc {
+    double* slot[NSLOT];
+    const double* rgeo;
+    int32_t nr;
+    int32_t nq;
+    int32_t left;    // table index of the inner neighbour, -1 at the inner boundary
+    int32_t right;   // table index of the outer neighbour, -1 at the outer boundary
+    int32_t order;   // radial position in the chain (0 = innermost)
+    int32_t index;   // the caller's block index (BlockIndex.0)
+};
+
+__host__ __device__ inline const double* rg_rv(const BlockDesc& b) { return b.rgeo; }
+__host__ __device__ inline const double* rg_volA(const BlockDesc& b) { return b.rgeo + (b.nr + 1); }
+__host__ __device__ inline const double* rg_rc(const BlockDesc& b) { return b.rgeo + (b.nr + 1) + b.nr; }
+__host__ __device__ inline const double* rg_dr(const BlockDesc& b) { return b.rgeo + (b.nr + 1) + 2 * b.nr; }
+inline int64_t rgeo_size(int nr) { return (int64_t)(nr + 1) + 3 * (int64_t)nr; }
+
+// Polar table (shared): sinq[nq+1], cosq[nq+1] at the vertices, dcost[nq] =
+// -(cos q1 - cos q0), cot[nq] = cos(qc)/sin(qc) at qc = (q0+q1)/2, dq[nq].
+struct ThetaTable {
+    const double* sinq;
+    const double* cosq;
+    const double* dcost;
+    const double* cot;
+    const double* dq;
+    const double* qc;
+    int32_t nq;
+};
+
+enum { BND_FIXED = 0, BND_OUTFLOW = 1 };
+
+// Boundary rows: primitive + scalar, five planes of 2*nq values
+// (rho | u_r | u_q | p | s), rows in radial order (inner side: rows -2,-1;
+// outer side: rows nr, nr+1).
+struct BoundaryDesc {
+    int32_t inner_mode;
+    int32_t outer_mode;
+    const double* inner_rows;
+    const double* outer_rows;
+};
+
+// One unit of work of a stage launch: rows [i0, i1) of one block.
+struct WorkItem {
+    int32_t block;
+    int32_t i0;
+    int32_t i1;
+    int32_t pad;
+};
+
+// Error log of a launch sequence.  key = (step << 48) | (order << 32) |
+// (i << 16) | j; the smallest key is the reported error.
+constexpr int ERR_LOG_CAP = 256;
+struct ErrEntry {
+    unsigned long long key;
+    int32_t code;
+    int32_t block_index;
+    double value;
+    double cons[4];
+};
+struct ErrLog {
+    unsigned long long min_key;
+    unsigned int count;
+    unsigned int pad;
+    ErrEntry entry[ERR_LOG_CAP];
+};
+
+// Newton statistics: iteration histogram, fallbacks, floors.
+constexpr int HIST_BINS = 64;
+struct C2PStats {
+    unsigned long long hist[HIST_BINS];
+    unsigned long long zones;
+    unsigned long long fallbacks;
+    unsigned long long floors;
+    unsigned long long learned_iters;
+};
+
+}  // namespace knc

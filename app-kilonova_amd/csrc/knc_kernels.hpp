@@ -1,0 +1,55 @@
+// knc_kernels.hpp -- launch interface of the kernels in knc_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "knc_device.hpp"
+#include "knc_zone.hpp"
+
+namespace knc {
+
+// Arguments of one stage (or dt) launch.  Passed by value; everything it
+// points to is device memory.
+struct StageArgs {
+    const BlockDesc* blocks;
+    const WorkItem* items;
+    const int32_t* nitems_ptr;   // device-resident work count, or null
+    int32_t nitems;              // used when nitems_ptr is null
+    int32_t in_slot;
+    int32_t out_slot;
+    int32_t avg_slot;            // < 0: no RK average
+    double bweight;              // RK weight b: out = (U + dU)(1-b) + U0 b
+    const double* dt_dev;        // device-resident dt, or null
+    double dt;                   // used when dt_dev is null
+    ZoneParams zp;
+    ThetaTable th;
+    BoundaryDesc bnd;
+    ErrLog* err;
+    C2PStats* stats;
+    int32_t step;                // error-key step number (0 = dt pass)
+    int32_t pad;
+};
+
+hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, hipStream_t s);
+hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s);
+hipError_t launch_dt_finish(unsigned long long* min_bits, double cfl, double* dt_out, hipStream_t s);
+hipError_t launch_dt_reset(unsigned long long* min_bits, hipStream_t s);
+
+hipError_t launch_try_to_primitive(int system, const ZoneParams& zp, int64_t n, const double* cons,
+                                   const double* guess, double* prim, int32_t* code, int32_t* iters, hipStream_t s);
+hipError_t launch_to_conserved(int system, const ZoneParams& zp, int64_t n, const double* prim, double* cons,
+                               hipStream_t s);
+hipError_t launch_intercell_flux(int system, int rs, int axis, const ZoneParams& zp, int64_t n, const double* pl,
+                                 const double* pr, const double* sl, const double* sr, double* flux, double* sflux,
+                                 hipStream_t s);
+hipError_t launch_sources(int system, const ZoneParams& zp, int64_t n, const double* prim, const double* r,
+                          const double* cot, double* out, hipStream_t s);
+hipError_t launch_max_signal_speed(int system, const ZoneParams& zp, int64_t n, const double* prim, double* out,
+                                   hipStream_t s);
+hipError_t launch_plm(double th, int64_t n, const double* a, const double* b, const double* c, double* out,
+                      hipStream_t s);
+hipError_t launch_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
+                             double* mad, hipStream_t s);
+
+}  // namespace knc

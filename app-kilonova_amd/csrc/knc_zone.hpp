@@ -1,0 +1,351 @@
+// knc_zone.hpp -- per-zone FP64 arithmetic of the KNC hot path, for gfx950.
+//
+// Device counterparts of the per-zone methods of the reference's
+// `Hydrodynamics` trait (src/traits.rs:41-136) as implemented by
+// `RelativisticHydro` (src/physics/relativistic_hydro.rs:42-143) and
+// `NewtonianHydro` (src/physics/newtonian_hydro.rs:34-111).  The crates that
+// hold the reference arithmetic (hydro-srhd@25bf21ec, hydro-euler@fc929f9b,
+// godunov-core@d8dd524a) are not vendored; every formula below is written
+// from the published method it names.  Only IEEE-754 double + - * / sqrt and
+// comparisons are used, in a fixed order, and the library is compiled with
+// -ffp-contract=off, so a zone's result does not depend on tiling, launch
+// shape or batching.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace knc {
+
+enum { SYS_EULER = 0, SYS_SRHD = 1 };
+enum { RS_HLLE = 0, RS_HLLC = 1 };
+enum {
+    E_OK = 0,
+    E_NEGATIVE_DENSITY = 1,
+    E_NEGATIVE_PRESSURE = 2,
+    E_NEGATIVE_ENERGY_DENSITY = 3,
+    E_ROOT_FINDER_FAILED = 4,
+};
+
+constexpr int C2P_MAX_ITER = 50;
+constexpr double C2P_TOL = 1e-12;
+constexpr double KNC_PI = 3.14159265358979323846;
+
+// Parameters read by the per-zone functions.  gfac = gamma/(gamma-1) is
+// computed once on the host with the same expression the formulas use.
+struct ZoneParams {
+    double gamma;
+    double gm1;        // gamma - 1
+    double gfac;       // gamma / (gamma - 1)
+    double plm_theta;
+    double c;          // light speed scale (1 for Euler)
+};
+
+struct Prim { double rho, ur, uq, p; };   // SRHD u = Gamma*beta; Euler u = v
+struct Cons { double d, sr, sq, tau; };   // SRHD (D,S_r,S_q,tau); Euler (rho,m_r,m_q,E)
+struct PS { Prim P; double s; };          // primitive + scalar concentration rho*X
+
+__host__ __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
+
+// ---------------------------------------------------------------------------
+// c2p, SRHD: Newton-Raphson on p (Marti & Mueller, Living Rev. Relativ. 6
+// (2003) 7, sec. 2.3).  Q = tau + D + p, den = Q^2 - S^2:
+//   rho h = den/Q,  rho = D sqrt(den)/Q,  v^2 = S^2/Q^2,
+//   f = (gamma-1)(rho h - rho - p) - p,  f' = v^2 c_s^2 - 1,  c_s^2 = gamma p/(rho h).
+// Converged when |dp| <= 1e-12 (tau + D + p).  Error mapping follows
+// relativistic_hydro.rs:68-88.
+// ---------------------------------------------------------------------------
+__device__ inline int c2p_srhd_newton(double D, double S2, double tau, const ZoneParams& zp,
+                                      double p, double* p_out, int* iters)
+{
+    const double E = tau + D;
+    for (int n = 1; n <= C2P_MAX_ITER; ++n) {
+        const double Q = E + p;
+        const double den = Q * Q - S2;
+        if (!(den > 0.0)) { *iters = n; return E_ROOT_FINDER_FAILED; }
+        const double rQ = 1.0 / Q;
+        const double rhoh = den * rQ;
+        const double rho = D * (sqrt(den) * rQ);
+        const double v2 = S2 * (rQ * rQ);
+        const double f = zp.gm1 * (rhoh - rho - p) - p;
+        const double dp = f * rhoh / (zp.gamma * p * v2 - rhoh);
+        p = p - dp;
+        if (!(fabs(p) < 1e300)) { *iters = n; return E_ROOT_FINDER_FAILED; }
+        if (fabs(dp) <= C2P_TOL * (E + p)) { *iters = n; *p_out = p; return E_OK; }
+    }
+    *iters = C2P_MAX_ITER;
+    return E_ROOT_FINDER_FAILED;
+}
+
+// Primitive from a converged pressure.  Returns E_NEGATIVE_PRESSURE with the
+// 1e-3 rho floor applied (relativistic_hydro.rs:79-81).
+__device__ inline int c2p_srhd_finish(const Cons& u, double S2, double p, Prim* out)
+{
+    const double Q = (u.tau + u.d) + p;
+    const double den = Q * Q - S2;
+    const double sq = sqrt(den);
+    const double rQ = 1.0 / Q;
+    out->rho = u.d * (sq * rQ);
+    out->ur = u.sr / sq;
+    out->uq = u.sq / sq;
+    out->p = p;
+    if (p < 0.0) { out->p = 1e-3 * out->rho; return E_NEGATIVE_PRESSURE; }
+    return E_OK;
+}
+
+__device__ inline int c2p_euler(const Cons& u, const ZoneParams& zp, Prim* out)
+{
+    if (u.d < 0.0) return E_NEGATIVE_DENSITY;
+    const double rho = u.d;
+    const double vr = u.sr / rho;
+    const double vq = u.sq / rho;
+    const double ek = (0.5 * rho) * (vr * vr + vq * vq);
+    out->rho = rho;
+    out->ur = vr;
+    out->uq = vq;
+    out->p = (u.tau - ek) * zp.gm1;
+    return E_OK;
+}
+
+template <int SYS>
+__device__ inline double lorentz(const Prim& p)
+{
+    if (SYS == SYS_EULER) return 1.0;
+    return sqrt(1.0 + (p.ur * p.ur + p.uq * p.uq));
+}
+
+template <int SYS>
+__device__ inline Cons p2c(const Prim& p, const ZoneParams& zp)
+{
+    Cons u;
+    if (SYS == SYS_EULER) {
+        u.d = p.rho;
+        u.sr = p.rho * p.ur;
+        u.sq = p.rho * p.uq;
+        u.tau = p.p / zp.gm1 + (0.5 * p.rho) * (p.ur * p.ur + p.uq * p.uq);
+    } else {
+        const double W = sqrt(1.0 + (p.ur * p.ur + p.uq * p.uq));
+        const double rhoh = p.rho + p.p * zp.gfac;
+        const double hw = rhoh * W;
+        u.d = p.rho * W;
+        u.sr = hw * p.ur;
+        u.sq = hw * p.uq;
+        u.tau = (hw * W - p.p) - u.d;
+    }
+    return u;
+}
+
+// Conserved state, physical flux and outer wave speeds along one axis.
+struct FluxState {
+    Cons u, f;
+    double us, fs, vn, lm, lp, p;
+};
+
+template <int SYS, int AXIS>
+__device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams& zp)
+{
+    FluxState st;
+    st.u = p2c<SYS>(P, zp);
+    const double un = AXIS == 0 ? P.ur : P.uq;
+    if (SYS == SYS_EULER) {
+        st.vn = un;
+        st.us = s;
+        const double cs = sqrt(zp.gamma * P.p / P.rho);
+        st.lm = st.vn - cs;
+        st.lp = st.vn + cs;
+    } else {
+        const double uu = P.ur * P.ur + P.uq * P.uq;
+        const double W = sqrt(1.0 + uu);
+        const double rhoh = P.rho + P.p * zp.gfac;
+        st.vn = un / W;
+        st.us = s * W;
+        // Outer characteristic speeds, Marti & Mueller (2003) eq. (72).
+        const double a2 = zp.gamma * P.p / rhoh;
+        const double vv = uu / (1.0 + uu);
+        const double v2n = st.vn * st.vn;
+        const double k0 = sqrt(a2 * (1.0 - vv) * (1.0 - vv * a2 - v2n * (1.0 - a2)));
+        const double rd = 1.0 / (1.0 - vv * a2);
+        const double b = st.vn * (1.0 - a2);
+        st.lm = (b - k0) * rd;
+        st.lp = (b + k0) * rd;
+    }
+    st.p = P.p;
+    st.f.d = st.u.d * st.vn;
+    st.f.sr = st.u.sr * st.vn;
+    st.f.sq = st.u.sq * st.vn;
+    if (AXIS == 0) st.f.sr = st.f.sr + P.p; else st.f.sq = st.f.sq + P.p;
+    st.f.tau = (st.u.tau + P.p) * st.vn;
+    st.fs = st.us * st.vn;
+    return st;
+}
+
+template <int AXIS> __device__ inline double cnorm(const Cons& u) { return AXIS == 0 ? u.sr : u.sq; }
+template <int AXIS> __device__ inline double ctang(const Cons& u) { return AXIS == 0 ? u.sq : u.sr; }
+
+// HLLE / HLLC Godunov flux with passive scalar (intercell_flux,
+// relativistic_hydro.rs:123-134, newtonian_hydro.rs:96-102), unscaled.
+// HLLE: Harten-Lax-van Leer-Einfeldt.  HLLC SRHD: Mignone & Bodo, MNRAS 364
+// (2005) 126, eqs. (16)-(18).  HLLC Euler: Toro (2009) sec. 10.4.
+template <int SYS, int RS, int AXIS>
+__device__ inline void riemann(const PS& Lq, const PS& Rq, const ZoneParams& zp, Cons* F, double* Fs)
+{
+    const FluxState L = flux_state<SYS, AXIS>(Lq.P, Lq.s, zp);
+    const FluxState R = flux_state<SYS, AXIS>(Rq.P, Rq.s, zp);
+    const double am = dmin(L.lm, R.lm);
+    const double ap = dmax(L.lp, R.lp);
+
+    if (RS == RS_HLLE) {
+        const double a0 = dmin(am, 0.0);
+        const double a1 = dmax(ap, 0.0);
+        const double rd = 1.0 / (a1 - a0);
+        const double ab = a1 * a0;
+        F->d   = (L.f.d   * a1 - R.f.d   * a0 + (R.u.d   - L.u.d)   * ab) * rd;
+        F->sr  = (L.f.sr  * a1 - R.f.sr  * a0 + (R.u.sr  - L.u.sr)  * ab) * rd;
+        F->sq  = (L.f.sq  * a1 - R.f.sq  * a0 + (R.u.sq  - L.u.sq)  * ab) * rd;
+        F->tau = (L.f.tau * a1 - R.f.tau * a0 + (R.u.tau - L.u.tau) * ab) * rd;
+        *Fs    = (L.fs    * a1 - R.fs    * a0 + (R.us    - L.us)    * ab) * rd;
+        return;
+    }
+    if (am >= 0.0) { *F = L.f; *Fs = L.fs; return; }
+    if (ap <= 0.0) { *F = R.f; *Fs = R.fs; return; }
+
+    double lstar;
+    double pstar = 0.0;
+    if (SYS == SYS_SRHD) {
+        const double rd = 1.0 / (ap - am);
+        const double ab = ap * am;
+        const double ud = (R.u.d * ap - L.u.d * am + L.f.d - R.f.d) * rd;
+        const double ut = (R.u.tau * ap - L.u.tau * am + L.f.tau - R.f.tau) * rd;
+        const double um = (cnorm<AXIS>(R.u) * ap - cnorm<AXIS>(L.u) * am + cnorm<AXIS>(L.f) - cnorm<AXIS>(R.f)) * rd;
+        const double fd = (L.f.d * ap - R.f.d * am + (R.u.d - L.u.d) * ab) * rd;
+        const double ft = (L.f.tau * ap - R.f.tau * am + (R.u.tau - L.u.tau) * ab) * rd;
+        const double fm = (cnorm<AXIS>(L.f) * ap - cnorm<AXIS>(R.f) * am + (cnorm<AXIS>(R.u) - cnorm<AXIS>(L.u)) * ab) * rd;
+        const double Ehll = ut + ud;
+        const double FEhll = ft + fd;
+        const double b = Ehll + fm;
+        const double disc = b * b - 4.0 * FEhll * um;
+        lstar = 2.0 * um / (b + sqrt(disc));
+        pstar = fm - FEhll * lstar;
+    } else {
+        const double dl = L.u.d * (am - L.vn);
+        const double dr = R.u.d * (ap - R.vn);
+        lstar = (R.p - L.p + dl * L.vn - dr * R.vn) / (dl - dr);
+    }
+    const bool left = lstar >= 0.0;
+    const FluxState K = left ? L : R;
+    const double lam = left ? am : ap;
+    const double lv = lam - K.vn;
+    const double r = 1.0 / (lam - lstar);
+    Cons us;
+    double uss;
+    if (SYS == SYS_SRHD) {
+        const double fac = lv * r;
+        const double EK = K.u.tau + K.u.d;
+        us.d = K.u.d * fac;
+        const double snst = (cnorm<AXIS>(K.u) * lv + pstar - K.p) * r;
+        const double stst = ctang<AXIS>(K.u) * fac;
+        const double Est = (EK * lv + pstar * lstar - K.p * K.vn) * r;
+        if (AXIS == 0) { us.sr = snst; us.sq = stst; } else { us.sq = snst; us.sr = stst; }
+        us.tau = Est - us.d;
+        uss = K.us * fac;
+    } else {
+        const double fac = K.u.d * lv * r;
+        const double rhoK = K.u.d;
+        const double vt = ctang<AXIS>(K.u) / rhoK;
+        const double en = K.u.tau / rhoK + (lstar - K.vn) * (lstar + K.p / (rhoK * lv));
+        us.d = fac;
+        if (AXIS == 0) { us.sr = fac * lstar; us.sq = fac * vt; }
+        else           { us.sq = fac * lstar; us.sr = fac * vt; }
+        us.tau = fac * en;
+        uss = K.us * (lv * r);
+    }
+    F->d   = K.f.d   + lam * (us.d   - K.u.d);
+    F->sr  = K.f.sr  + lam * (us.sr  - K.u.sr);
+    F->sq  = K.f.sq  + lam * (us.sq  - K.u.sq);
+    F->tau = K.f.tau + lam * (us.tau - K.u.tau);
+    *Fs    = K.fs    + lam * (uss    - K.us);
+}
+
+// Spherical-polar geometric sources per unit volume at the zone centre
+// (geometrical_source_terms, relativistic_hydro.rs:136-138,
+// newtonian_hydro.rs:104-106); mass and energy sources are zero.
+template <int SYS>
+__device__ inline void sources(const Prim& P, double r, double cot, const ZoneParams& zp, double* sr, double* sq)
+{
+    const double rhoh = SYS == SYS_EULER ? P.rho : P.rho + P.p * zp.gfac;
+    *sr = (2.0 * P.p + rhoh * (P.uq * P.uq)) / r;
+    *sq = (P.p * cot - rhoh * (P.ur * P.uq)) / r;
+}
+
+// max_signal_speed (relativistic_hydro.rs:98-100 scales it by c; the scale is
+// applied by the caller).  SRHD: relativistic sum of |v| and c_s.
+template <int SYS>
+__device__ inline double max_signal_speed(const Prim& P, const ZoneParams& zp)
+{
+    if (SYS == SYS_EULER)
+        return sqrt(P.ur * P.ur + P.uq * P.uq) + sqrt(zp.gamma * P.p / P.rho);
+    const double uu = P.ur * P.ur + P.uq * P.uq;
+    const double rhoh = P.rho + P.p * zp.gfac;
+    const double cs = sqrt(zp.gamma * P.p / rhoh);
+    const double v = sqrt(uu / (1.0 + uu));
+    return (v + cs) / (1.0 + v * cs);
+}
+
+// Generalised minmod (godunov-core piecewise_linear::plm_gradient).
+__device__ inline double plm(double th, double a, double b, double c)
+{
+    const double x = (b - a) * th;
+    const double y = (c - a) * 0.5;
+    const double z = (c - b) * th;
+    if (x > 0.0 && y > 0.0 && z > 0.0) return dmin(x, dmin(y, z));
+    if (x < 0.0 && y < 0.0 && z < 0.0) return dmax(x, dmax(y, z));
+    return 0.0;
+}
+
+__device__ inline PS plm5(double th, const PS& a, const PS& b, const PS& c)
+{
+    PS g;
+    g.P.rho = plm(th, a.P.rho, b.P.rho, c.P.rho);
+    g.P.ur = plm(th, a.P.ur, b.P.ur, c.P.ur);
+    g.P.uq = plm(th, a.P.uq, b.P.uq, c.P.uq);
+    g.P.p = plm(th, a.P.p, b.P.p, c.P.p);
+    g.s = plm(th, a.s, b.s, c.s);
+    return g;
+}
+
+__device__ inline PS add_half(const PS& a, const PS& g)
+{
+    PS r;
+    r.P.rho = a.P.rho + g.P.rho * 0.5;
+    r.P.ur = a.P.ur + g.P.ur * 0.5;
+    r.P.uq = a.P.uq + g.P.uq * 0.5;
+    r.P.p = a.P.p + g.P.p * 0.5;
+    r.s = a.s + g.s * 0.5;
+    return r;
+}
+
+__device__ inline PS sub_half(const PS& a, const PS& g)
+{
+    PS r;
+    r.P.rho = a.P.rho - g.P.rho * 0.5;
+    r.P.ur = a.P.ur - g.P.ur * 0.5;
+    r.P.uq = a.P.uq - g.P.uq * 0.5;
+    r.P.p = a.P.p - g.P.p * 0.5;
+    r.s = a.s - g.s * 0.5;
+    return r;
+}
+
+// Face area of the frustum between (r0,q0) and (r1,q1) (mesh.rs:125-134).
+__device__ inline double face_area(double r0, double sq0, double cq0, double r1, double sq1, double cq1)
+{
+    const double s0 = r0 * sq0;
+    const double s1 = r1 * sq1;
+    const double z0 = r0 * cq0;
+    const double z1 = r1 * cq1;
+    const double ds = s1 - s0;
+    const double dz = z1 - z0;
+    return KNC_PI * (s0 + s1) * sqrt(ds * ds + dz * dz);
+}
+
+}  // namespace knc

@@ -1,0 +1,326 @@
+"""ctypes binding of ``libknc_gpu.so`` (the C ABI in ``include/knc_gpu.h``).
+
+The names mirror the reference's Rust interface: ``Hydro`` carries the fields
+of ``RelativisticHydro`` / ``NewtonianHydro``
+(``src/physics/relativistic_hydro.rs:13-36``, ``src/physics/newtonian_hydro.rs:13-28``),
+``Context.time_step`` is ``State::time_step`` (``src/state.rs:143-162``),
+``Context.advance`` is ``scheme::advance`` (``src/scheme.rs:212-240``) and the
+``try_to_primitive`` ... ``plm_gradient`` functions are the per-zone methods of
+``trait Hydrodynamics`` (``src/traits.rs:41-136``).  Hydro failures raise
+``HydroError`` with the reference's error kinds (``src/physics/mod.rs:82-117``).
+
+There is no CPU fallback: every call runs on the GPU through the HIP library
+and raises ``RuntimeError`` when the library is missing or fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libknc_gpu.so")
+
+LIGHT_SPEED = 3e10  # src/physics/mod.rs:7
+
+EULER, SRHD = 0, 1
+HLLE, HLLC = 0, 1
+RADIAL, POLAR = 0, 1
+FIXED, OUTFLOW = 0, 1
+INNER, OUTER = 0, 1
+PER_BLOCK, BATCHED = 0, 1
+GUESS_PLAIN, GUESS_LEARNED = 0, 1
+
+ERROR_KINDS = {1: "NegativeDensity", 2: "NegativePressure", 3: "NegativeEnergyDensity",
+               4: "RootFinderFailed"}
+
+
+class Hydro(ctypes.Structure):
+    _fields_ = [("system", ctypes.c_int32), ("riemann_solver", ctypes.c_int32),
+                ("runge_kutta_order", ctypes.c_int32), ("adaptive_time_step", ctypes.c_int32),
+                ("gamma_law_index", ctypes.c_double), ("plm_theta", ctypes.c_double),
+                ("cfl_number", ctypes.c_double), ("light_speed", ctypes.c_double)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("launch_mode", ctypes.c_int32), ("c2p_guess", ctypes.c_int32),
+                ("collect_stats", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+
+class Error(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("block", ctypes.c_int32), ("i", ctypes.c_int32),
+                ("j", ctypes.c_int32), ("r", ctypes.c_double), ("theta", ctypes.c_double),
+                ("value", ctypes.c_double), ("conserved", ctypes.c_double * 4)]
+
+
+class C2PStats(ctypes.Structure):
+    _fields_ = [("hist", ctypes.c_uint64 * 64), ("zones", ctypes.c_uint64), ("fallbacks", ctypes.c_uint64),
+                ("floors", ctypes.c_uint64), ("learned_iters", ctypes.c_uint64)]
+
+
+def relativistic_hydro(gamma_law_index=4.0 / 3.0, plm_theta=1.5, cfl_number=0.4, runge_kutta_order=2,
+                       riemann_solver=HLLC, adaptive_time_step=False):
+    """``RelativisticHydro`` (relativistic_hydro.rs:13-36)."""
+    return Hydro(SRHD, riemann_solver, runge_kutta_order, int(adaptive_time_step), gamma_law_index,
+                 plm_theta, cfl_number, LIGHT_SPEED)
+
+
+def newtonian_hydro(gamma_law_index=5.0 / 3.0, plm_theta=1.5, cfl_number=0.4, runge_kutta_order=2,
+                    riemann_solver=HLLE):
+    """``NewtonianHydro`` (newtonian_hydro.rs:13-28); HLLC is an extension."""
+    return Hydro(EULER, riemann_solver, runge_kutta_order, 1, gamma_law_index, plm_theta, cfl_number, 1.0)
+
+
+class HydroError(RuntimeError):
+    """``HydroError``: kind, block, zone (i, j) and (r, theta) of the failure."""
+
+    def __init__(self, code, err=None, message=""):
+        self.code = code
+        self.kind = ERROR_KINDS.get(code, f"error {code}")
+        if err is not None:
+            self.block, self.i, self.j = err.block, err.i, err.j
+            self.r, self.theta, self.value = err.r, err.theta, err.value
+            self.conserved = tuple(err.conserved)
+            message = (f"{self.kind}: {err.value:.4e} at position (r, theta) = ({err.r:.4e}, {err.theta:.4f}) "
+                       f"block {err.block} zone ({err.i}, {err.j})")
+        super().__init__(message)
+
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libknc_gpu.so not found at {LIB_PATH}: run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I32, I64, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    L.knc_last_error_message.restype = ctypes.c_char_p
+    L.knc_context_create.argtypes = [ctypes.POINTER(Hydro), ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.knc_context_destroy.argtypes = [P]
+    L.knc_context_destroy.restype = None
+    L.knc_block_insert.argtypes = [P, I32, I32, I32, P, P, P, P]
+    L.knc_block_remove.argtypes = [P, I32]
+    L.knc_block_download.argtypes = [P, I32, P, P]
+    L.knc_block_count.argtypes = [P, P, P, P]
+    L.knc_set_boundary.argtypes = [P, I32, I32, P, P]
+    L.knc_set_time.argtypes = [P, D, I64]
+    L.knc_get_time.argtypes = [P, P, P]
+    L.knc_time_step.argtypes = [P, P, ctypes.POINTER(Error)]
+    L.knc_advance_steps.argtypes = [P, D, I32, ctypes.POINTER(Error)]
+    L.knc_advance.argtypes = [P, I32, P, ctypes.POINTER(Error)]
+    L.knc_get_c2p_stats.argtypes = [P, ctypes.POINTER(C2PStats)]
+    L.knc_reset_c2p_stats.argtypes = [P]
+    L.knc_synchronize.argtypes = [P]
+    L.knc_set_timing.argtypes = [P, I32]
+    L.knc_last_stage_time.argtypes = [P, P, P]
+    L.knc_hydro_try_to_primitive.argtypes = [ctypes.POINTER(Hydro), I64, P, P, P, P, P]
+    L.knc_hydro_to_conserved.argtypes = [ctypes.POINTER(Hydro), I64, P, P]
+    L.knc_hydro_intercell_flux.argtypes = [ctypes.POINTER(Hydro), I64, P, P, P, P, I32, P, P]
+    L.knc_hydro_geometrical_source_terms.argtypes = [ctypes.POINTER(Hydro), I64, P, P, P, P]
+    L.knc_hydro_max_signal_speed.argtypes = [ctypes.POINTER(Hydro), I64, P, P]
+    L.knc_hydro_plm_gradient.argtypes = [D, I64, P, P, P, P]
+    L.knc_fp64_probe.argtypes = [I64, P, P, P, P, P, P]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f"knc error {rc}: {lib().knc_last_error_message().decode()}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """A KNC ``State`` resident on one GPU plus the scheme that advances it."""
+
+    def __init__(self, hydro, launch_mode=PER_BLOCK, c2p_guess=GUESS_PLAIN, collect_stats=False,
+                 rows_per_item=0, device=0, use_graph=True):
+        self.hydro = hydro
+        self.options = Options(launch_mode, c2p_guess, int(collect_stats), rows_per_item, device, int(use_graph))
+        h = ctypes.c_void_p()
+        _check(lib().knc_context_create(ctypes.byref(hydro), ctypes.byref(self.options), ctypes.byref(h)))
+        self._h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().knc_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- block set -----------------------------------------------------------
+    def insert_block(self, index, radial_vertices, polar_vertices, conserved, scalar_mass):
+        rv, qv = _f64(radial_vertices), _f64(polar_vertices)
+        nr, nq = rv.size - 1, qv.size - 1
+        cons = _f64(conserved).reshape(nr, nq, 4)
+        sm = _f64(scalar_mass).reshape(nr, nq)
+        _check(lib().knc_block_insert(self._h, int(index), nr, nq, _p(rv), _p(qv), _p(cons), _p(sm)))
+        self.shapes[int(index)] = (nr, nq)
+
+    def remove_block(self, index):
+        _check(lib().knc_block_remove(self._h, int(index)))
+        self.shapes.pop(int(index), None)
+
+    def download_block(self, index):
+        nr, nq = self.shapes[int(index)]
+        cons = np.zeros((nr, nq, 4))
+        sm = np.zeros((nr, nq))
+        _check(lib().knc_block_download(self._h, int(index), _p(cons), _p(sm)))
+        return cons, sm
+
+    def block_indexes(self):
+        return sorted(self.shapes)
+
+    def set_boundary(self, side, mode, primitive=None, scalar=None):
+        if mode == FIXED:
+            primitive = _f64(primitive)
+            scalar = _f64(scalar)
+        _check(lib().knc_set_boundary(self._h, side, mode, _p(primitive), _p(scalar)))
+
+    # -- time ----------------------------------------------------------------
+    def set_time(self, time, iteration=0):
+        _check(lib().knc_set_time(self._h, float(time), int(iteration)))
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        it = ctypes.c_int64()
+        _check(lib().knc_get_time(self._h, ctypes.byref(t), ctypes.byref(it)))
+        return t.value
+
+    @property
+    def iteration(self):
+        t = ctypes.c_double()
+        it = ctypes.c_int64()
+        _check(lib().knc_get_time(self._h, ctypes.byref(t), ctypes.byref(it)))
+        return it.value
+
+    def _raise(self, rc, err):
+        if rc in ERROR_KINDS:
+            raise HydroError(rc, err)
+        _check(rc)
+
+    def time_step(self):
+        dt = ctypes.c_double()
+        err = Error()
+        rc = lib().knc_time_step(self._h, ctypes.byref(dt), ctypes.byref(err))
+        if rc:
+            self._raise(rc, err)
+        return dt.value
+
+    def advance_steps(self, dt, nsteps):
+        err = Error()
+        rc = lib().knc_advance_steps(self._h, float(dt), int(nsteps), ctypes.byref(err))
+        if rc:
+            self._raise(rc, err)
+
+    def advance(self, fold):
+        dt = ctypes.c_double()
+        err = Error()
+        rc = lib().knc_advance(self._h, int(fold), ctypes.byref(dt), ctypes.byref(err))
+        if rc:
+            self._raise(rc, err)
+        return dt.value
+
+    def synchronize(self):
+        _check(lib().knc_synchronize(self._h))
+
+    def set_timing(self, enable=True):
+        _check(lib().knc_set_timing(self._h, int(enable)))
+
+    def last_stage_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int32()
+        _check(lib().knc_last_stage_time(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def c2p_stats(self):
+        s = C2PStats()
+        _check(lib().knc_get_c2p_stats(self._h, ctypes.byref(s)))
+        return dict(hist=np.array(s.hist[:], dtype=np.int64), zones=s.zones, fallbacks=s.fallbacks,
+                    floors=s.floors, learned_iters=s.learned_iters)
+
+    def reset_c2p_stats(self):
+        _check(lib().knc_reset_c2p_stats(self._h))
+
+
+# ---- per-zone Hydrodynamics methods ------------------------------------------
+def try_to_primitive(h, conserved, guess=None):
+    cons = _f64(conserved).reshape(-1, 4)
+    n = cons.shape[0]
+    prim = np.zeros((n, 4))
+    code = np.zeros(n, np.int32)
+    iters = np.zeros(n, np.int32)
+    g = None if guess is None else _f64(guess).reshape(-1)
+    _check(lib().knc_hydro_try_to_primitive(ctypes.byref(h), n, _p(cons), _p(g), _p(prim), _p(code), _p(iters)))
+    return prim, code, iters
+
+
+def to_conserved(h, primitive):
+    prim = _f64(primitive).reshape(-1, 4)
+    cons = np.zeros_like(prim)
+    _check(lib().knc_hydro_to_conserved(ctypes.byref(h), prim.shape[0], _p(prim), _p(cons)))
+    return cons
+
+
+def intercell_flux(h, pl, pr, sl, sr, direction):
+    pl = _f64(pl).reshape(-1, 4)
+    pr = _f64(pr).reshape(-1, 4)
+    n = pl.shape[0]
+    sl = _f64(np.broadcast_to(sl, (n,)))
+    sr = _f64(np.broadcast_to(sr, (n,)))
+    f = np.zeros((n, 4))
+    g = np.zeros(n)
+    _check(lib().knc_hydro_intercell_flux(ctypes.byref(h), n, _p(pl), _p(pr), _p(sl), _p(sr), int(direction),
+                                          _p(f), _p(g)))
+    return f, g
+
+
+def geometrical_source_terms(h, primitive, r, theta):
+    prim = _f64(primitive).reshape(-1, 4)
+    n = prim.shape[0]
+    r = _f64(np.broadcast_to(r, (n,)))
+    theta = _f64(np.broadcast_to(theta, (n,)))
+    out = np.zeros((n, 4))
+    _check(lib().knc_hydro_geometrical_source_terms(ctypes.byref(h), n, _p(prim), _p(r), _p(theta), _p(out)))
+    return out
+
+
+def max_signal_speed(h, primitive):
+    prim = _f64(primitive).reshape(-1, 4)
+    out = np.zeros(prim.shape[0])
+    _check(lib().knc_hydro_max_signal_speed(ctypes.byref(h), prim.shape[0], _p(prim), _p(out)))
+    return out
+
+
+def plm_gradient(plm_theta, a, b, c):
+    a, b, c = (_f64(x).reshape(-1) for x in (a, b, c))
+    out = np.zeros_like(a)
+    _check(lib().knc_hydro_plm_gradient(float(plm_theta), a.size, _p(a), _p(b), _p(c), _p(out)))
+    return out
+
+
+def fp64_probe(a, b, c):
+    a, b, c = (_f64(x).reshape(-1) for x in (a, b, c))
+    div, sq, mad = np.zeros_like(a), np.zeros_like(a), np.zeros_like(a)
+    _check(lib().knc_fp64_probe(a.size, _p(a), _p(b), _p(c), _p(div), _p(sq), _p(mad)))
+    return div, sq, mad

@@ -1,0 +1,190 @@
+/* knc_gpu.h -- C ABI of the MI355X (gfx950) implementation of the KNC
+ * per-block RK stage.
+ *
+ * Drop-in boundary for the hot path of clemson-cal/app-kilonova:
+ *   - the per-scheme seam  scheme::advance      (src/scheme.rs:212-240),
+ *                          State::time_step     (src/state.rs:143-162),
+ *                          the block set of State (src/state.rs:26-43) and
+ *                          add_remove_blocks    (src/scheme.rs:179-206);
+ *   - the per-zone seam    trait Hydrodynamics  (src/traits.rs:41-136),
+ *     implemented by RelativisticHydro (src/physics/relativistic_hydro.rs)
+ *     and NewtonianHydro (src/physics/newtonian_hydro.rs).
+ *
+ * Conventions: every function returns 0 on success or a KNC_ERR_* /
+ * KNC_HYDRO_* code, never unwinds and never aborts; knc_last_error_message()
+ * describes the last failure of the calling thread.  Host buffers belong to
+ * the caller and are copied in or out before the call returns; the context
+ * owns all device memory.  One context drives one HIP stream on one device
+ * and must not be used from two threads at once.
+ *
+ * Array layouts follow the reference's ndarray layouts: a block state is
+ * (nr, nq) row-major with theta fastest; conserved is an array of 4-double
+ * tuples (D, S_r, S_theta, tau) for SRHD or (rho, m_r, m_theta, E) for
+ * Euler; primitives are (rho, u_r, u_theta, p) with u = Gamma*beta for SRHD
+ * and u = v for Euler (hydro_srhd::srhd_2d::Primitive,
+ * src/physics/relativistic_hydro.rs:110-112).
+ */
+#ifndef KNC_GPU_H
+#define KNC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KNC_ABI_VERSION 1
+
+/* hydrodynamics system and options */
+enum { KNC_EULER = 0, KNC_SRHD = 1 };
+enum { KNC_HLLE = 0, KNC_HLLC = 1 };
+enum { KNC_DIRECTION_RADIAL = 0, KNC_DIRECTION_POLAR = 1 };
+enum { KNC_BOUNDARY_FIXED = 0, KNC_BOUNDARY_OUTFLOW = 1 };
+enum { KNC_SIDE_INNER = 0, KNC_SIDE_OUTER = 1 };
+enum { KNC_LAUNCH_PER_BLOCK = 0, KNC_LAUNCH_BATCHED = 1 };
+enum { KNC_GUESS_PLAIN = 0, KNC_GUESS_LEARNED = 1 };
+
+/* return codes: hydro errors mirror HydroErrorType (src/physics/mod.rs:82-95) */
+enum {
+    KNC_OK = 0,
+    KNC_HYDRO_NEGATIVE_DENSITY = 1,
+    KNC_HYDRO_NEGATIVE_PRESSURE = 2,
+    KNC_HYDRO_NEGATIVE_ENERGY_DENSITY = 3,
+    KNC_HYDRO_ROOT_FINDER_FAILED = 4,
+    KNC_ERR_INVALID_ARGUMENT = 10,
+    KNC_ERR_HIP = 11,
+    KNC_ERR_NO_BLOCK = 12,
+    KNC_ERR_UNSUPPORTED = 13
+};
+
+/* Hydrodynamics configuration: the fields of RelativisticHydro
+ * (src/physics/relativistic_hydro.rs:13-36) and NewtonianHydro
+ * (src/physics/newtonian_hydro.rs:13-28).  light_speed scales SRHD fluxes,
+ * sources and signal speeds (LIGHT_SPEED = 3e10, src/physics/mod.rs:7); it
+ * must be 1 for Euler. */
+typedef struct knc_hydro {
+    int32_t system;              /* KNC_EULER | KNC_SRHD */
+    int32_t riemann_solver;      /* KNC_HLLE | KNC_HLLC */
+    int32_t runge_kutta_order;   /* 1, 2 or 3 */
+    int32_t adaptive_time_step;  /* SRHD: 0 = cfl*dr0/c, 1 = CFL reduction; Euler: always reduction */
+    double gamma_law_index;
+    double plm_theta;            /* [1, 2] */
+    double cfl_number;           /* [0, 0.7] */
+    double light_speed;
+} knc_hydro;
+
+/* Execution options (no counterpart in the reference). */
+typedef struct knc_options {
+    int32_t launch_mode;     /* KNC_LAUNCH_PER_BLOCK: one stage launch per block;
+                                KNC_LAUNCH_BATCHED: one persistent launch per stage
+                                over every block, fold steps replayed as a hipGraph */
+    int32_t c2p_guess;       /* KNC_GUESS_PLAIN | KNC_GUESS_LEARNED */
+    int32_t collect_stats;   /* 1: Newton iteration histogram and fallback counts */
+    int32_t rows_per_item;   /* radial rows per workgroup work item; 0 = autotune */
+    int32_t device;          /* HIP device ordinal */
+    int32_t use_graph;       /* batched mode: 1 = capture fold steps in a hipGraph */
+} knc_options;
+
+/* HydroError (src/physics/mod.rs:109-117): code, the failing zone and its
+ * (r, theta) centre, the offending value, and U/dV at that zone. */
+typedef struct knc_error {
+    int32_t code;
+    int32_t block;        /* caller's block index (BlockIndex.0) */
+    int32_t i;
+    int32_t j;
+    double r;
+    double theta;
+    double value;
+    double conserved[4];
+} knc_error;
+
+/* Newton statistics of the SRHD c2p. */
+typedef struct knc_c2p_stats {
+    uint64_t hist[64];       /* zones by Newton iteration count */
+    uint64_t zones;
+    uint64_t fallbacks;      /* learned guesses that fell back to the plain guess */
+    uint64_t floors;         /* NegativePressure floors (p = 1e-3 rho) */
+    uint64_t learned_iters;  /* Newton iterations spent from learned guesses */
+} knc_c2p_stats;
+
+typedef struct knc_context knc_context;
+
+const char* knc_last_error_message(void);
+int knc_abi_version(void);
+
+/* ---- context and block set (State, src/state.rs:26-43) ----------------- */
+int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_context** out);
+void knc_context_destroy(knc_context* ctx);
+
+/* Inserts block `index` (BlockIndex.0; the chain must stay contiguous).
+ * radial_vertices: nr+1, polar_vertices: nq+1 (GridGeometry::radial_vertices
+ * / polar_vertices, src/mesh.rs:21-29; every block shares the polar grid).
+ * conserved: (nr, nq, 4) volume-integrated; scalar_mass: (nr, nq)
+ * (BlockState, src/state.rs:26-30).  Replaces add_remove_blocks' insert
+ * (src/scheme.rs:197-205). */
+int knc_block_insert(knc_context* ctx, int32_t index, int32_t nr, int32_t nq, const double* radial_vertices,
+                     const double* polar_vertices, const double* conserved, const double* scalar_mass);
+/* Removes block `index` (src/scheme.rs:191-194). */
+int knc_block_remove(knc_context* ctx, int32_t index);
+int knc_block_download(knc_context* ctx, int32_t index, double* conserved, double* scalar_mass);
+int knc_block_count(knc_context* ctx, int32_t* count, int32_t* inner_index, int32_t* outer_index);
+
+/* Ghost rows beyond the inner (side 0) or outer (side 1) block: FIXED takes
+ * primitive (2*nq*4) and scalar concentration rho*X (2*nq) rows in radial
+ * order; OUTFLOW repeats the edge row (zero gradient).  Stands in for the
+ * model-built ghost blocks of src/scheme.rs:46-52. */
+int knc_set_boundary(knc_context* ctx, int32_t side, int32_t mode, const double* primitive, const double* scalar);
+
+int knc_set_time(knc_context* ctx, double time, int64_t iteration);
+int knc_get_time(knc_context* ctx, double* time, int64_t* iteration);
+
+/* State::time_step (src/state.rs:143-162). */
+int knc_time_step(knc_context* ctx, double* dt, knc_error* err);
+
+/* nsteps RK steps (RungeKuttaOrder::try_advance_async with the stage of
+ * try_advance_rk, src/scheme.rs:16-173) with a fixed dt. */
+int knc_advance_steps(knc_context* ctx, double dt, int32_t nsteps, knc_error* err);
+
+/* scheme::advance without excision (src/scheme.rs:212-240): dt from
+ * time_step once, then `fold` RK steps.  dt_used may be NULL. */
+int knc_advance(knc_context* ctx, int32_t fold, double* dt_used, knc_error* err);
+
+int knc_get_c2p_stats(knc_context* ctx, knc_c2p_stats* out);
+int knc_reset_c2p_stats(knc_context* ctx);
+int knc_synchronize(knc_context* ctx);
+/* Average device time of the last advance call's stage launches (ms), from
+ * HIP events on the context's stream; 0 if not measured. */
+int knc_last_stage_time(knc_context* ctx, double* ms_per_stage, int32_t* launches);
+int knc_set_timing(knc_context* ctx, int32_t enable);
+
+/* ---- per-zone Hydrodynamics methods over host arrays ------------------- */
+/* try_to_primitive (src/traits.rs:77-82): code[k] is 0 or a KNC_HYDRO_*
+ * code; KNC_HYDRO_NEGATIVE_PRESSURE marks the 1e-3 rho floor and returns
+ * the floored primitive.  guess may be NULL (plain guess p = 0). */
+int knc_hydro_try_to_primitive(const knc_hydro* h, int64_t n, const double* conserved, const double* guess,
+                               double* primitive, int32_t* code, int32_t* iterations);
+/* to_conserved (src/traits.rs:90-93) */
+int knc_hydro_to_conserved(const knc_hydro* h, int64_t n, const double* primitive, double* conserved);
+/* intercell_flux (src/traits.rs:118-124), scaled by light_speed */
+int knc_hydro_intercell_flux(const knc_hydro* h, int64_t n, const double* pl, const double* pr, const double* sl,
+                             const double* sr, int32_t direction, double* flux, double* scalar_flux);
+/* geometrical_source_terms (src/traits.rs:126-130) at (r, theta) */
+int knc_hydro_geometrical_source_terms(const knc_hydro* h, int64_t n, const double* primitive, const double* r,
+                                       const double* theta, double* out);
+/* max_signal_speed (src/traits.rs:95-98) */
+int knc_hydro_max_signal_speed(const knc_hydro* h, int64_t n, const double* primitive, double* out);
+/* plm_gradient_scalar / the per-component rule of plm_gradient_primitive
+ * (src/traits.rs:66-75) */
+int knc_hydro_plm_gradient(double plm_theta, int64_t n, const double* a, const double* b, const double* c,
+                           double* out);
+
+/* FP64 arithmetic probe of the device (a/b, sqrt|a|, a*b+c with two
+ * roundings), used to check that device arithmetic rounds like the host's. */
+int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sqrt_abs_a,
+                   double* mul_add);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KNC_GPU_H */

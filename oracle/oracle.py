@@ -1,0 +1,262 @@
+"""ctypes wrapper of the CPU oracle (``oracle/knc_oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU-baseline leg import this module, and only as the checker
+or as the timed CPU baseline; the product path never calls it.
+
+The oracle restates, on the CPU, the reference's per-block RK stage
+(``src/scheme.rs:16-240``, ``src/state.rs:55-245``, ``src/mesh.rs``) and the
+per-zone arithmetic of the un-vendored crates ``hydro-srhd@25bf21ec``,
+``hydro-euler@fc929f9b`` and ``godunov-core@d8dd524a`` from the published
+methods named in the C++ source.  How each function is pinned:
+
+* p2c, c2p            -- closed-form round trips, v=0 special case, exact
+                         Euler closed form (tests/test_oracle_zone.py)
+* wave speeds         -- 1-D characteristic speeds of Marti & Mueller, the
+                         v=0 / p->0 limits and relativistic velocity addition
+* HLLE / HLLC         -- consistency F(U,U)=F(U), mirror symmetry, supersonic
+                         upwinding, exact stationary contact (HLLC), Sod and
+                         Marti-Mueller blast wave 1 plateaus against exact
+                         Riemann solutions (tests/test_oracle_riemann.py)
+* PLM                 -- brute force over sign patterns
+* geometry            -- telescoping volumes, closed-form polar face areas
+* stage / RK / dt     -- mass conservation, mirror symmetry, 1-D vs 2-D
+                         consistency, RK weights on a linear ODE, uniform
+                         state dt closed form (tests/test_oracle_stage.py)
+
+No reference test or golden vector pins the crate-internal choices (c2p
+tolerance, HLLC variant, scalar flux form): those are "parity unpinned"
+against the reference and documented as such.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libknc_oracle.so")
+
+LIGHT_SPEED = 3e10
+
+EULER, SRHD = 0, 1
+HLLE, HLLC = 0, 1
+FIXED, OUTFLOW = 0, 1
+CODES = {0: "ok", 1: "NegativeDensity", 2: "NegativePressure", 3: "NegativeEnergyDensity",
+         4: "RootFinderFailed"}
+
+
+class KoHydro(ctypes.Structure):
+    _fields_ = [("system", ctypes.c_int32), ("riemann_solver", ctypes.c_int32),
+                ("runge_kutta_order", ctypes.c_int32), ("adaptive_time_step", ctypes.c_int32),
+                ("gamma_law_index", ctypes.c_double), ("plm_theta", ctypes.c_double),
+                ("cfl_number", ctypes.c_double), ("light_speed", ctypes.c_double)]
+
+
+class KoError(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("block", ctypes.c_int32), ("i", ctypes.c_int32),
+                ("j", ctypes.c_int32), ("r", ctypes.c_double), ("theta", ctypes.c_double),
+                ("value", ctypes.c_double), ("conserved", ctypes.c_double * 4)]
+
+
+def hydro(system=SRHD, gamma=4.0 / 3.0, plm_theta=1.5, cfl=0.4, rk=2, riemann=HLLE, adaptive=True,
+          light_speed=None):
+    if light_speed is None:
+        light_speed = LIGHT_SPEED if system == SRHD else 1.0
+    return KoHydro(system, riemann, rk, int(adaptive), gamma, plm_theta, cfl, light_speed)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.ko_sim_create.restype = P
+        L.ko_sim_destroy.argtypes = [P]
+        for name in ("ko_sim_push_block", "ko_sim_pop_inner_block", "ko_sim_set_boundary",
+                     "ko_sim_get_block", "ko_sim_time_step", "ko_sim_advance", "ko_sim_scheme_advance",
+                     "ko_sim_num_blocks", "ko_sim_c2p_stats", "ko_sim_set_time"):
+            getattr(L, name).restype = ctypes.c_int
+        L.ko_sim_set_time.argtypes = [P, ctypes.c_double]
+        L.ko_sim_time.restype = ctypes.c_double
+        L.ko_sim_time.argtypes = [P]
+        L.ko_sim_iteration.restype = ctypes.c_longlong
+        L.ko_sim_iteration.argtypes = [P]
+        L.ko_sim_advance.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(KoError)]
+        L.ko_plm_gradient.argtypes = [ctypes.c_double, ctypes.c_int64, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+# ---- per-zone trait methods --------------------------------------------------
+def try_to_primitive(h, cons, guess=None):
+    cons = _f64(cons).reshape(-1, 4)
+    n = cons.shape[0]
+    prim = np.zeros((n, 4))
+    code = np.zeros(n, np.int32)
+    iters = np.zeros(n, np.int32)
+    g = _f64(guess).reshape(-1) if guess is not None else None
+    lib().ko_try_to_primitive(ctypes.byref(h), ctypes.c_int64(n), _p(cons), _p(g), _p(prim), _p(code), _p(iters))
+    return prim, code, iters
+
+
+def to_conserved(h, prim):
+    prim = _f64(prim).reshape(-1, 4)
+    cons = np.zeros_like(prim)
+    lib().ko_to_conserved(ctypes.byref(h), ctypes.c_int64(prim.shape[0]), _p(prim), _p(cons))
+    return cons
+
+
+def intercell_flux(h, pl, pr, sl, sr, axis):
+    pl = _f64(pl).reshape(-1, 4)
+    pr = _f64(pr).reshape(-1, 4)
+    n = pl.shape[0]
+    sl = _f64(np.broadcast_to(sl, (n,)))
+    sr = _f64(np.broadcast_to(sr, (n,)))
+    f = np.zeros((n, 4))
+    g = np.zeros(n)
+    lib().ko_intercell_flux(ctypes.byref(h), ctypes.c_int64(n), _p(pl), _p(pr), _p(sl), _p(sr),
+                            ctypes.c_int32(axis), _p(f), _p(g))
+    return f, g
+
+
+def wavespeeds(h, prim, axis):
+    prim = _f64(prim).reshape(-1, 4)
+    n = prim.shape[0]
+    lm = np.zeros(n)
+    lp = np.zeros(n)
+    lib().ko_wavespeeds(ctypes.byref(h), ctypes.c_int64(n), _p(prim), ctypes.c_int32(axis), _p(lm), _p(lp))
+    return lm, lp
+
+
+def geometrical_source_terms(h, prim, r, theta):
+    prim = _f64(prim).reshape(-1, 4)
+    n = prim.shape[0]
+    r = _f64(np.broadcast_to(r, (n,)))
+    theta = _f64(np.broadcast_to(theta, (n,)))
+    out = np.zeros((n, 4))
+    lib().ko_geometrical_source_terms(ctypes.byref(h), ctypes.c_int64(n), _p(prim), _p(r), _p(theta), _p(out))
+    return out
+
+
+def max_signal_speed(h, prim):
+    prim = _f64(prim).reshape(-1, 4)
+    out = np.zeros(prim.shape[0])
+    lib().ko_max_signal_speed(ctypes.byref(h), ctypes.c_int64(prim.shape[0]), _p(prim), _p(out))
+    return out
+
+
+def plm_gradient(theta, a, b, c):
+    a, b, c = (_f64(x).reshape(-1) for x in (a, b, c))
+    out = np.zeros_like(a)
+    lib().ko_plm_gradient(theta, a.size, _p(a), _p(b), _p(c), _p(out))
+    return out
+
+
+def geometry(rv, qv):
+    rv = _f64(rv)
+    qv = _f64(qv)
+    nr, nq = rv.size - 1, qv.size - 1
+    Ar = np.zeros((nr + 1, nq))
+    Aq = np.zeros((nr, nq + 1))
+    dV = np.zeros((nr, nq))
+    rc = np.zeros((nr, nq))
+    cot = np.zeros((nr, nq))
+    dl = np.zeros((nr, nq))
+    lib().ko_geometry(nr, nq, _p(rv), _p(qv), _p(Ar), _p(Aq), _p(dV), _p(rc), _p(cot), _p(dl))
+    return dict(Ar=Ar, Aq=Aq, dV=dV, rc=rc, cot=cot, dl=dl)
+
+
+# ---- whole-mesh simulation ---------------------------------------------------
+class OracleError(RuntimeError):
+    def __init__(self, err):
+        self.code = err.code
+        self.block, self.i, self.j = err.block, err.i, err.j
+        self.r, self.theta, self.value = err.r, err.theta, err.value
+        super().__init__(f"{CODES.get(err.code, err.code)} at block {err.block} zone ({err.i},{err.j}) "
+                         f"(r, theta) = ({err.r:.4e}, {err.theta:.4f}) value={err.value:.4e}")
+
+
+class Sim:
+    """A radially ordered chain of blocks advanced by the oracle."""
+
+    def __init__(self, h, nthreads=1):
+        self.h = h
+        self._p = ctypes.c_void_p(lib().ko_sim_create(ctypes.byref(h), int(nthreads)))
+        self.nq = None
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().ko_sim_destroy(self._p)
+            self._p = None
+
+    def push_block(self, rv, qv, cons, scalar_mass):
+        rv, qv = _f64(rv), _f64(qv)
+        nr, nq = rv.size - 1, qv.size - 1
+        cons = _f64(cons).reshape(nr, nq, 4)
+        sm = _f64(scalar_mass).reshape(nr, nq)
+        self.nq = nq
+        lib().ko_sim_push_block(self._p, nr, nq, _p(rv), _p(qv), _p(cons), _p(sm))
+
+    def pop_inner_block(self):
+        lib().ko_sim_pop_inner_block(self._p)
+
+    def num_blocks(self):
+        return lib().ko_sim_num_blocks(self._p)
+
+    def set_boundary(self, side, mode, prim=None, scalar=None):
+        nq = self.nq
+        if mode == FIXED:
+            prim = _f64(prim).reshape(2 * nq, 4)
+            scalar = _f64(scalar).reshape(2 * nq)
+        lib().ko_sim_set_boundary(self._p, side, mode, nq, _p(prim), _p(scalar))
+
+    def get_block(self, b, nr, nq):
+        cons = np.zeros((nr, nq, 4))
+        sm = np.zeros((nr, nq))
+        lib().ko_sim_get_block(self._p, b, _p(cons), _p(sm))
+        return cons, sm
+
+    @property
+    def time(self):
+        return lib().ko_sim_time(self._p)
+
+    @time.setter
+    def time(self, t):
+        lib().ko_sim_set_time(self._p, float(t))
+
+    @property
+    def iteration(self):
+        return lib().ko_sim_iteration(self._p)
+
+    def time_step(self):
+        dt = ctypes.c_double(0.0)
+        err = KoError()
+        if lib().ko_sim_time_step(self._p, ctypes.byref(dt), ctypes.byref(err)):
+            raise OracleError(err)
+        return dt.value
+
+    def advance(self, dt, nsteps):
+        err = KoError()
+        if lib().ko_sim_advance(self._p, float(dt), int(nsteps), ctypes.byref(err)):
+            raise OracleError(err)
+
+    def c2p_stats(self):
+        z, it, fl = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        lib().ko_sim_c2p_stats(self._p, ctypes.byref(z), ctypes.byref(it), ctypes.byref(fl))
+        return z.value, it.value, fl.value

@@ -1,0 +1,51 @@
+"""The C-ABI library loads and exports every entry point include/knc_gpu.h
+declares; host-side argument checks answer without touching a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import knc
+from knc import capi
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "knc_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(knc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("knc_context_create", "knc_block_insert", "knc_block_remove", "knc_time_step",
+                     "knc_advance", "knc_hydro_try_to_primitive", "knc_hydro_intercell_flux"):
+        assert required in names
+
+
+def test_library_exports_every_declared_function():
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_abi_version():
+    assert knc.lib().knc_abi_version() == 1
+
+
+def test_invalid_hydro_is_rejected_before_any_device_call():
+    h = knc.relativistic_hydro(plm_theta=2.5)
+    out = ctypes.c_void_p()
+    rc = knc.lib().knc_context_create(ctypes.byref(h), None, ctypes.byref(out))
+    assert rc == 10
+    assert b"plm_theta" in knc.lib().knc_last_error_message()
+    h = knc.relativistic_hydro(cfl_number=0.9)
+    assert knc.lib().knc_context_create(ctypes.byref(h), None, ctypes.byref(out)) == 10
+    assert out.value is None
+
+
+def test_context_errors_are_reported_not_raised_in_c():
+    with pytest.raises(RuntimeError):
+        knc.Context(knc.relativistic_hydro(runge_kutta_order=4))
