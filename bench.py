@@ -1,0 +1,177 @@
+"""Benchmark of the KNC per-block RK stage on one MI355X.
+
+Workload: the SRHD kinetic-bomb + jet mesh, 4096 x 1024 zones in 32 uneven
+radial blocks, gamma = 4/3, HLLE, RK2, PLM theta = 1.5, CFL 0.4 with the CFL
+reduction run once per fold of 10 steps.  A step is one full RK2 step
+(two stage launches) over every zone; the metric is FP64 zone-updates/s
+(zones x steps / wall seconds), the reference's Mzps x 1e6
+(src/main.rs:42-48).
+
+Paths timed on the same initial data:
+  plain    one stage launch per block per stage (KNC_LAUNCH_PER_BLOCK)
+  batched  one persistent launch per stage over every block, the fold's
+           steps replayed as a hipGraph (KNC_LAUNCH_BATCHED)
+The reported value is the batched path; the plain path's rate and the
+rel-L1 between the two final states are reported beside it.
+
+Roofline of the stage kernel: algorithmic HBM bytes per zone-update are
+40 B (5 FP64 fields) read + 40 B written on stage 1 and 80 B read + 40 B
+written on stage 2 (fused RK average), i.e. 100 B per zone per stage
+launch on average; achieved = 100 B x zones / average stage duration,
+measured with HIP events on the context's stream.
+
+CPU baseline: the C++ oracle (oracle/) on the host cores, a bounded sample
+of the same mesh (a few RK2 steps), reported as zone-updates/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "app-kilonova_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ZONE_STAGE = 100.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(setup, launch_mode, steps, warmup, fold):
+    import knc
+    ctx = knc.context_from_setup(setup, launch_mode=launch_mode, use_graph=True)
+    ctx.set_timing(True)
+    done = 0
+    while done < warmup:
+        f = min(fold, warmup - done)
+        ctx.advance(f)
+        done += f
+    ctx.synchronize()
+    stage_ms = []
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        f = min(fold, steps - done)
+        ctx.advance(f)
+        ms, _ = ctx.last_stage_time()
+        stage_ms.append((ms, f))
+        done += f
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    avg_stage_ms = sum(m * f for m, f in stage_ms) / sum(f for _, f in stage_ms)
+    return ctx, wall, avg_stage_ms
+
+
+def cpu_baseline(setup, nsteps, nthreads):
+    import oracle  # noqa: F401  (the checker / CPU baseline only)
+    from util import oracle_from_setup
+    sim = oracle_from_setup(setup, nthreads)
+    dt = sim.time_step()
+    sim.advance(dt, 1)  # warm-up step
+    t0 = time.perf_counter()
+    dt = sim.time_step()
+    sim.advance(dt, nsteps)
+    wall = time.perf_counter() - t0
+    return setup.total_zones * nsteps / wall, wall
+
+
+def read_traffic():
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary,
+    or None."""
+    path = os.path.join(ROOT, "profiles", "k_stage_bomb_jet_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_stage_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fold", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--nr", type=int, default=4096)
+    ap.add_argument("--nq", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X; multi-GPU sharding of the "
+              "radial block chain is described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    import knc
+    setup = knc.setups.bomb_jet(nr=args.nr, nq=args.nq)
+    zones = setup.total_zones
+    log(f"bench: bomb_jet {args.nr}x{args.nq} zones={zones} blocks={len(setup.blocks)} "
+        f"steps={args.steps} warmup={args.warmup} fold={args.fold}")
+
+    ctx_b, wall_b, stage_ms_b = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold)
+    rate_b = zones * args.steps / wall_b
+    log(f"bench: batched {rate_b / 1e9:.3f} Gzu/s, stage {stage_ms_b:.4f} ms")
+
+    plain = None
+    parity = None
+    if not args.no_plain:
+        ctx_p, wall_p, stage_ms_p = run_gpu(setup, knc.PER_BLOCK, args.steps, args.warmup, args.fold)
+        rate_p = zones * args.steps / wall_p
+        log(f"bench: plain {rate_p / 1e9:.3f} Gzu/s, stage {stage_ms_p:.4f} ms")
+        from util import gpu_state, rel_l1  # noqa: E402
+        parity = rel_l1(gpu_state(ctx_b), gpu_state(ctx_p))
+        plain = {"value": rate_p, "ms_per_step": 1e3 * wall_p / args.steps, "stage_ms": stage_ms_p}
+
+    achieved = BYTES_PER_ZONE_STAGE * zones / (stage_ms_b * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        ncores = min(16, os.cpu_count() or 1)
+        rate_c, wall_c = cpu_baseline(setup, args.cpu_steps, ncores)
+        cpu = {"value": rate_c, "unit": "zone-updates/s", "cores": ncores, "kind": "port",
+               "sample": f"oracle (C++ restatement), bomb_jet {args.nr}x{args.nq}, {args.cpu_steps} RK2 steps "
+                         f"after 1 warm-up step, {wall_c:.1f} s"}
+        log(f"bench: cpu {rate_c / 1e6:.3f} Mzu/s on {ncores} threads")
+
+    out = {
+        "metric": "FP64 zone-updates/s (RK2 step of the per-block stage)",
+        "value": rate_b,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_b / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": f"bomb_jet SRHD kinetic bomb + jet, {args.nr}x{args.nq} zones, "
+                               f"{len(setup.blocks)} uneven blocks, HLLE, RK2, fold {args.fold}",
+                   "path": "batched persistent stage launches, hipGraph per fold, plain c2p guess",
+                   "zones": zones},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
+        "cpu_baseline": cpu,
+        "plain_per_block": plain,
+        "batched_vs_plain_rel_l1": parity,
+        "stage_ms": stage_ms_b,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -147,6 +147,6 @@ def test_negative_density_is_reported_with_position():
     assert (e.value.block, e.value.i, e.value.j) == (1, 5, 7)
     sim = oracle_from_setup(s)
     with pytest.raises(oracle.OracleError) as eo:
-        sim.time_step()
+        sim.advance(sim.time_step(), 1)
     assert (eo.value.block, eo.value.i, eo.value.j) == (1, 5, 7)
     assert e.value.r == eo.value.r and e.value.theta == eo.value.theta
