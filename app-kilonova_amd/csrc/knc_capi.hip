@@ -80,20 +80,28 @@ ZoneParams zone_params(const knc_hydro& h)
     return zp;
 }
 
-// Radial table of one block, built with the operation order of the oracle /
-// the reference formulas (src/mesh.rs:119-123,157-161).
+// Radial table of one block (layout in knc_device.hpp), from the vertices
+// with the formulas of src/mesh.rs:119-134,157-161.
 std::vector<double> radial_table(int nr, const double* rv)
 {
     std::vector<double> t((size_t)rgeo_size(nr));
     double* v = t.data();
-    double* volA = v + (nr + 1);
-    double* rc = volA + nr;
-    double* dr = rc + nr;
-    for (int i = 0; i <= nr; ++i) v[i] = rv[i];
+    double* rr2 = v + (nr + 1);
+    double* volA = rr2 + (nr + 1);
+    double* rvolA = volA + nr;
+    double* rinv = rvolA + nr;
+    double* rb = rinv + nr;
+    double* dr = rb + nr;
+    for (int i = 0; i <= nr; ++i) {
+        v[i] = rv[i];
+        rr2[i] = rv[i] * rv[i];
+    }
     for (int i = 0; i < nr; ++i) {
         const double r0 = rv[i], r1 = rv[i + 1];
         volA[i] = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
-        rc[i] = std::sqrt(r0 * r1);
+        rvolA[i] = 1.0 / volA[i];
+        rinv[i] = 1.0 / std::sqrt(r0 * r1);
+        rb[i] = (KNC_PI * (r0 + r1)) * (r1 - r0);
         dr[i] = r1 - r0;
     }
     return t;
@@ -168,37 +176,41 @@ void drop_graphs(knc_context* c)
     c->graphs.clear();
 }
 
+// Polar table (layout in knc_device.hpp), with separate sin and cos calls.
 int upload_theta(knc_context* c, int nq, const double* qv)
 {
     c->nq = nq;
     c->qv.assign(qv, qv + nq + 1);
-    std::vector<double> t((size_t)6 * (nq + 1), 0.0);
-    double* sinq = t.data();
-    double* cosq = sinq + (nq + 1);
-    double* dcost = cosq + (nq + 1);
-    double* cot = dcost + (nq + 1);
-    double* dq = cot + (nq + 1);
-    double* qc = dq + (nq + 1);
+    const size_t m = (size_t)nq + 1;
+    std::vector<double> t(6 * m, 0.0);
+    double* dcost = t.data();
+    double* rdcost = dcost + m;
+    double* aq = rdcost + m;
+    double* cq = aq + m;
+    double* cot = cq + m;
+    double* dq = cot + m;
     for (int j = 0; j <= nq; ++j) {
-        sinq[j] = std::sin(qv[j]);
-        cosq[j] = std::cos(qv[j]);
-    }
-    for (int j = 0; j < nq; ++j) {
-        dcost[j] = -(std::cos(qv[j + 1]) - std::cos(qv[j]));
-        const double q = 0.5 * (qv[j] + qv[j + 1]);
-        qc[j] = q;
+        const double q0 = qv[j], q1 = j < nq ? qv[j + 1] : qv[j];
+        const double s0 = std::sin(q0), s1 = std::sin(q1);
+        const double c0 = std::cos(q0), c1 = std::cos(q1);
+        dcost[j] = -(c1 - c0);
+        rdcost[j] = 1.0 / dcost[j];
+        const double ds = s1 - s0, dc = c1 - c0;
+        aq[j] = (KNC_PI * (s0 + s1)) * std::sqrt(ds * ds + dc * dc);
+        cq[j] = s0 * std::sqrt(s0 * s0 + c0 * c0);
+        const double q = 0.5 * (q0 + q1);
         cot[j] = std::cos(q) / std::sin(q);
-        dq[j] = qv[j + 1] - qv[j];
+        dq[j] = q1 - q0;
     }
     if (c->d_theta) HIP_TRY(hipFree(c->d_theta));
     HIP_TRY(hipMalloc(&c->d_theta, t.size() * sizeof(double)));
     HIP_TRY(hipMemcpy(c->d_theta, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-    c->th.sinq = c->d_theta;
-    c->th.cosq = c->d_theta + (nq + 1);
-    c->th.dcost = c->d_theta + 2 * (nq + 1);
-    c->th.cot = c->d_theta + 3 * (nq + 1);
-    c->th.dq = c->d_theta + 4 * (nq + 1);
-    c->th.qc = c->d_theta + 5 * (nq + 1);
+    c->th.dcost = c->d_theta;
+    c->th.rdcost = c->d_theta + m;
+    c->th.aq = c->d_theta + 2 * m;
+    c->th.cq = c->d_theta + 3 * m;
+    c->th.cot = c->d_theta + 4 * m;
+    c->th.dq = c->d_theta + 5 * m;
     c->th.nq = nq;
     if (c->d_bnd) HIP_TRY(hipFree(c->d_bnd));
     HIP_TRY(hipMalloc(&c->d_bnd, (size_t)2 * 5 * 2 * nq * sizeof(double)));
@@ -899,15 +911,18 @@ int knc_hydro_geometrical_source_terms(const knc_hydro* h, int64_t n, const doub
     if (rc) return rc;
     if (n < 0 || (n && (!prim || !r || !theta || !out))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
     if (n == 0) return KNC_OK;
-    std::vector<double> cot((size_t)n);
-    for (int64_t k = 0; k < n; ++k) cot[k] = std::cos(theta[k]) / std::sin(theta[k]);
+    std::vector<double> cot((size_t)n), rinv((size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+        cot[k] = std::cos(theta[k]) / std::sin(theta[k]);
+        rinv[k] = 1.0 / r[k];
+    }
     DevBuf dp, dr, dc, dout;
     ZONE_ALLOC(dp, n * 32);
     ZONE_ALLOC(dr, n * 8);
     ZONE_ALLOC(dc, n * 8);
     ZONE_ALLOC(dout, n * 32);
     HIP_TRY(hipMemcpy(dp.p, prim, n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(dr.p, r, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dr.p, rinv.data(), n * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dc.p, cot.data(), n * 8, hipMemcpyHostToDevice));
     HIP_TRY(launch_sources(h->system, zone_params(*h), n, (const double*)dp.p, (const double*)dr.p,
                            (const double*)dc.p, (double*)dout.p, nullptr));

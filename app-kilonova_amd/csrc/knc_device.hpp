@@ -19,11 +19,16 @@ namespace knc {
 constexpr int NSLOT = 3;
 constexpr int MAX_NQ = 1024;
 
-// Radial geometry table of one block, offsets into BlockDesc::rgeo.
-//   rv[nr+1]   vertices
-//   volA[nr]   ((2 pi)(r1^3 - r0^3))/3   so dV = volA[i]*dcost[j]
-//   rc[nr]     sqrt(r0 r1)               zone centre radius (mesh.rs:157-161)
-//   dr[nr]     r1 - r0
+// Radial geometry table of one block (BlockDesc::rgeo), built on the host:
+//   rv[nr+1]    vertices
+//   rr2[nr+1]   rv^2                     radial face area = rr2[f] * aq[j]
+//   volA[nr]    ((2 pi)(r1^3 - r0^3))/3  dV = volA[i] * dcost[j]
+//   rvolA[nr]   1/volA                   1/dV ~ rvolA[i] * rdcost[j]
+//   rinv[nr]    1/sqrt(r0 r1)            1/r at the zone centre (mesh.rs:157-161)
+//   rb[nr]      (pi (r0 + r1))(r1 - r0)  polar face area = rb[i] * cq[j]
+//   dr[nr]      r1 - r0
+// The frustum area pi (s0+s1) sqrt(ds^2+dz^2) of mesh.rs:125-134 is used in
+// its exact factored forms (radial x polar).
 struct BlockDesc {
     double* slot[NSLOT];
     const double* rgeo;
@@ -35,21 +40,34 @@ struct BlockDesc {
     int32_t index;   // the caller's block index (BlockIndex.0)
 };
 
-__host__ __device__ inline const double* rg_rv(const BlockDesc& b) { return b.rgeo; }
-__host__ __device__ inline const double* rg_volA(const BlockDesc& b) { return b.rgeo + (b.nr + 1); }
-__host__ __device__ inline const double* rg_rc(const BlockDesc& b) { return b.rgeo + (b.nr + 1) + b.nr; }
-__host__ __device__ inline const double* rg_dr(const BlockDesc& b) { return b.rgeo + (b.nr + 1) + 2 * b.nr; }
-inline int64_t rgeo_size(int nr) { return (int64_t)(nr + 1) + 3 * (int64_t)nr; }
+// Slot pointer by a runtime index without indexing the array (an indexed
+// access would force the descriptor into scratch memory).
+__host__ __device__ inline double* slot_of(const BlockDesc& b, int s)
+{
+    return s == 0 ? b.slot[0] : (s == 1 ? b.slot[1] : b.slot[2]);
+}
 
-// Polar table (shared): sinq[nq+1], cosq[nq+1] at the vertices, dcost[nq] =
-// -(cos q1 - cos q0), cot[nq] = cos(qc)/sin(qc) at qc = (q0+q1)/2, dq[nq].
+__host__ __device__ inline const double* rg_rv(const BlockDesc& b) { return b.rgeo; }
+__host__ __device__ inline const double* rg_rr2(const BlockDesc& b) { return b.rgeo + (b.nr + 1); }
+__host__ __device__ inline const double* rg_volA(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1); }
+__host__ __device__ inline const double* rg_rvolA(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + b.nr; }
+__host__ __device__ inline const double* rg_rinv(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 2 * b.nr; }
+__host__ __device__ inline const double* rg_rb(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 3 * b.nr; }
+__host__ __device__ inline const double* rg_dr(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 4 * b.nr; }
+inline int64_t rgeo_size(int nr) { return 2 * (int64_t)(nr + 1) + 5 * (int64_t)nr; }
+
+// Polar table shared by every block, nq+1 entries per array:
+//   dcost = -(cos q1 - cos q0), rdcost = 1/dcost,
+//   aq = (pi (sin q0 + sin q1)) sqrt(dsin^2 + dcos^2),
+//   cq = sin q0 sqrt(sin^2 q0 + cos^2 q0)   (index nq: the last vertex),
+//   cot = cos(qc)/sin(qc) at qc = (q0+q1)/2, dq = q1 - q0.
 struct ThetaTable {
-    const double* sinq;
-    const double* cosq;
     const double* dcost;
+    const double* rdcost;
+    const double* aq;
+    const double* cq;
     const double* cot;
     const double* dq;
-    const double* qc;
     int32_t nq;
 };
 

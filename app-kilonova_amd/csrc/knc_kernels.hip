@@ -47,16 +47,17 @@ __device__ inline void record_error(ErrLog* log, int step, int order, int row, i
 
 // c2p of one zone (BlockState::try_to_primitive, src/state.rs:76-94, and the
 // scalar concentration s = M/dV/Gamma of src/scheme.rs:35).  u is the
-// volume-integrated state; dv the zone volume.
+// volume-integrated state; rdv the reciprocal volume factor.
 template <int SYS, bool STATS>
 __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, double usq, double utau,
-                                    double um, double dv, int order, int row, int col, int block_index,
+                                    double um, double rdv, int order, int row, int col, int block_index,
                                     unsigned int* lds_hist)
 {
-    const Cons u{ud / dv, usr / dv, usq / dv, utau / dv};
+    const Cons u{ud * rdv, usr * rdv, usq * rdv, utau * rdv};
     PS out;
     int code;
     double value = u.d;
+    double rW = 1.0;
     if (SYS == SYS_EULER) {
         code = c2p_euler(u, a.zp, &out.P);
     } else {
@@ -71,7 +72,7 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
             double p = 0.0;
             code = c2p_srhd_newton(u.d, S2, u.tau, a.zp, 0.0, &p, &it);
             if (code == E_OK) {
-                code = c2p_srhd_finish(u, S2, p, &out.P);
+                code = c2p_srhd_finish(u, S2, p, &out.P, &rW);
                 if (code == E_NEGATIVE_PRESSURE) {
                     if (STATS) atomicAdd(&a.stats->floors, 1ull);
                     code = E_OK;
@@ -88,7 +89,7 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
         out.s = 0.0;
         return out;
     }
-    out.s = um / dv / lorentz<SYS>(out.P);
+    out.s = (um * rdv) * rW;
     return out;
 }
 
@@ -115,9 +116,9 @@ __device__ inline PS fetch_row(const StageArgs& a, const BlockDesc& B, int k, in
             const BlockDesc N = a.blocks[B.left];
             const int row = N.nr + k;
             const int64_t np = (int64_t)N.nr * nq;
-            const double* s = N.slot[a.in_slot] + (int64_t)row * nq + j;
-            const double dv = rg_volA(N)[row] * a.th.dcost[j];
-            return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], dv, N.order, row, j,
+            const double* s = slot_of(N, a.in_slot) + (int64_t)row * nq + j;
+            const double rdv = rg_rvolA(N)[row] * a.th.rdcost[j];
+            return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, N.order, row, j,
                                               N.index, lds_hist);
         }
     } else if (k >= B.nr) {
@@ -135,16 +136,16 @@ __device__ inline PS fetch_row(const StageArgs& a, const BlockDesc& B, int k, in
             const BlockDesc N = a.blocks[B.right];
             const int row = k - B.nr;
             const int64_t np = (int64_t)N.nr * nq;
-            const double* s = N.slot[a.in_slot] + (int64_t)row * nq + j;
-            const double dv = rg_volA(N)[row] * a.th.dcost[j];
-            return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], dv, N.order, row, j,
+            const double* s = slot_of(N, a.in_slot) + (int64_t)row * nq + j;
+            const double rdv = rg_rvolA(N)[row] * a.th.rdcost[j];
+            return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, N.order, row, j,
                                               N.index, lds_hist);
         }
     }
     const int64_t np = (int64_t)B.nr * nq;
-    const double* s = B.slot[a.in_slot] + (int64_t)k * nq + j;
-    const double dv = rg_volA(B)[k] * a.th.dcost[j];
-    return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], dv, B.order, k, j, B.index,
+    const double* s = slot_of(B, a.in_slot) + (int64_t)k * nq + j;
+    const double rdv = rg_rvolA(B)[k] * a.th.rdcost[j];
+    return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, B.order, k, j, B.index,
                                       lds_hist);
 }
 
@@ -189,11 +190,11 @@ __global__ void __launch_bounds__(MAXT) k_stage(StageArgs a)
         const int nq = B.nq;
         const bool act = tid < nq;
         const int j = act ? tid : nq - 1;
-        const double sq0 = a.th.sinq[j], cq0 = a.th.cosq[j];
-        const double sq1 = a.th.sinq[j + 1], cq1 = a.th.cosq[j + 1];
+        const double aqj = a.th.aq[j];
+        const double cqj = a.th.cq[j];
         const double dcost = a.th.dcost[j];
         const double cotj = a.th.cot[j];
-        const double* rv = rg_rv(B);
+        const double* rr2 = rg_rr2(B);
         const int64_t np = (int64_t)B.nr * nq;
 
         PS Pm2, Pm1, gm2;
@@ -207,8 +208,7 @@ __global__ void __launch_bounds__(MAXT) k_stage(StageArgs a)
                     Cons F;
                     double Fs;
                     riemann<SYS, RS, 0>(add_half(Pm2, gm2), sub_half(Pm1, g), a.zp, &F, &Fs);
-                    const double rf = rv[k - 1];
-                    const double A = face_area(rf, sq0, cq0, rf, sq1, cq1);
+                    const double A = rr2[k - 1] * aqj;
                     const Flux5 Fk{F.d * c * A, F.sr * c * A, F.sq * c * A, F.tau * c * A, Fs * c * A};
                     if (k >= it.i0 + 2) {
                         // Zone row r = k-2: polar fluxes (scheme.rs:107-128) via LDS.
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(MAXT) k_stage(StageArgs a)
                             Cons G;
                             double Gs;
                             riemann<SYS, RS, 1>(lds_get(lds, NT, j - 1), QR, a.zp, &G, &Gs);
-                            const double Aq = face_area(rv[r], sq0, cq0, rv[r + 1], sq0, cq0);
+                            const double Aq = rg_rb(B)[r] * cqj;
                             Fyj = Flux5{G.d * c * Aq, G.sr * c * Aq, G.sq * c * Aq, G.tau * c * Aq, Gs * c * Aq};
                         }
                         __syncthreads();
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(MAXT) k_stage(StageArgs a)
                             // Sources x dV and the update (scheme.rs:130-158).
                             const double dv = rg_volA(B)[r] * dcost;
                             double ssr, ssq;
-                            sources<SYS>(P.P, rg_rc(B)[r], cotj, a.zp, &ssr, &ssq);
+                            sources<SYS>(P.P, rg_rinv(B)[r], cotj, a.zp, &ssr, &ssq);
                             const double sc0 = 0.0 * c * dv;
                             const double sc1 = ssr * c * dv;
                             const double sc2 = ssq * c * dv;
@@ -259,15 +259,15 @@ __global__ void __launch_bounds__(MAXT) k_stage(StageArgs a)
                             const double du3 = (sc3 - (Fk.tau - Fm.tau) - (Fyn.tau - Fyj.tau)) * dt;
                             const double ds = ((Fk.s - Fm.s) + (Fyn.s - Fyj.s)) * -dt;
                             const int64_t o = (int64_t)r * nq + j;
-                            const double* Ui = B.slot[a.in_slot] + o;
-                            double* Uo = B.slot[a.out_slot] + o;
+                            const double* Ui = slot_of(B, a.in_slot) + o;
+                            double* Uo = slot_of(B, a.out_slot) + o;
                             double v0 = Ui[0] + du0;
                             double v1 = Ui[np] + du1;
                             double v2 = Ui[2 * np] + du2;
                             double v3 = Ui[3 * np] + du3;
                             double v4 = Ui[4 * np] + ds;
                             if (a.avg_slot >= 0) {
-                                const double* U0 = B.slot[a.avg_slot] + o;
+                                const double* U0 = slot_of(B, a.avg_slot) + o;
                                 const double b = a.bweight;
                                 const double w1 = -b + 1.0;
                                 v0 = v0 * w1 + U0[0] * b;
@@ -325,9 +325,9 @@ __global__ void __launch_bounds__(256) k_dt(StageArgs a, unsigned long long* min
         for (int z = threadIdx.x; z < nz; z += blockDim.x) {
             const int i = it.i0 + z / nq;
             const int j = z % nq;
-            const double* s = B.slot[a.in_slot] + (int64_t)i * nq + j;
-            const double dv = rg_volA(B)[i] * a.th.dcost[j];
-            const PS q = zone_primitive<SYS, false>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], dv, B.order, i, j,
+            const double* s = slot_of(B, a.in_slot) + (int64_t)i * nq + j;
+            const double rdv = rg_rvolA(B)[i] * a.th.rdcost[j];
+            const PS q = zone_primitive<SYS, false>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, B.order, i, j,
                                                     B.index, nullptr);
             const double dr = rg_dr(B)[i];
             const double dl = dmin(dr, a.th.dq[j] * rg_rv(B)[i]);
@@ -380,9 +380,9 @@ __global__ void k_try_to_primitive(ZoneParams zp, int64_t n, const double* cons,
             rc = E_NEGATIVE_ENERGY_DENSITY;
         } else {
             const double S2 = u.sr * u.sr + u.sq * u.sq;
-            double p = 0.0;
+            double p = 0.0, rW = 1.0;
             rc = c2p_srhd_newton(u.d, S2, u.tau, zp, guess ? guess[k] : 0.0, &p, &it);
-            if (rc == E_OK) rc = c2p_srhd_finish(u, S2, p, &P);
+            if (rc == E_OK) rc = c2p_srhd_finish(u, S2, p, &P, &rW);
         }
         prim[4 * k] = P.rho; prim[4 * k + 1] = P.ur; prim[4 * k + 2] = P.uq; prim[4 * k + 3] = P.p;
         code[k] = rc;
@@ -419,11 +419,12 @@ __global__ void k_intercell_flux(ZoneParams zp, int64_t n, const double* pl, con
 }
 
 template <int SYS>
-__global__ void k_sources(ZoneParams zp, int64_t n, const double* prim, const double* r, const double* cot, double* out)
+__global__ void k_sources(ZoneParams zp, int64_t n, const double* prim, const double* rinv, const double* cot,
+                          double* out)
 {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         double a, b;
-        sources<SYS>(Prim{prim[4 * k], prim[4 * k + 1], prim[4 * k + 2], prim[4 * k + 3]}, r[k], cot[k], zp, &a, &b);
+        sources<SYS>(Prim{prim[4 * k], prim[4 * k + 1], prim[4 * k + 2], prim[4 * k + 3]}, rinv[k], cot[k], zp, &a, &b);
         const double c = zp.c;
         out[4 * k] = 0.0 * c; out[4 * k + 1] = a * c; out[4 * k + 2] = b * c; out[4 * k + 3] = 0.0 * c;
     }

@@ -53,6 +53,8 @@ __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b 
 // (2003) 7, sec. 2.3).  Q = tau + D + p, den = Q^2 - S^2:
 //   rho h = den/Q,  rho = D sqrt(den)/Q,  v^2 = S^2/Q^2,
 //   f = (gamma-1)(rho h - rho - p) - p,  f' = v^2 c_s^2 - 1,  c_s^2 = gamma p/(rho h).
+// With F = Q f = (gamma-1)(den - D sqrt(den) - pQ) - pQ the step is
+// dp = F den / (gamma p S^2 - den Q): one division per iteration.
 // Converged when |dp| <= 1e-12 (tau + D + p).  Error mapping follows
 // relativistic_hydro.rs:68-88.
 // ---------------------------------------------------------------------------
@@ -64,12 +66,10 @@ __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const Zon
         const double Q = E + p;
         const double den = Q * Q - S2;
         if (!(den > 0.0)) { *iters = n; return E_ROOT_FINDER_FAILED; }
-        const double rQ = 1.0 / Q;
-        const double rhoh = den * rQ;
-        const double rho = D * (sqrt(den) * rQ);
-        const double v2 = S2 * (rQ * rQ);
-        const double f = zp.gm1 * (rhoh - rho - p) - p;
-        const double dp = f * rhoh / (zp.gamma * p * v2 - rhoh);
+        const double sq = sqrt(den);
+        const double pQ = p * Q;
+        const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
+        const double dp = F * den / (zp.gamma * p * S2 - den * Q);
         p = p - dp;
         if (!(fabs(p) < 1e300)) { *iters = n; return E_ROOT_FINDER_FAILED; }
         if (fabs(dp) <= C2P_TOL * (E + p)) { *iters = n; *p_out = p; return E_OK; }
@@ -78,17 +78,20 @@ __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const Zon
     return E_ROOT_FINDER_FAILED;
 }
 
-// Primitive from a converged pressure.  Returns E_NEGATIVE_PRESSURE with the
+// Primitive from a converged pressure: rho = D sqrt(den)/Q, u = S/sqrt(den);
+// *rW receives 1/Gamma = sqrt(den)/Q.  Returns E_NEGATIVE_PRESSURE with the
 // 1e-3 rho floor applied (relativistic_hydro.rs:79-81).
-__device__ inline int c2p_srhd_finish(const Cons& u, double S2, double p, Prim* out)
+__device__ inline int c2p_srhd_finish(const Cons& u, double S2, double p, Prim* out, double* rW)
 {
     const double Q = (u.tau + u.d) + p;
     const double den = Q * Q - S2;
     const double sq = sqrt(den);
     const double rQ = 1.0 / Q;
-    out->rho = u.d * (sq * rQ);
-    out->ur = u.sr / sq;
-    out->uq = u.sq / sq;
+    const double rsq = 1.0 / sq;
+    *rW = sq * rQ;
+    out->rho = u.d * *rW;
+    out->ur = u.sr * rsq;
+    out->uq = u.sq * rsq;
     out->p = p;
     if (p < 0.0) { out->p = 1e-3 * out->rho; return E_NEGATIVE_PRESSURE; }
     return E_OK;
@@ -98,21 +101,15 @@ __device__ inline int c2p_euler(const Cons& u, const ZoneParams& zp, Prim* out)
 {
     if (u.d < 0.0) return E_NEGATIVE_DENSITY;
     const double rho = u.d;
-    const double vr = u.sr / rho;
-    const double vq = u.sq / rho;
+    const double rr = 1.0 / rho;
+    const double vr = u.sr * rr;
+    const double vq = u.sq * rr;
     const double ek = (0.5 * rho) * (vr * vr + vq * vq);
     out->rho = rho;
     out->ur = vr;
     out->uq = vq;
     out->p = (u.tau - ek) * zp.gm1;
     return E_OK;
-}
-
-template <int SYS>
-__device__ inline double lorentz(const Prim& p)
-{
-    if (SYS == SYS_EULER) return 1.0;
-    return sqrt(1.0 + (p.ur * p.ur + p.uq * p.uq));
 }
 
 template <int SYS>
@@ -158,13 +155,16 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
         const double uu = P.ur * P.ur + P.uq * P.uq;
         const double W = sqrt(1.0 + uu);
         const double rhoh = P.rho + P.p * zp.gfac;
-        st.vn = un / W;
+        const double rW = 1.0 / W;
+        const double rA = 1.0 / rhoh;
+        st.vn = un * rW;
         st.us = s * W;
-        // Outer characteristic speeds, Marti & Mueller (2003) eq. (72).
-        const double a2 = zp.gamma * P.p / rhoh;
-        const double vv = uu / (1.0 + uu);
+        // Outer characteristic speeds, Marti & Mueller (2003) eq. (72), with
+        // 1 - v^2 = 1/W^2.
+        const double a2 = zp.gamma * P.p * rA;
+        const double vv = uu * (rW * rW);
         const double v2n = st.vn * st.vn;
-        const double k0 = sqrt(a2 * (1.0 - vv) * (1.0 - vv * a2 - v2n * (1.0 - a2)));
+        const double k0 = sqrt(a2 * (rW * rW) * (1.0 - vv * a2 - v2n * (1.0 - a2)));
         const double rd = 1.0 / (1.0 - vv * a2);
         const double b = st.vn * (1.0 - a2);
         st.lm = (b - k0) * rd;
@@ -269,13 +269,13 @@ __device__ inline void riemann(const PS& Lq, const PS& Rq, const ZoneParams& zp,
 
 // Spherical-polar geometric sources per unit volume at the zone centre
 // (geometrical_source_terms, relativistic_hydro.rs:136-138,
-// newtonian_hydro.rs:104-106); mass and energy sources are zero.
+// newtonian_hydro.rs:104-106), given 1/r; mass and energy sources are zero.
 template <int SYS>
-__device__ inline void sources(const Prim& P, double r, double cot, const ZoneParams& zp, double* sr, double* sq)
+__device__ inline void sources(const Prim& P, double rinv, double cot, const ZoneParams& zp, double* sr, double* sq)
 {
     const double rhoh = SYS == SYS_EULER ? P.rho : P.rho + P.p * zp.gfac;
-    *sr = (2.0 * P.p + rhoh * (P.uq * P.uq)) / r;
-    *sq = (P.p * cot - rhoh * (P.ur * P.uq)) / r;
+    *sr = (2.0 * P.p + rhoh * (P.uq * P.uq)) * rinv;
+    *sq = (P.p * cot - rhoh * (P.ur * P.uq)) * rinv;
 }
 
 // max_signal_speed (relativistic_hydro.rs:98-100 scales it by c; the scale is
@@ -334,18 +334,6 @@ __device__ inline PS sub_half(const PS& a, const PS& g)
     r.P.p = a.P.p - g.P.p * 0.5;
     r.s = a.s - g.s * 0.5;
     return r;
-}
-
-// Face area of the frustum between (r0,q0) and (r1,q1) (mesh.rs:125-134).
-__device__ inline double face_area(double r0, double sq0, double cq0, double r1, double sq1, double cq1)
-{
-    const double s0 = r0 * sq0;
-    const double s1 = r1 * sq1;
-    const double z0 = r0 * cq0;
-    const double z1 = r1 * cq1;
-    const double ds = s1 - s0;
-    const double dz = z1 - z0;
-    return KNC_PI * (s0 + s1) * sqrt(ds * ds + dz * dz);
 }
 
 }  // namespace knc

@@ -71,7 +71,10 @@ struct Cons { double d, sr, sq, tau; };   // SRHD (D,S_r,S_q,tau); Euler (rho,m_
 // Relativ. 2003, sec. 2.3).  With Q = tau + D + p and den = Q^2 - S^2:
 //   rho h = den/Q,  rho = D sqrt(den)/Q,  v^2 = S^2/Q^2,
 //   f(p)  = (gamma-1)(rho h - rho - p) - p,   f'(p) = v^2 c_s^2 - 1,
-//   c_s^2 = gamma p/(rho h)   =>  p <- p - f rho h/(gamma p v^2 - rho h).
+//   c_s^2 = gamma p/(rho h).
+// Multiplying f by Q leaves one division per iteration:
+//   F = Q f = (gamma-1)(den - D sqrt(den) - pQ) - pQ,
+//   dp = f/f' = F den / (gamma p S^2 - den Q).
 // Converged when |dp| <= 1e-12 (tau + D + p): the rounding noise of f scales
 // with Q, so a test relative to p alone stalls for cold or Gamma~100 gas.
 // Error mapping follows relativistic_hydro.rs:68-88: D<0 and tau<0 are
@@ -86,12 +89,10 @@ int c2p_srhd_newton(double D, double S2, double tau, double gamma, double p,
         const double Q = E + p;
         const double den = Q * Q - S2;
         if (!(den > 0.0)) { *iters = n; return ROOT_FINDER_FAILED; }
-        const double rQ = 1.0 / Q;
-        const double rhoh = den * rQ;
-        const double rho = D * (std::sqrt(den) * rQ);
-        const double v2 = S2 * (rQ * rQ);
-        const double f = gm1 * (rhoh - rho - p) - p;
-        const double dp = f * rhoh / (gamma * p * v2 - rhoh);
+        const double sq = std::sqrt(den);
+        const double pQ = p * Q;
+        const double F = gm1 * (den - D * sq - pQ) - pQ;
+        const double dp = F * den / (gamma * p * S2 - den * Q);
         p = p - dp;
         if (!(std::fabs(p) < 1e300)) { *iters = n; return ROOT_FINDER_FAILED; }
         if (std::fabs(dp) <= C2P_TOL * (E + p)) {
@@ -102,7 +103,9 @@ int c2p_srhd_newton(double D, double S2, double tau, double gamma, double p,
     return ROOT_FINDER_FAILED;
 }
 
-int c2p_srhd(const Cons& u, double gamma, double guess, Prim* out, int* iters)
+// Primitive from the converged pressure: rho = D sqrt(den)/Q, u = S/sqrt(den)
+// (u = Gamma v with Gamma = Q/sqrt(den)); *rW receives 1/Gamma = sqrt(den)/Q.
+int c2p_srhd(const Cons& u, double gamma, double guess, Prim* out, int* iters, double* rW)
 {
     *iters = 0;
     if (u.d < 0.0) return NEGATIVE_DENSITY;
@@ -115,9 +118,11 @@ int c2p_srhd(const Cons& u, double gamma, double guess, Prim* out, int* iters)
     const double den = Q * Q - S2;
     const double sq = std::sqrt(den);
     const double rQ = 1.0 / Q;
-    out->rho = u.d * (sq * rQ);
-    out->ur = u.sr / sq;
-    out->uq = u.sq / sq;
+    const double rsq = 1.0 / sq;
+    *rW = sq * rQ;
+    out->rho = u.d * *rW;
+    out->ur = u.sr * rsq;
+    out->uq = u.sq * rsq;
     out->p = p;
     if (p < 0.0) { out->p = 1e-3 * out->rho; return NEGATIVE_PRESSURE; }
     return OK;
@@ -128,21 +133,15 @@ int c2p_euler(const Cons& u, double gamma, Prim* out)
 {
     if (u.d < 0.0) return NEGATIVE_DENSITY;
     const double rho = u.d;
-    const double vr = u.sr / rho;
-    const double vq = u.sq / rho;
+    const double rr = 1.0 / rho;
+    const double vr = u.sr * rr;
+    const double vq = u.sq * rr;
     const double ek = (0.5 * rho) * (vr * vr + vq * vq);
     out->rho = rho;
     out->ur = vr;
     out->uq = vq;
     out->p = (u.tau - ek) * (gamma - 1.0);
     return OK;
-}
-
-// Lorentz factor used for s = M/dV/Gamma (scheme.rs:35; traits Primitive).
-inline double lorentz(const Hydro& h, const Prim& p)
-{
-    if (h.system == SYS_EULER) return 1.0;
-    return std::sqrt(1.0 + (p.ur * p.ur + p.uq * p.uq));
 }
 
 // p2c (to_conserved), closed form.
@@ -190,13 +189,16 @@ FluxState flux_state(const Hydro& h, const Prim& P, double s, int axis)
         const double uu = P.ur * P.ur + P.uq * P.uq;
         const double W = std::sqrt(1.0 + uu);
         const double rhoh = P.rho + P.p * (g / (g - 1.0));
-        st.vn = un / W;
+        const double rW = 1.0 / W;
+        const double rA = 1.0 / rhoh;
+        st.vn = un * rW;
         st.us = s * W;
-        // Outer characteristic speeds, Marti & Mueller (2003) eq. (72).
-        const double a2 = g * P.p / rhoh;
-        const double vv = uu / (1.0 + uu);
+        // Outer characteristic speeds, Marti & Mueller (2003) eq. (72), with
+        // 1 - v^2 = 1/W^2.
+        const double a2 = g * P.p * rA;
+        const double vv = uu * (rW * rW);
         const double v2n = st.vn * st.vn;
-        const double k0 = std::sqrt(a2 * (1.0 - vv) * (1.0 - vv * a2 - v2n * (1.0 - a2)));
+        const double k0 = std::sqrt(a2 * (rW * rW) * (1.0 - vv * a2 - v2n * (1.0 - a2)));
         const double rd = 1.0 / (1.0 - vv * a2);
         const double b = st.vn * (1.0 - a2);
         st.lm = (b - k0) * rd;
@@ -301,17 +303,17 @@ void riemann(const Hydro& h, const Prim& PL, const Prim& PR, double sL, double s
 }
 
 // Spherical-polar geometric source terms per unit volume at the zone centre
-// (r, theta); mass and energy sources are zero.
+// (r, theta), given 1/r; mass and energy sources are zero.
 //   SRHD:  S_r = (2p + rho h u_q^2)/r,  S_q = (p cot - rho h u_r u_q)/r
 //   Euler: S_r = (2p + rho v_q^2)/r,    S_q = (p cot - rho v_r v_q)/r
-Cons sources(const Hydro& h, const Prim& P, double r, double cot)
+Cons sources(const Hydro& h, const Prim& P, double rinv, double cot)
 {
     const double g = h.gamma_law_index;
     const double rhoh = h.system == SYS_EULER ? P.rho : P.rho + P.p * (g / (g - 1.0));
     Cons s;
     s.d = 0.0;
-    s.sr = (2.0 * P.p + rhoh * (P.uq * P.uq)) / r;
-    s.sq = (P.p * cot - rhoh * (P.ur * P.uq)) / r;
+    s.sr = (2.0 * P.p + rhoh * (P.uq * P.uq)) * rinv;
+    s.sq = (P.p * cot - rhoh * (P.ur * P.uq)) * rinv;
     s.tau = 0.0;
     return s;
 }
@@ -367,31 +369,60 @@ inline PS sub_half(const PS& a, const PS& g)
 }
 
 // ---------------------------------------------------------------------------
-// Geometry of one block, exactly the 2-D arrays of GridGeometry
-// (mesh.rs:21-29) built with the formulas of mesh.rs:119-134,157-161.
+// Geometry of one block: the 2-D arrays of GridGeometry (mesh.rs:21-29) from
+// the formulas of mesh.rs:119-134,157-161.  The frustum area
+// pi (s0+s1) sqrt(ds^2+dz^2) (mesh.rs:125-134) is evaluated in its exact
+// factored forms, r^2 * pi (sin q0 + sin q1) sqrt((sin q1-sin q0)^2 +
+// (cos q1-cos q0)^2) for a radial face and pi (r0+r1)(r1-r0) *
+// sin q sqrt(sin^2 q + cos^2 q) for a polar face, and each zone also keeps
+// the reciprocal volume factors 1/volA and 1/dcost used for U/dV.
 // ---------------------------------------------------------------------------
-double cell_volume(double r0, double q0, double r1, double q1)
+struct RadialRow {            // per radial index, from the vertices
+    double volA;              // ((2 pi)(r1^3 - r0^3))/3
+    double rvolA;             // 1/volA
+    double rc, rinv;          // sqrt(r0 r1), 1/rc
+    double rb;                // (pi (r0 + r1)) (r1 - r0)
+};
+
+struct PolarCol {             // per polar index, from the vertices
+    double dcost, rdcost;     // -(cos q1 - cos q0), 1/dcost
+    double aq;                // (pi (sin q0 + sin q1)) sqrt(dsin^2 + dcos^2)
+    double cq;                // sin q0 sqrt(sin^2 q0 + cos^2 q0)
+    double cot;               // cos(qc)/sin(qc), qc = (q0+q1)/2
+};
+
+RadialRow radial_row(double r0, double r1)
 {
-    const double dcost = -(std::cos(q1) - std::cos(q0));
-    return 2.0 * PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0 * dcost;
+    RadialRow t;
+    t.volA = 2.0 * PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
+    t.rvolA = 1.0 / t.volA;
+    t.rc = std::sqrt(r0 * r1);
+    t.rinv = 1.0 / t.rc;
+    t.rb = (PI * (r0 + r1)) * (r1 - r0);
+    return t;
 }
 
-double face_area(double r0, double q0, double r1, double q1)
+PolarCol polar_col(double q0, double q1)
 {
-    const double s0 = r0 * std::sin(q0);
-    const double s1 = r1 * std::sin(q1);
-    const double z0 = r0 * std::cos(q0);
-    const double z1 = r1 * std::cos(q1);
-    const double ds = s1 - s0;
-    const double dz = z1 - z0;
-    return PI * (s0 + s1) * std::sqrt(ds * ds + dz * dz);
+    PolarCol t;
+    const double s0 = std::sin(q0), s1 = std::sin(q1);
+    const double c0 = std::cos(q0), c1 = std::cos(q1);
+    t.dcost = -(c1 - c0);
+    t.rdcost = 1.0 / t.dcost;
+    const double ds = s1 - s0, dc = c1 - c0;
+    t.aq = (PI * (s0 + s1)) * std::sqrt(ds * ds + dc * dc);
+    t.cq = s0 * std::sqrt(s0 * s0 + c0 * c0);
+    const double qc = 0.5 * (q0 + q1);
+    t.cot = std::cos(qc) / std::sin(qc);
+    return t;
 }
 
 struct Block {
     int nr = 0, nq = 0;
     std::vector<double> rv, qv;      // vertices (nr+1), (nq+1)
     std::vector<double> Ar, Aq, dV;  // (nr+1,nq), (nr,nq+1), (nr,nq)
-    std::vector<double> rc, cot, dl; // centre radius / cot(centre theta) / linear dim per zone
+    std::vector<double> rdV, rinv, cot, dl; // 1/dV factors, 1/rc, cot(qc), linear dim per zone
+    std::vector<double> rc;          // zone centre radius (error positions)
     std::vector<Cons> U;             // conserved * dV, (nr,nq) row-major, theta fastest
     std::vector<double> M;           // scalar mass
 };
@@ -402,22 +433,30 @@ void build_geometry(Block& b)
     b.Ar.assign((size_t)(nr + 1) * nq, 0.0);
     b.Aq.assign((size_t)nr * (nq + 1), 0.0);
     b.dV.assign((size_t)nr * nq, 0.0);
+    b.rdV.assign((size_t)nr * nq, 0.0);
+    b.rinv.assign((size_t)nr * nq, 0.0);
     b.rc.assign((size_t)nr * nq, 0.0);
     b.cot.assign((size_t)nr * nq, 0.0);
     b.dl.assign((size_t)nr * nq, 0.0);
+    std::vector<RadialRow> R(nr);
+    std::vector<PolarCol> C(nq + 1);
+    for (int i = 0; i < nr; ++i) R[i] = radial_row(b.rv[i], b.rv[i + 1]);
+    for (int j = 0; j < nq; ++j) C[j] = polar_col(b.qv[j], b.qv[j + 1]);
+    C[nq] = polar_col(b.qv[nq], b.qv[nq]);
     for (int i = 0; i <= nr; ++i)
-        for (int j = 0; j < nq; ++j)   // zone(i,j).face_area_r(): inner radius, theta0..theta1
-            b.Ar[(size_t)i * nq + j] = face_area(b.rv[i], b.qv[j], b.rv[i], b.qv[j + 1]);
+        for (int j = 0; j < nq; ++j)   // zone(i,j).face_area_r()
+            b.Ar[(size_t)i * nq + j] = (b.rv[i] * b.rv[i]) * C[j].aq;
     for (int i = 0; i < nr; ++i)
-        for (int j = 0; j <= nq; ++j)  // zone(i,j).face_area_q(): r0..r1 at lower theta
-            b.Aq[(size_t)i * (nq + 1) + j] = face_area(b.rv[i], b.qv[j], b.rv[i + 1], b.qv[j]);
+        for (int j = 0; j <= nq; ++j)  // zone(i,j).face_area_q()
+            b.Aq[(size_t)i * (nq + 1) + j] = R[i].rb * C[j].cq;
     for (int i = 0; i < nr; ++i)
         for (int j = 0; j < nq; ++j) {
             const size_t k = (size_t)i * nq + j;
-            b.dV[k] = cell_volume(b.rv[i], b.qv[j], b.rv[i + 1], b.qv[j + 1]);
-            b.rc[k] = std::sqrt(b.rv[i] * b.rv[i + 1]);
-            const double qc = 0.5 * (b.qv[j] + b.qv[j + 1]);
-            b.cot[k] = std::cos(qc) / std::sin(qc);
+            b.dV[k] = R[i].volA * C[j].dcost;
+            b.rdV[k] = R[i].rvolA * C[j].rdcost;
+            b.rc[k] = R[i].rc;
+            b.rinv[k] = R[i].rinv;
+            b.cot[k] = C[j].cot;
             const double dr = b.rv[i + 1] - b.rv[i];
             const double dq = b.qv[j + 1] - b.qv[j];
             b.dl[k] = std::min(dr, dq * b.rv[i]);
@@ -486,15 +525,16 @@ ErrRec block_primitives(Sim& sim, int bi, const std::vector<Cons>& U, const std:
     ErrRec err;
     long long iters_sum = 0, floors = 0;
     for (size_t k = 0; k < n; ++k) {
-        const double dv = b.dV[k];
-        const Cons u{U[k].d / dv, U[k].sr / dv, U[k].sq / dv, U[k].tau / dv};
+        const double rdv = b.rdV[k];
+        const Cons u{U[k].d * rdv, U[k].sr * rdv, U[k].sq * rdv, U[k].tau * rdv};
         int rc, it = 0;
         double value = 0.0;
+        double rW = 1.0;
         if (sim.h.system == SYS_EULER) {
             rc = c2p_euler(u, sim.h.gamma_law_index, &out[k].P);
             value = u.d;
         } else {
-            rc = c2p_srhd(u, sim.h.gamma_law_index, 0.0, &out[k].P, &it);
+            rc = c2p_srhd(u, sim.h.gamma_law_index, 0.0, &out[k].P, &it, &rW);
             value = rc == NEGATIVE_DENSITY ? u.d : u.tau;
         }
         iters_sum += it;
@@ -507,7 +547,8 @@ ErrRec block_primitives(Sim& sim, int bi, const std::vector<Cons>& U, const std:
             out[k] = PS{{0, 0, 0, 0}, 0};
             continue;
         }
-        out[k].s = M[k] / dv / lorentz(sim.h, out[k].P);
+        // s = M/dV/Gamma (scheme.rs:35) with 1/dV and 1/Gamma from c2p
+        out[k].s = (M[k] * rdv) * rW;
     }
     sim.stats.zones += (long long)n;
     sim.stats.iterations += iters_sum;
@@ -612,7 +653,7 @@ ErrRec stage(Sim& sim, const std::vector<std::vector<Cons>>& Uin, const std::vec
             for (int j = 0; j < nq; ++j) {
                 const size_t z = (size_t)i * nq + j;
                 const Prim& P = prim[b][z].P;
-                const Cons S = sources(h, P, B.rc[z], B.cot[z]);
+                const Cons S = sources(h, P, B.rinv[z], B.cot[z]);
                 const double dv = B.dV[z];
                 const Cons sc{S.d * c * dv, S.sr * c * dv, S.sq * c * dv, S.tau * c * dv};
                 const Cons& a = Fx[(size_t)(i - i0) * nq + j];
@@ -756,8 +797,9 @@ int ko_try_to_primitive(const Hydro* h, int64_t n, const double* cons, const dou
         const Cons u{cons[4 * k], cons[4 * k + 1], cons[4 * k + 2], cons[4 * k + 3]};
         Prim P{0, 0, 0, 0};
         int rc, it = 0;
+        double rW = 1.0;
         if (h->system == SYS_EULER) rc = c2p_euler(u, h->gamma_law_index, &P);
-        else rc = c2p_srhd(u, h->gamma_law_index, guess ? guess[k] : 0.0, &P, &it);
+        else rc = c2p_srhd(u, h->gamma_law_index, guess ? guess[k] : 0.0, &P, &it, &rW);
         prim[4 * k] = P.rho; prim[4 * k + 1] = P.ur; prim[4 * k + 2] = P.uq; prim[4 * k + 3] = P.p;
         if (code) code[k] = rc;
         if (iters) iters[k] = it;
@@ -804,7 +846,7 @@ int ko_geometrical_source_terms(const Hydro* h, int64_t n, const double* prim, c
 {
     for (int64_t k = 0; k < n; ++k) {
         const double cot = std::cos(theta[k]) / std::sin(theta[k]);
-        const Cons s = sources(*h, Prim{prim[4 * k], prim[4 * k + 1], prim[4 * k + 2], prim[4 * k + 3]}, r[k], cot);
+        const Cons s = sources(*h, Prim{prim[4 * k], prim[4 * k + 1], prim[4 * k + 2], prim[4 * k + 3]}, 1.0 / r[k], cot);
         const double c = h->light_speed;
         out[4 * k] = s.d * c; out[4 * k + 1] = s.sr * c; out[4 * k + 2] = s.sq * c; out[4 * k + 3] = s.tau * c;
     }
