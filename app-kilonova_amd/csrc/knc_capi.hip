@@ -150,6 +150,7 @@ struct knc_context {
     int nitems = 0;
     std::vector<std::pair<int, int>> block_items;  // (offset, count) per table entry
     int threads = 64;
+    int cpt = 1;
     int persistent_grid = 0;
 
     // per-call device state
@@ -177,6 +178,24 @@ void drop_graphs(knc_context* c)
 }
 
 // Polar table (layout in knc_device.hpp), with separate sin and cos calls.
+// Stage workgroup shape: one thread per polar column up to 512 columns,
+// two columns per thread above; overridden by the options.
+int choose_stage_shape(knc_context* c)
+{
+    const int nq = c->nq;
+    int t = c->o.stage_threads, q = c->o.columns_per_thread;
+    if (q <= 0) q = nq > 512 ? 2 : 1;
+    if (t <= 0) {
+        const int need = (nq + q - 1) / q;
+        t = need <= 64 ? 64 : need <= 128 ? 128 : need <= 256 ? 256 : need <= 512 ? 512 : 1024;
+    }
+    if (!stage_variant_supported(t, q)) return fail(KNC_ERR_INVALID_ARGUMENT, "unsupported stage shape %d x %d", t, q);
+    if (t * q < nq) return fail(KNC_ERR_INVALID_ARGUMENT, "stage shape %d x %d does not cover nq=%d", t, q, nq);
+    c->threads = t;
+    c->cpt = q;
+    return KNC_OK;
+}
+
 int upload_theta(knc_context* c, int nq, const double* qv)
 {
     c->nq = nq;
@@ -215,9 +234,7 @@ int upload_theta(knc_context* c, int nq, const double* qv)
     if (c->d_bnd) HIP_TRY(hipFree(c->d_bnd));
     HIP_TRY(hipMalloc(&c->d_bnd, (size_t)2 * 5 * 2 * nq * sizeof(double)));
     HIP_TRY(hipMemset(c->d_bnd, 0, (size_t)2 * 5 * 2 * nq * sizeof(double)));
-    int threads = (nq + 63) / 64 * 64;
-    c->threads = threads;
-    return KNC_OK;
+    return choose_stage_shape(c);
 }
 
 int choose_rows(knc_context* c)
@@ -227,7 +244,9 @@ int choose_rows(knc_context* c)
     // 4 recomputed halo rows per item a small fraction of the item.
     int64_t rows = 0;
     for (auto& kv : c->blocks) rows += kv.second.nr;
-    const int64_t want = (int64_t)c->num_cu * (c->threads <= 256 ? 4 : 1);
+    const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, c->threads,
+                                                c->cpt));
+    const int64_t want = (int64_t)c->num_cu * occ;
     int r = (int)std::max<int64_t>(4, std::min<int64_t>(32, rows / std::max<int64_t>(1, want)));
     return r;
 }
@@ -287,8 +306,8 @@ int rebuild_tables(knc_context* c)
     HIP_TRY(hipMemcpyAsync(c->d_nitems, &n, sizeof n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     // Persistent grid: every CU holds as many stage workgroups as fit.
-    int per_cu = 1;
-    if (c->threads <= 256) per_cu = std::max(1, 512 / c->threads);  // ~2 waves/SIMD at <=256 VGPRs
+    const int per_cu = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
+                                                   c->threads, c->cpt));
     c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
     c->dirty = false;
     return KNC_OK;
@@ -339,14 +358,15 @@ int launch_one_stage(knc_context* c, StageArgs a)
     if (c->o.launch_mode == KNC_LAUNCH_BATCHED) {
         a.items = c->d_items;
         a.nitems_ptr = c->d_nitems;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, c->persistent_grid, c->threads, c->stream));
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, c->persistent_grid, c->threads, c->cpt,
+                             c->stream));
         c->last_launches += 1;
     } else {
         for (auto& bi : c->block_items) {
             a.items = c->d_items + bi.first;
             a.nitems_ptr = nullptr;
             a.nitems = bi.second;
-            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, bi.second, c->threads, c->stream));
+            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, bi.second, c->threads, c->cpt, c->stream));
             c->last_launches += 1;
         }
     }

@@ -93,87 +93,126 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
     return out;
 }
 
-// Primitive + scalar of row k (relative to block B; -2 <= k < nr+2) at
-// column j: the block's own rows, the last/first two rows of the inner/outer
-// neighbour (src/scheme.rs:73-77), or the boundary rows.  k is uniform
-// across the workgroup, so every branch here is uniform.
-template <int SYS, bool STATS>
-__device__ inline PS fetch_row(const StageArgs& a, const BlockDesc& B, int k, int j, unsigned int* lds_hist)
+// Where row k (relative to block B; -2 <= k < nr+2) comes from: the block's
+// own rows, the last/first two rows of the inner/outer neighbour
+// (src/scheme.rs:73-77), or the boundary rows.  k is uniform across the
+// workgroup, so the resolution and every branch on it are uniform.
+struct RowSrc {
+    const double* base;   // first plane at (row, 0): U planes, or boundary prim planes
+    int64_t np;           // plane stride
+    double rvolA;         // 1/volA of the row (conserved rows)
+    int32_t fixed;        // 1: boundary primitive rows (no c2p)
+    int32_t order, row, index;
+};
+
+__device__ inline RowSrc fixed_rows(const double* rows, int k2, int nq)
 {
-    const int nq = B.nq;
+    RowSrc r;
+    r.fixed = 1;
+    r.base = rows + (int64_t)k2 * nq;
+    r.np = 2 * (int64_t)nq;
+    r.rvolA = 1.0;
+    r.order = 0;
+    r.row = 0;
+    r.index = 0;
+    return r;
+}
+
+__device__ inline RowSrc block_row(const StageArgs& a, const BlockDesc& S, int k)
+{
+    RowSrc r;
+    r.fixed = 0;
+    r.base = slot_of(S, a.in_slot) + (int64_t)k * S.nq;
+    r.np = (int64_t)S.nr * S.nq;
+    r.rvolA = rg_rvolA(S)[k];
+    r.order = S.order;
+    r.row = k;
+    r.index = S.index;
+    return r;
+}
+
+__device__ inline RowSrc resolve_row(const StageArgs& a, const BlockDesc& B, int k)
+{
     if (k < 0) {
-        if (B.left < 0) {
-            if (a.bnd.inner_mode == BND_FIXED) {
-                const double* r = a.bnd.inner_rows;
-                const int64_t o = (int64_t)(k + 2) * nq + j, n2 = 2 * (int64_t)nq;
-                PS q;
-                q.P = Prim{r[o], r[o + n2], r[o + 2 * n2], r[o + 3 * n2]};
-                q.s = r[o + 4 * n2];
-                return q;
-            }
-            k = 0;
-        } else {
+        if (B.left >= 0) {
             const BlockDesc N = a.blocks[B.left];
-            const int row = N.nr + k;
-            const int64_t np = (int64_t)N.nr * nq;
-            const double* s = slot_of(N, a.in_slot) + (int64_t)row * nq + j;
-            const double rdv = rg_rvolA(N)[row] * a.th.rdcost[j];
-            return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, N.order, row, j,
-                                              N.index, lds_hist);
+            return block_row(a, N, N.nr + k);
         }
-    } else if (k >= B.nr) {
-        if (B.right < 0) {
-            if (a.bnd.outer_mode == BND_FIXED) {
-                const double* r = a.bnd.outer_rows;
-                const int64_t o = (int64_t)(k - B.nr) * nq + j, n2 = 2 * (int64_t)nq;
-                PS q;
-                q.P = Prim{r[o], r[o + n2], r[o + 2 * n2], r[o + 3 * n2]};
-                q.s = r[o + 4 * n2];
-                return q;
-            }
-            k = B.nr - 1;
-        } else {
-            const BlockDesc N = a.blocks[B.right];
-            const int row = k - B.nr;
-            const int64_t np = (int64_t)N.nr * nq;
-            const double* s = slot_of(N, a.in_slot) + (int64_t)row * nq + j;
-            const double rdv = rg_rvolA(N)[row] * a.th.rdcost[j];
-            return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, N.order, row, j,
-                                              N.index, lds_hist);
-        }
+        if (a.bnd.inner_mode == BND_FIXED) return fixed_rows(a.bnd.inner_rows, k + 2, B.nq);
+        return block_row(a, B, 0);
     }
-    const int64_t np = (int64_t)B.nr * nq;
-    const double* s = slot_of(B, a.in_slot) + (int64_t)k * nq + j;
-    const double rdv = rg_rvolA(B)[k] * a.th.rdcost[j];
-    return zone_primitive<SYS, STATS>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, B.order, k, j, B.index,
+    if (k >= B.nr) {
+        if (B.right >= 0) {
+            const BlockDesc N = a.blocks[B.right];
+            return block_row(a, N, k - B.nr);
+        }
+        if (a.bnd.outer_mode == BND_FIXED) return fixed_rows(a.bnd.outer_rows, k - B.nr, B.nq);
+        return block_row(a, B, B.nr - 1);
+    }
+    return block_row(a, B, k);
+}
+
+struct Raw5 { double v[5]; };
+
+__device__ inline Raw5 load_raw(const RowSrc& r, int j)
+{
+    const double* p = r.base + j;
+    Raw5 x;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) x.v[f] = p[f * r.np];
+    return x;
+}
+
+template <int SYS, bool STATS>
+__device__ inline PS raw_primitive(const StageArgs& a, const RowSrc& r, const Raw5& x, int j, unsigned int* lds_hist)
+{
+    if (r.fixed) {
+        PS q;
+        q.P = Prim{x.v[0], x.v[1], x.v[2], x.v[3]};
+        q.s = x.v[4];
+        return q;
+    }
+    const double rdv = r.rvolA * a.th.rdcost[j];
+    return zone_primitive<SYS, STATS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], rdv, r.order, r.row, j, r.index,
                                       lds_hist);
 }
 
-__device__ inline void lds_put(double* L, int NT, int j, const PS& q)
+__device__ inline void lds_put(double* L, int W, int j, const PS& q)
 {
     L[j] = q.P.rho;
-    L[NT + j] = q.P.ur;
-    L[2 * NT + j] = q.P.uq;
-    L[3 * NT + j] = q.P.p;
-    L[4 * NT + j] = q.s;
+    L[W + j] = q.P.ur;
+    L[2 * W + j] = q.P.uq;
+    L[3 * W + j] = q.P.p;
+    L[4 * W + j] = q.s;
 }
 
-__device__ inline PS lds_get(const double* L, int NT, int j)
+__device__ inline PS lds_get(const double* L, int W, int j)
 {
     PS q;
-    q.P = Prim{L[j], L[NT + j], L[2 * NT + j], L[3 * NT + j]};
-    q.s = L[4 * NT + j];
+    q.P = Prim{L[j], L[W + j], L[2 * W + j], L[3 * W + j]};
+    q.s = L[4 * W + j];
     return q;
 }
 
 struct Flux5 { double d, sr, sq, tau, s; };
 
-template <int SYS, int RS, bool STATS, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_stage(StageArgs a)
+// (F * c) * A, the order of intercell_flux's scaling followed by the face
+// area product (relativistic_hydro.rs:133, scheme.rs:96-97,127-128).
+__device__ inline Flux5 scaled(const Cons& F, double Fs, double c, double A)
 {
-    extern __shared__ double lds[];            // 5 * blockDim.x doubles
+    return Flux5{F.d * c * A, F.sr * c * A, F.sq * c * A, F.tau * c * A, Fs * c * A};
+}
+
+// NT threads, CPT polar columns per thread (column tid + c*NT).  LDS: two
+// buffers of 5 x NT*CPT doubles (primitives of the row being finished, then
+// its polar fluxes), so each zone row costs two workgroup barriers.
+template <int SYS, int RS, bool STATS, int NT, int CPT>
+__global__ void __launch_bounds__(NT) k_stage(StageArgs a)
+{
+    constexpr int W = NT * CPT;
+    __shared__ double ldsP[5 * W];
+    __shared__ double ldsF[5 * W];
     __shared__ unsigned int lds_hist[HIST_BINS];
-    const int NT = blockDim.x;
     const int tid = threadIdx.x;
     if (STATS) {
         for (int b = tid; b < HIST_BINS; b += NT) lds_hist[b] = 0u;
@@ -183,112 +222,172 @@ __global__ void __launch_bounds__(MAXT) k_stage(StageArgs a)
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
     const double th = a.zp.plm_theta;
     const double c = a.zp.c;
+    const bool avg = a.avg_slot >= 0;
+    const double bw = a.bweight;
+    const double w1 = -bw + 1.0;
 
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const WorkItem it = a.items[w];
         const BlockDesc B = a.blocks[it.block];
         const int nq = B.nq;
-        const bool act = tid < nq;
-        const int j = act ? tid : nq - 1;
-        const double aqj = a.th.aq[j];
-        const double cqj = a.th.cq[j];
-        const double dcost = a.th.dcost[j];
-        const double cotj = a.th.cot[j];
-        const double* rr2 = rg_rr2(B);
         const int64_t np = (int64_t)B.nr * nq;
+        const double* rr2 = rg_rr2(B);
+        const double* Uin = slot_of(B, a.in_slot);
+        double* Uout = slot_of(B, a.out_slot);
+        const double* U0 = slot_of(B, avg ? a.avg_slot : a.in_slot);
 
-        PS Pm2, Pm1, gm2;
-        Flux5 Fm;
+        int jc[CPT];
+        bool act[CPT];
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int slot = tid + q * NT;
+            act[q] = slot < nq;
+            jc[q] = act[q] ? slot : nq - 1;
+        }
+
+        PS Pm2[CPT], Pm1[CPT], gm2[CPT];
+        Flux5 Fm[CPT];
+        Raw5 cur[CPT], nxt[CPT];
+        RowSrc src = resolve_row(a, B, it.i0 - 2);
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) cur[q] = load_raw(src, jc[q]);
+
         for (int k = it.i0 - 2; k < it.i1 + 2; ++k) {
-            const PS Pk = fetch_row<SYS, STATS>(a, B, k, j, lds_hist);
-            if (k >= it.i0) {
-                const PS g = plm5(th, Pm2, Pm1, Pk);                       // gx[k-1]
-                if (k >= it.i0 + 1) {
-                    // Radial face k-1, between rows k-2 and k-1 (scheme.rs:81-97).
-                    Cons F;
-                    double Fs;
-                    riemann<SYS, RS, 0>(add_half(Pm2, gm2), sub_half(Pm1, g), a.zp, &F, &Fs);
-                    const double A = rr2[k - 1] * aqj;
-                    const Flux5 Fk{F.d * c * A, F.sr * c * A, F.sq * c * A, F.tau * c * A, Fs * c * A};
-                    if (k >= it.i0 + 2) {
-                        // Zone row r = k-2: polar fluxes (scheme.rs:107-128) via LDS.
-                        const int r = k - 2;
-                        const PS& P = Pm2;
-                        __syncthreads();
-                        lds_put(lds, NT, tid, P);
-                        __syncthreads();
-                        PS gy{{0.0, 0.0, 0.0, 0.0}, 0.0};
-                        if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(lds, NT, j - 1), P, lds_get(lds, NT, j + 1));
-                        const PS QR = sub_half(P, gy);
-                        const PS QLn = add_half(P, gy);
-                        __syncthreads();
-                        lds_put(lds, NT, tid, QLn);
-                        __syncthreads();
-                        Flux5 Fyj{0.0, 0.0, 0.0, 0.0, 0.0};
-                        if (act && j >= 1) {
-                            Cons G;
-                            double Gs;
-                            riemann<SYS, RS, 1>(lds_get(lds, NT, j - 1), QR, a.zp, &G, &Gs);
-                            const double Aq = rg_rb(B)[r] * cqj;
-                            Fyj = Flux5{G.d * c * Aq, G.sr * c * Aq, G.sq * c * Aq, G.tau * c * Aq, Gs * c * Aq};
-                        }
-                        __syncthreads();
-                        lds[tid] = Fyj.d;
-                        lds[NT + tid] = Fyj.sr;
-                        lds[2 * NT + tid] = Fyj.sq;
-                        lds[3 * NT + tid] = Fyj.tau;
-                        lds[4 * NT + tid] = Fyj.s;
-                        __syncthreads();
-                        Flux5 Fyn{0.0, 0.0, 0.0, 0.0, 0.0};
-                        if (j + 1 < nq) {
-                            Fyn = Flux5{lds[j + 1], lds[NT + j + 1], lds[2 * NT + j + 1], lds[3 * NT + j + 1],
-                                        lds[4 * NT + j + 1]};
-                        }
-                        if (act) {
-                            // Sources x dV and the update (scheme.rs:130-158).
-                            const double dv = rg_volA(B)[r] * dcost;
-                            double ssr, ssq;
-                            sources<SYS>(P.P, rg_rinv(B)[r], cotj, a.zp, &ssr, &ssq);
-                            const double sc0 = 0.0 * c * dv;
-                            const double sc1 = ssr * c * dv;
-                            const double sc2 = ssq * c * dv;
-                            const double sc3 = 0.0 * c * dv;
-                            const double du0 = (sc0 - (Fk.d - Fm.d) - (Fyn.d - Fyj.d)) * dt;
-                            const double du1 = (sc1 - (Fk.sr - Fm.sr) - (Fyn.sr - Fyj.sr)) * dt;
-                            const double du2 = (sc2 - (Fk.sq - Fm.sq) - (Fyn.sq - Fyj.sq)) * dt;
-                            const double du3 = (sc3 - (Fk.tau - Fm.tau) - (Fyn.tau - Fyj.tau)) * dt;
-                            const double ds = ((Fk.s - Fm.s) + (Fyn.s - Fyj.s)) * -dt;
-                            const int64_t o = (int64_t)r * nq + j;
-                            const double* Ui = slot_of(B, a.in_slot) + o;
-                            double* Uo = slot_of(B, a.out_slot) + o;
-                            double v0 = Ui[0] + du0;
-                            double v1 = Ui[np] + du1;
-                            double v2 = Ui[2 * np] + du2;
-                            double v3 = Ui[3 * np] + du3;
-                            double v4 = Ui[4 * np] + ds;
-                            if (a.avg_slot >= 0) {
-                                const double* U0 = slot_of(B, a.avg_slot) + o;
-                                const double b = a.bweight;
-                                const double w1 = -b + 1.0;
-                                v0 = v0 * w1 + U0[0] * b;
-                                v1 = v1 * w1 + U0[np] * b;
-                                v2 = v2 * w1 + U0[2 * np] * b;
-                                v3 = v3 * w1 + U0[3 * np] * b;
-                                v4 = v4 * w1 + U0[4 * np] * b;
-                            }
-                            Uo[0] = v0;
-                            Uo[np] = v1;
-                            Uo[2 * np] = v2;
-                            Uo[3 * np] = v3;
-                            Uo[4 * np] = v4;
-                        }
-                    }
-                    Fm = Fk;
-                }
-                gm2 = g;
+            // Prefetch the next row and this step's update operands.
+            RowSrc nsrc = src;
+            if (k + 1 < it.i1 + 2) {
+                nsrc = resolve_row(a, B, k + 1);
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) nxt[q] = load_raw(nsrc, jc[q]);
             }
-            Pm2 = Pm1;
-            Pm1 = Pk;
+            const bool update = k >= it.i0 + 2;
+            Raw5 ui[CPT], u0[CPT];
+            if (update) {
+                const int64_t o = (int64_t)(k - 2) * nq;
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) {
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) ui[q].v[f] = Uin[o + jc[q] + f * np];
+                    if (avg) {
+#pragma unroll
+                        for (int f = 0; f < 5; ++f) u0[q].v[f] = U0[o + jc[q] + f * np];
+                    }
+                }
+            }
+            Flux5 Fk[CPT];
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const PS Pk = raw_primitive<SYS, STATS>(a, src, cur[q], jc[q], lds_hist);
+                if (k >= it.i0) {
+                    const PS g = plm5(th, Pm2[q], Pm1[q], Pk);                 // gx[k-1]
+                    if (k >= it.i0 + 1) {
+                        // Radial face k-1, between rows k-2 and k-1 (scheme.rs:81-97).
+                        Cons F;
+                        double Fs;
+                        riemann<SYS, RS, 0>(add_half(Pm2[q], gm2[q]), sub_half(Pm1[q], g), a.zp, &F, &Fs);
+                        Fk[q] = scaled(F, Fs, c, rr2[k - 1] * a.th.aq[jc[q]]);
+                    }
+                    gm2[q] = g;
+                }
+                if (!update) {
+                    Fm[q] = Fk[q];
+                    Pm2[q] = Pm1[q];
+                    Pm1[q] = Pk;
+                } else {
+                    // Pm2 stays row r = k-2 until the update below; Pk waits in
+                    // cur (its raw values are no longer needed).
+                    cur[q].v[0] = Pk.P.rho;
+                    cur[q].v[1] = Pk.P.ur;
+                    cur[q].v[2] = Pk.P.uq;
+                    cur[q].v[3] = Pk.P.p;
+                    cur[q].v[4] = Pk.s;
+                }
+            }
+            if (update) {
+                // Zone row r = k-2: polar slopes and fluxes (scheme.rs:107-128).
+                const int r = k - 2;
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) lds_put(ldsP, W, tid + q * NT, Pm2[q]);
+                __syncthreads();
+                Flux5 Fy[CPT];
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) {
+                    const int j = jc[q];
+                    const PS& P = Pm2[q];
+                    PS gy{{0.0, 0.0, 0.0, 0.0}, 0.0};
+                    if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(ldsP, W, j - 1), P, lds_get(ldsP, W, j + 1));
+                    Fy[q] = Flux5{0.0, 0.0, 0.0, 0.0, 0.0};
+                    if (act[q] && j >= 1) {
+                        const PS Pl = lds_get(ldsP, W, j - 1);
+                        PS gl{{0.0, 0.0, 0.0, 0.0}, 0.0};
+                        if (j >= 2) gl = plm5(th, lds_get(ldsP, W, j - 2), Pl, P);
+                        Cons G;
+                        double Gs;
+                        riemann<SYS, RS, 1>(add_half(Pl, gl), sub_half(P, gy), a.zp, &G, &Gs);
+                        Fy[q] = scaled(G, Gs, c, rg_rb(B)[r] * a.th.cq[j]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) {
+                    const int s = tid + q * NT;
+                    ldsF[s] = Fy[q].d;
+                    ldsF[W + s] = Fy[q].sr;
+                    ldsF[2 * W + s] = Fy[q].sq;
+                    ldsF[3 * W + s] = Fy[q].tau;
+                    ldsF[4 * W + s] = Fy[q].s;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) {
+                    const int j = jc[q];
+                    Flux5 Fn{0.0, 0.0, 0.0, 0.0, 0.0};
+                    if (j + 1 < nq)
+                        Fn = Flux5{ldsF[j + 1], ldsF[W + j + 1], ldsF[2 * W + j + 1], ldsF[3 * W + j + 1],
+                                   ldsF[4 * W + j + 1]};
+                    if (act[q]) {
+                        // Sources x dV and the update (scheme.rs:130-158).
+                        const double dv = rg_volA(B)[r] * a.th.dcost[j];
+                        double ssr, ssq;
+                        sources<SYS>(Pm2[q].P, rg_rinv(B)[r], a.th.cot[j], a.zp, &ssr, &ssq);
+                        const double sc0 = 0.0 * c * dv;
+                        const double sc1 = ssr * c * dv;
+                        const double sc2 = ssq * c * dv;
+                        const double sc3 = 0.0 * c * dv;
+                        const Flux5& Fa = Fm[q];
+                        const Flux5& Fb = Fk[q];
+                        const Flux5& Fc = Fy[q];
+                        const double du0 = (sc0 - (Fb.d - Fa.d) - (Fn.d - Fc.d)) * dt;
+                        const double du1 = (sc1 - (Fb.sr - Fa.sr) - (Fn.sr - Fc.sr)) * dt;
+                        const double du2 = (sc2 - (Fb.sq - Fa.sq) - (Fn.sq - Fc.sq)) * dt;
+                        const double du3 = (sc3 - (Fb.tau - Fa.tau) - (Fn.tau - Fc.tau)) * dt;
+                        const double ds = ((Fb.s - Fa.s) + (Fn.s - Fc.s)) * -dt;
+                        double v0 = ui[q].v[0] + du0;
+                        double v1 = ui[q].v[1] + du1;
+                        double v2 = ui[q].v[2] + du2;
+                        double v3 = ui[q].v[3] + du3;
+                        double v4 = ui[q].v[4] + ds;
+                        if (avg) {
+                            v0 = v0 * w1 + u0[q].v[0] * bw;
+                            v1 = v1 * w1 + u0[q].v[1] * bw;
+                            v2 = v2 * w1 + u0[q].v[2] * bw;
+                            v3 = v3 * w1 + u0[q].v[3] * bw;
+                            v4 = v4 * w1 + u0[q].v[4] * bw;
+                        }
+                        double* o = Uout + (int64_t)r * nq + j;
+                        o[0] = v0;
+                        o[np] = v1;
+                        o[2 * np] = v2;
+                        o[3 * np] = v3;
+                        o[4 * np] = v4;
+                    }
+                    Fm[q] = Fk[q];
+                    Pm2[q] = Pm1[q];
+                    Pm1[q] = PS{{cur[q].v[0], cur[q].v[1], cur[q].v[2], cur[q].v[3]}, cur[q].v[4]};
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) cur[q] = nxt[q];
+            src = nsrc;
         }
     }
     if (STATS) {
@@ -465,25 +564,72 @@ static int grid_for(int64_t n, int block)
     return (int)g;
 }
 
-hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, hipStream_t s)
+// Stage kernel variants: (threads, columns per thread).
+template <int SYS, int RS, bool ST>
+static hipError_t launch_stage_sys(const StageArgs& a, int grid, int threads, int cpt, hipStream_t s)
 {
-    const size_t shmem = (size_t)5 * threads * sizeof(double);
-#define KNC_STAGE(SYS, RS, ST)                                                                       \
-    do {                                                                                             \
-        if (threads <= 256)                                                                          \
-            hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256>), dim3(grid), dim3(threads), shmem, s, a); \
-        else                                                                                         \
-            hipLaunchKernelGGL((k_stage<SYS, RS, ST, 1024>), dim3(grid), dim3(threads), shmem, s, a); \
-    } while (0)
-    if (system == SYS_SRHD) {
-        if (rs == RS_HLLC) { if (stats) KNC_STAGE(SYS_SRHD, RS_HLLC, true); else KNC_STAGE(SYS_SRHD, RS_HLLC, false); }
-        else               { if (stats) KNC_STAGE(SYS_SRHD, RS_HLLE, true); else KNC_STAGE(SYS_SRHD, RS_HLLE, false); }
-    } else {
-        if (rs == RS_HLLC) { if (stats) KNC_STAGE(SYS_EULER, RS_HLLC, true); else KNC_STAGE(SYS_EULER, RS_HLLC, false); }
-        else               { if (stats) KNC_STAGE(SYS_EULER, RS_HLLE, true); else KNC_STAGE(SYS_EULER, RS_HLLE, false); }
-    }
-#undef KNC_STAGE
+    if (cpt == 1 && threads == 64) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 64, 1>), dim3(grid), dim3(64), 0, s, a);
+    else if (cpt == 1 && threads == 128) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 128, 1>), dim3(grid), dim3(128), 0, s, a);
+    else if (cpt == 1 && threads == 256) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256, 1>), dim3(grid), dim3(256), 0, s, a);
+    else if (cpt == 1 && threads == 512) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512, 1>), dim3(grid), dim3(512), 0, s, a);
+    else if (cpt == 1 && threads == 1024) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 1024, 1>), dim3(grid), dim3(1024), 0, s, a);
+    else if (cpt == 2 && threads == 256) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256, 2>), dim3(grid), dim3(256), 0, s, a);
+    else if (cpt == 2 && threads == 512) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512, 2>), dim3(grid), dim3(512), 0, s, a);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+template <int SYS, int RS, bool ST>
+static int occupancy_sys(int threads, int cpt)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define KNC_OCC(T, C) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, T, C>, T, 0)
+    if (cpt == 1 && threads == 64) KNC_OCC(64, 1);
+    else if (cpt == 1 && threads == 128) KNC_OCC(128, 1);
+    else if (cpt == 1 && threads == 256) KNC_OCC(256, 1);
+    else if (cpt == 1 && threads == 512) KNC_OCC(512, 1);
+    else if (cpt == 1 && threads == 1024) KNC_OCC(1024, 1);
+    else if (cpt == 2 && threads == 256) KNC_OCC(256, 2);
+    else if (cpt == 2 && threads == 512) KNC_OCC(512, 2);
+#undef KNC_OCC
+    return e == hipSuccess ? n : 0;
+}
+
+int stage_occupancy(int system, int rs, bool stats, int threads, int cpt)
+{
+    if (system == SYS_SRHD) {
+        if (rs == RS_HLLC) return stats ? occupancy_sys<SYS_SRHD, RS_HLLC, true>(threads, cpt)
+                                        : occupancy_sys<SYS_SRHD, RS_HLLC, false>(threads, cpt);
+        return stats ? occupancy_sys<SYS_SRHD, RS_HLLE, true>(threads, cpt)
+                     : occupancy_sys<SYS_SRHD, RS_HLLE, false>(threads, cpt);
+    }
+    if (rs == RS_HLLC) return stats ? occupancy_sys<SYS_EULER, RS_HLLC, true>(threads, cpt)
+                                    : occupancy_sys<SYS_EULER, RS_HLLC, false>(threads, cpt);
+    return stats ? occupancy_sys<SYS_EULER, RS_HLLE, true>(threads, cpt)
+                 : occupancy_sys<SYS_EULER, RS_HLLE, false>(threads, cpt);
+}
+
+bool stage_variant_supported(int threads, int cpt)
+{
+    if (cpt == 1) return threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024;
+    if (cpt == 2) return threads == 256 || threads == 512;
+    return false;
+}
+
+hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, int cpt,
+                        hipStream_t s)
+{
+    if (system == SYS_SRHD) {
+        if (rs == RS_HLLC) return stats ? launch_stage_sys<SYS_SRHD, RS_HLLC, true>(a, grid, threads, cpt, s)
+                                        : launch_stage_sys<SYS_SRHD, RS_HLLC, false>(a, grid, threads, cpt, s);
+        return stats ? launch_stage_sys<SYS_SRHD, RS_HLLE, true>(a, grid, threads, cpt, s)
+                     : launch_stage_sys<SYS_SRHD, RS_HLLE, false>(a, grid, threads, cpt, s);
+    }
+    if (rs == RS_HLLC) return stats ? launch_stage_sys<SYS_EULER, RS_HLLC, true>(a, grid, threads, cpt, s)
+                                    : launch_stage_sys<SYS_EULER, RS_HLLC, false>(a, grid, threads, cpt, s);
+    return stats ? launch_stage_sys<SYS_EULER, RS_HLLE, true>(a, grid, threads, cpt, s)
+                 : launch_stage_sys<SYS_EULER, RS_HLLE, false>(a, grid, threads, cpt, s);
 }
 
 hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s)
