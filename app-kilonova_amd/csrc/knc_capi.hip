@@ -150,7 +150,6 @@ struct knc_context {
     int nitems = 0;
     std::vector<std::pair<int, int>> block_items;  // (offset, count) per table entry
     int threads = 64;
-    int cpt = 1;
     int persistent_grid = 0;
 
     // per-call device state
@@ -178,21 +177,15 @@ void drop_graphs(knc_context* c)
 }
 
 // Polar table (layout in knc_device.hpp), with separate sin and cos calls.
-// Stage workgroup shape: one thread per polar column up to 512 columns,
-// two columns per thread above; overridden by the options.
+// Stage workgroup size: one thread per polar column for meshes up to 256
+// columns; wider meshes are cut into 256-column tiles.
 int choose_stage_shape(knc_context* c)
 {
     const int nq = c->nq;
-    int t = c->o.stage_threads, q = c->o.columns_per_thread;
-    if (q <= 0) q = nq > 512 ? 2 : 1;
-    if (t <= 0) {
-        const int need = (nq + q - 1) / q;
-        t = need <= 64 ? 64 : need <= 128 ? 128 : need <= 256 ? 256 : need <= 512 ? 512 : 1024;
-    }
-    if (!stage_variant_supported(t, q)) return fail(KNC_ERR_INVALID_ARGUMENT, "unsupported stage shape %d x %d", t, q);
-    if (t * q < nq) return fail(KNC_ERR_INVALID_ARGUMENT, "stage shape %d x %d does not cover nq=%d", t, q, nq);
+    int t = c->o.stage_threads;
+    if (t <= 0) t = nq <= 64 ? 64 : nq <= 128 ? 128 : 256;
+    if (!stage_variant_supported(t)) return fail(KNC_ERR_INVALID_ARGUMENT, "unsupported stage workgroup size %d", t);
     c->threads = t;
-    c->cpt = q;
     return KNC_OK;
 }
 
@@ -239,16 +232,16 @@ int upload_theta(knc_context* c, int nq, const double* qv)
 
 int choose_rows(knc_context* c)
 {
-    if (c->o.rows_per_item > 0) return c->o.rows_per_item;
-    // Enough work items to give every CU several workgroups, while keeping the
-    // 4 recomputed halo rows per item a small fraction of the item.
+    if (c->o.rows_per_item > 0) return std::min(c->o.rows_per_item, stage_max_rows());
+    // Enough work items for one full round of resident workgroups, while
+    // keeping the 4 recomputed halo rows per item a small fraction.
     int64_t rows = 0;
     for (auto& kv : c->blocks) rows += kv.second.nr;
-    const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, c->threads,
-                                                c->cpt));
+    const int tiles = (c->nq + c->threads - 1) / c->threads;
+    const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, c->threads));
     const int64_t want = (int64_t)c->num_cu * occ;
-    int r = (int)std::max<int64_t>(4, std::min<int64_t>(32, rows / std::max<int64_t>(1, want)));
-    return r;
+    int64_t r = rows * tiles / std::max<int64_t>(1, want);
+    return (int)std::max<int64_t>(4, std::min<int64_t>(32, r));
 }
 
 // Rebuilds the device block table and work list after the block set changed.
@@ -280,7 +273,15 @@ int rebuild_tables(knc_context* c)
         desc.push_back(d);
         const int off = (int)items.size();
         for (int i0 = 0; i0 < b.nr; i0 += c->rows_per_item)
-            items.push_back(WorkItem{order, i0, std::min(b.nr, i0 + c->rows_per_item), 0});
+            for (int j0 = 0; j0 < b.nq; j0 += c->threads) {
+                WorkItem w{};
+                w.block = order;
+                w.i0 = i0;
+                w.i1 = std::min(b.nr, i0 + c->rows_per_item);
+                w.j0 = j0;
+                w.j1 = std::min(b.nq, j0 + c->threads);
+                items.push_back(w);
+            }
         c->block_items.push_back({off, (int)items.size() - off});
         ++order;
     }
@@ -307,7 +308,7 @@ int rebuild_tables(knc_context* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     // Persistent grid: every CU holds as many stage workgroups as fit.
     const int per_cu = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
-                                                   c->threads, c->cpt));
+                                                   c->threads));
     c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
     c->dirty = false;
     return KNC_OK;
@@ -358,15 +359,14 @@ int launch_one_stage(knc_context* c, StageArgs a)
     if (c->o.launch_mode == KNC_LAUNCH_BATCHED) {
         a.items = c->d_items;
         a.nitems_ptr = c->d_nitems;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, c->persistent_grid, c->threads, c->cpt,
-                             c->stream));
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, c->persistent_grid, c->threads, c->stream));
         c->last_launches += 1;
     } else {
         for (auto& bi : c->block_items) {
             a.items = c->d_items + bi.first;
             a.nitems_ptr = nullptr;
             a.nitems = bi.second;
-            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, bi.second, c->threads, c->cpt, c->stream));
+            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, bi.second, c->threads, c->stream));
             c->last_launches += 1;
         }
     }

@@ -83,12 +83,15 @@ struct BoundaryDesc {
     const double* outer_rows;
 };
 
-// One unit of work of a stage launch: rows [i0, i1) of one block.
+// One unit of work of a stage launch: rows [i0, i1) x polar columns
+// [j0, j1) of one block.
 struct WorkItem {
     int32_t block;
     int32_t i0;
     int32_t i1;
-    int32_t pad;
+    int32_t j0;
+    int32_t j1;
+    int32_t pad[3];
 };
 
 // Error log of a launch sequence.  key = (step << 48) | (order << 32) |

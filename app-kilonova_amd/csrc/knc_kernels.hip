@@ -203,15 +203,30 @@ __device__ inline Flux5 scaled(const Cons& F, double Fs, double c, double A)
     return Flux5{F.d * c * A, F.sr * c * A, F.sq * c * A, F.tau * c * A, Fs * c * A};
 }
 
-// NT threads, CPT polar columns per thread (column tid + c*NT).  LDS: two
-// buffers of 5 x NT*CPT doubles (primitives of the row being finished, then
-// its polar fluxes), so each zone row costs two workgroup barriers.
-template <int SYS, int RS, bool STATS, int NT, int CPT>
+// Stage kernel.  NT threads, one polar column each; a work item is rows
+// [i0,i1) x columns [j0,j1) of one block (j1 - j0 <= NT).  When the item does
+// not span every column, its halo columns j0-2, j0-1 and j1-2 .. j1+1 are
+// recovered once per item for all its rows, and the fluxes through its right
+// polar face j1, into LDS before the march; each zone row then costs two
+// workgroup barriers (primitive exchange, polar flux exchange).
+constexpr int MAX_ROWS = 64;
+constexpr int NHALO = 6;   // j0-2, j0-1 | j1-2, j1-1, j1, j1+1
+
+template <int SYS, int RS, bool STATS>
+__device__ inline PS column_primitive(const StageArgs& a, const BlockDesc& B, int row, int col, unsigned int* lds_hist)
+{
+    const RowSrc src = block_row(a, B, row);
+    return raw_primitive<SYS, STATS>(a, src, load_raw(src, col), col, lds_hist);
+}
+
+template <int SYS, int RS, bool STATS, int NT>
 __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 {
-    constexpr int W = NT * CPT;
-    __shared__ double ldsP[5 * W];
-    __shared__ double ldsF[5 * W];
+    constexpr int WP = NT + 3;                 // primitive row: 2 left halo, NT own, 1 right halo
+    __shared__ double ldsP[5 * WP];
+    __shared__ double ldsF[5 * NT];
+    __shared__ double ldsH[NHALO * MAX_ROWS * 5];
+    __shared__ double ldsR[MAX_ROWS * 5];      // flux through face j1 per row
     __shared__ unsigned int lds_hist[HIST_BINS];
     const int tid = threadIdx.x;
     if (STATS) {
@@ -225,168 +240,178 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
     const bool avg = a.avg_slot >= 0;
     const double bw = a.bweight;
     const double w1 = -bw + 1.0;
+    const PS zero{{0.0, 0.0, 0.0, 0.0}, 0.0};
 
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const WorkItem it = a.items[w];
         const BlockDesc B = a.blocks[it.block];
         const int nq = B.nq;
+        const int j0 = it.j0, j1 = it.j1;
+        const int nrow = it.i1 - it.i0;
         const int64_t np = (int64_t)B.nr * nq;
         const double* rr2 = rg_rr2(B);
         const double* Uin = slot_of(B, a.in_slot);
         double* Uout = slot_of(B, a.out_slot);
         const double* U0 = slot_of(B, avg ? a.avg_slot : a.in_slot);
+        const bool act = j0 + tid < j1;
+        const int j = act ? j0 + tid : j1 - 1;
+        const bool left_halo = j0 > 0;
+        const bool right_halo = j1 < nq;
 
-        int jc[CPT];
-        bool act[CPT];
+        if (left_halo || right_halo) {
+            // Halo primitives of this item's rows, then the right-face fluxes.
+            __syncthreads();
+            for (int z = tid; z < NHALO * nrow; z += NT) {
+                const int h = z % NHALO, rr = z / NHALO;
+                const int col = h < 2 ? j0 - 2 + h : j1 - 4 + h;
+                const bool need = h < 2 ? left_halo : right_halo;
+                PS q = zero;
+                if (need && col >= 0 && col < nq) q = column_primitive<SYS, RS, STATS>(a, B, it.i0 + rr, col, lds_hist);
+                double* d = ldsH + (h * MAX_ROWS + rr) * 5;
+                d[0] = q.P.rho; d[1] = q.P.ur; d[2] = q.P.uq; d[3] = q.P.p; d[4] = q.s;
+            }
+            __syncthreads();
+            if (right_halo) {
+                for (int rr = tid; rr < nrow; rr += NT) {
+                    PS q[4];
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-            const int slot = tid + q * NT;
-            act[q] = slot < nq;
-            jc[q] = act[q] ? slot : nq - 1;
+                    for (int h = 0; h < 4; ++h) {
+                        const double* d = ldsH + ((2 + h) * MAX_ROWS + rr) * 5;
+                        q[h] = PS{{d[0], d[1], d[2], d[3]}, d[4]};
+                    }
+                    // face j1 between columns j1-1 (q[1]) and j1 (q[2])
+                    PS gl = zero, gr = zero;
+                    if (j1 - 1 >= 1) gl = plm5(th, q[0], q[1], q[2]);
+                    if (j1 + 1 < nq) gr = plm5(th, q[1], q[2], q[3]);
+                    Cons G;
+                    double Gs;
+                    riemann<SYS, RS, 1>(add_half(q[1], gl), sub_half(q[2], gr), a.zp, &G, &Gs);
+                    const Flux5 f = scaled(G, Gs, c, rg_rb(B)[it.i0 + rr] * a.th.cq[j1]);
+                    double* d = ldsR + rr * 5;
+                    d[0] = f.d; d[1] = f.sr; d[2] = f.sq; d[3] = f.tau; d[4] = f.s;
+                }
+            }
+            // the first row exchange below begins with a barrier
         }
 
-        PS Pm2[CPT], Pm1[CPT], gm2[CPT];
-        Flux5 Fm[CPT];
-        Raw5 cur[CPT], nxt[CPT];
+        PS Pm2, Pm1, gm2;
+        Flux5 Fm;
         RowSrc src = resolve_row(a, B, it.i0 - 2);
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) cur[q] = load_raw(src, jc[q]);
+        Raw5 cur = load_raw(src, j), nxt;
 
         for (int k = it.i0 - 2; k < it.i1 + 2; ++k) {
             // Prefetch the next row and this step's update operands.
             RowSrc nsrc = src;
             if (k + 1 < it.i1 + 2) {
                 nsrc = resolve_row(a, B, k + 1);
-#pragma unroll
-                for (int q = 0; q < CPT; ++q) nxt[q] = load_raw(nsrc, jc[q]);
+                nxt = load_raw(nsrc, j);
             }
             const bool update = k >= it.i0 + 2;
-            Raw5 ui[CPT], u0[CPT];
+            Raw5 ui, u0;
             if (update) {
-                const int64_t o = (int64_t)(k - 2) * nq;
+                const int64_t o = (int64_t)(k - 2) * nq + j;
 #pragma unroll
-                for (int q = 0; q < CPT; ++q) {
+                for (int f = 0; f < 5; ++f) ui.v[f] = Uin[o + f * np];
+                if (avg) {
 #pragma unroll
-                    for (int f = 0; f < 5; ++f) ui[q].v[f] = Uin[o + jc[q] + f * np];
-                    if (avg) {
-#pragma unroll
-                        for (int f = 0; f < 5; ++f) u0[q].v[f] = U0[o + jc[q] + f * np];
-                    }
+                    for (int f = 0; f < 5; ++f) u0.v[f] = U0[o + f * np];
                 }
             }
-            Flux5 Fk[CPT];
-#pragma unroll
-            for (int q = 0; q < CPT; ++q) {
-                const PS Pk = raw_primitive<SYS, STATS>(a, src, cur[q], jc[q], lds_hist);
-                if (k >= it.i0) {
-                    const PS g = plm5(th, Pm2[q], Pm1[q], Pk);                 // gx[k-1]
-                    if (k >= it.i0 + 1) {
-                        // Radial face k-1, between rows k-2 and k-1 (scheme.rs:81-97).
-                        Cons F;
-                        double Fs;
-                        riemann<SYS, RS, 0>(add_half(Pm2[q], gm2[q]), sub_half(Pm1[q], g), a.zp, &F, &Fs);
-                        Fk[q] = scaled(F, Fs, c, rr2[k - 1] * a.th.aq[jc[q]]);
-                    }
-                    gm2[q] = g;
+            const PS Pk = raw_primitive<SYS, STATS>(a, src, cur, j, lds_hist);
+            Flux5 Fk;
+            if (k >= it.i0) {
+                const PS g = plm5(th, Pm2, Pm1, Pk);                 // gx[k-1]
+                if (k >= it.i0 + 1) {
+                    // Radial face k-1, between rows k-2 and k-1 (scheme.rs:81-97).
+                    Cons F;
+                    double Fs;
+                    riemann<SYS, RS, 0>(add_half(Pm2, gm2), sub_half(Pm1, g), a.zp, &F, &Fs);
+                    Fk = scaled(F, Fs, c, rr2[k - 1] * a.th.aq[j]);
                 }
-                if (!update) {
-                    Fm[q] = Fk[q];
-                    Pm2[q] = Pm1[q];
-                    Pm1[q] = Pk;
-                } else {
-                    // Pm2 stays row r = k-2 until the update below; Pk waits in
-                    // cur (its raw values are no longer needed).
-                    cur[q].v[0] = Pk.P.rho;
-                    cur[q].v[1] = Pk.P.ur;
-                    cur[q].v[2] = Pk.P.uq;
-                    cur[q].v[3] = Pk.P.p;
-                    cur[q].v[4] = Pk.s;
-                }
+                gm2 = g;
             }
             if (update) {
                 // Zone row r = k-2: polar slopes and fluxes (scheme.rs:107-128).
-                const int r = k - 2;
-#pragma unroll
-                for (int q = 0; q < CPT; ++q) lds_put(ldsP, W, tid + q * NT, Pm2[q]);
+                const int r = k - 2, rr = r - it.i0;
+                const PS& P = Pm2;
                 __syncthreads();
-                Flux5 Fy[CPT];
-#pragma unroll
-                for (int q = 0; q < CPT; ++q) {
-                    const int j = jc[q];
-                    const PS& P = Pm2[q];
-                    PS gy{{0.0, 0.0, 0.0, 0.0}, 0.0};
-                    if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(ldsP, W, j - 1), P, lds_get(ldsP, W, j + 1));
-                    Fy[q] = Flux5{0.0, 0.0, 0.0, 0.0, 0.0};
-                    if (act[q] && j >= 1) {
-                        const PS Pl = lds_get(ldsP, W, j - 1);
-                        PS gl{{0.0, 0.0, 0.0, 0.0}, 0.0};
-                        if (j >= 2) gl = plm5(th, lds_get(ldsP, W, j - 2), Pl, P);
-                        Cons G;
-                        double Gs;
-                        riemann<SYS, RS, 1>(add_half(Pl, gl), sub_half(P, gy), a.zp, &G, &Gs);
-                        Fy[q] = scaled(G, Gs, c, rg_rb(B)[r] * a.th.cq[j]);
-                    }
+                lds_put(ldsP, WP, tid + 2, P);
+                if (tid < 2 && left_halo) {
+                    const double* d = ldsH + (tid * MAX_ROWS + rr) * 5;
+                    lds_put(ldsP, WP, tid, PS{{d[0], d[1], d[2], d[3]}, d[4]});
                 }
-#pragma unroll
-                for (int q = 0; q < CPT; ++q) {
-                    const int s = tid + q * NT;
-                    ldsF[s] = Fy[q].d;
-                    ldsF[W + s] = Fy[q].sr;
-                    ldsF[2 * W + s] = Fy[q].sq;
-                    ldsF[3 * W + s] = Fy[q].tau;
-                    ldsF[4 * W + s] = Fy[q].s;
+                if (tid == NT - 1 && right_halo) {
+                    const double* d = ldsH + (4 * MAX_ROWS + rr) * 5;
+                    lds_put(ldsP, WP, NT + 2, PS{{d[0], d[1], d[2], d[3]}, d[4]});
                 }
                 __syncthreads();
-#pragma unroll
-                for (int q = 0; q < CPT; ++q) {
-                    const int j = jc[q];
-                    Flux5 Fn{0.0, 0.0, 0.0, 0.0, 0.0};
-                    if (j + 1 < nq)
-                        Fn = Flux5{ldsF[j + 1], ldsF[W + j + 1], ldsF[2 * W + j + 1], ldsF[3 * W + j + 1],
-                                   ldsF[4 * W + j + 1]};
-                    if (act[q]) {
-                        // Sources x dV and the update (scheme.rs:130-158).
-                        const double dv = rg_volA(B)[r] * a.th.dcost[j];
-                        double ssr, ssq;
-                        sources<SYS>(Pm2[q].P, rg_rinv(B)[r], a.th.cot[j], a.zp, &ssr, &ssq);
-                        const double sc0 = 0.0 * c * dv;
-                        const double sc1 = ssr * c * dv;
-                        const double sc2 = ssq * c * dv;
-                        const double sc3 = 0.0 * c * dv;
-                        const Flux5& Fa = Fm[q];
-                        const Flux5& Fb = Fk[q];
-                        const Flux5& Fc = Fy[q];
-                        const double du0 = (sc0 - (Fb.d - Fa.d) - (Fn.d - Fc.d)) * dt;
-                        const double du1 = (sc1 - (Fb.sr - Fa.sr) - (Fn.sr - Fc.sr)) * dt;
-                        const double du2 = (sc2 - (Fb.sq - Fa.sq) - (Fn.sq - Fc.sq)) * dt;
-                        const double du3 = (sc3 - (Fb.tau - Fa.tau) - (Fn.tau - Fc.tau)) * dt;
-                        const double ds = ((Fb.s - Fa.s) + (Fn.s - Fc.s)) * -dt;
-                        double v0 = ui[q].v[0] + du0;
-                        double v1 = ui[q].v[1] + du1;
-                        double v2 = ui[q].v[2] + du2;
-                        double v3 = ui[q].v[3] + du3;
-                        double v4 = ui[q].v[4] + ds;
-                        if (avg) {
-                            v0 = v0 * w1 + u0[q].v[0] * bw;
-                            v1 = v1 * w1 + u0[q].v[1] * bw;
-                            v2 = v2 * w1 + u0[q].v[2] * bw;
-                            v3 = v3 * w1 + u0[q].v[3] * bw;
-                            v4 = v4 * w1 + u0[q].v[4] * bw;
-                        }
-                        double* o = Uout + (int64_t)r * nq + j;
-                        o[0] = v0;
-                        o[np] = v1;
-                        o[2 * np] = v2;
-                        o[3 * np] = v3;
-                        o[4 * np] = v4;
+                const int lp = tid + 2;
+                PS gy = zero;
+                if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(ldsP, WP, lp - 1), P, lds_get(ldsP, WP, lp + 1));
+                Flux5 Fy{0.0, 0.0, 0.0, 0.0, 0.0};
+                if (act && j >= 1) {
+                    const PS Pl = lds_get(ldsP, WP, lp - 1);
+                    PS gl = zero;
+                    if (j >= 2) gl = plm5(th, lds_get(ldsP, WP, lp - 2), Pl, P);
+                    Cons G;
+                    double Gs;
+                    riemann<SYS, RS, 1>(add_half(Pl, gl), sub_half(P, gy), a.zp, &G, &Gs);
+                    Fy = scaled(G, Gs, c, rg_rb(B)[r] * a.th.cq[j]);
+                }
+                ldsF[tid] = Fy.d;
+                ldsF[NT + tid] = Fy.sr;
+                ldsF[2 * NT + tid] = Fy.sq;
+                ldsF[3 * NT + tid] = Fy.tau;
+                ldsF[4 * NT + tid] = Fy.s;
+                __syncthreads();
+                Flux5 Fn{0.0, 0.0, 0.0, 0.0, 0.0};
+                if (j + 1 < nq) {
+                    if (j + 1 < j1) {
+                        Fn = Flux5{ldsF[tid + 1], ldsF[NT + tid + 1], ldsF[2 * NT + tid + 1], ldsF[3 * NT + tid + 1],
+                                   ldsF[4 * NT + tid + 1]};
+                    } else {
+                        const double* d = ldsR + rr * 5;
+                        Fn = Flux5{d[0], d[1], d[2], d[3], d[4]};
                     }
-                    Fm[q] = Fk[q];
-                    Pm2[q] = Pm1[q];
-                    Pm1[q] = PS{{cur[q].v[0], cur[q].v[1], cur[q].v[2], cur[q].v[3]}, cur[q].v[4]};
+                }
+                if (act) {
+                    // Sources x dV and the update (scheme.rs:130-158).
+                    const double dv = rg_volA(B)[r] * a.th.dcost[j];
+                    double ssr, ssq;
+                    sources<SYS>(P.P, rg_rinv(B)[r], a.th.cot[j], a.zp, &ssr, &ssq);
+                    const double sc0 = 0.0 * c * dv;
+                    const double sc1 = ssr * c * dv;
+                    const double sc2 = ssq * c * dv;
+                    const double sc3 = 0.0 * c * dv;
+                    const double du0 = (sc0 - (Fk.d - Fm.d) - (Fn.d - Fy.d)) * dt;
+                    const double du1 = (sc1 - (Fk.sr - Fm.sr) - (Fn.sr - Fy.sr)) * dt;
+                    const double du2 = (sc2 - (Fk.sq - Fm.sq) - (Fn.sq - Fy.sq)) * dt;
+                    const double du3 = (sc3 - (Fk.tau - Fm.tau) - (Fn.tau - Fy.tau)) * dt;
+                    const double ds = ((Fk.s - Fm.s) + (Fn.s - Fy.s)) * -dt;
+                    double v0 = ui.v[0] + du0;
+                    double v1 = ui.v[1] + du1;
+                    double v2 = ui.v[2] + du2;
+                    double v3 = ui.v[3] + du3;
+                    double v4 = ui.v[4] + ds;
+                    if (avg) {
+                        v0 = v0 * w1 + u0.v[0] * bw;
+                        v1 = v1 * w1 + u0.v[1] * bw;
+                        v2 = v2 * w1 + u0.v[2] * bw;
+                        v3 = v3 * w1 + u0.v[3] * bw;
+                        v4 = v4 * w1 + u0.v[4] * bw;
+                    }
+                    double* o = Uout + (int64_t)r * nq + j;
+                    o[0] = v0;
+                    o[np] = v1;
+                    o[2 * np] = v2;
+                    o[3 * np] = v3;
+                    o[4 * np] = v4;
                 }
             }
-#pragma unroll
-            for (int q = 0; q < CPT; ++q) cur[q] = nxt[q];
+            Fm = Fk;
+            Pm2 = Pm1;
+            Pm1 = Pk;
+            cur = nxt;
             src = nsrc;
         }
     }
@@ -564,72 +589,61 @@ static int grid_for(int64_t n, int block)
     return (int)g;
 }
 
-// Stage kernel variants: (threads, columns per thread).
+// Stage kernel variants by workgroup size.
 template <int SYS, int RS, bool ST>
-static hipError_t launch_stage_sys(const StageArgs& a, int grid, int threads, int cpt, hipStream_t s)
+static hipError_t launch_stage_sys(const StageArgs& a, int grid, int threads, hipStream_t s)
 {
-    if (cpt == 1 && threads == 64) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 64, 1>), dim3(grid), dim3(64), 0, s, a);
-    else if (cpt == 1 && threads == 128) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 128, 1>), dim3(grid), dim3(128), 0, s, a);
-    else if (cpt == 1 && threads == 256) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256, 1>), dim3(grid), dim3(256), 0, s, a);
-    else if (cpt == 1 && threads == 512) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512, 1>), dim3(grid), dim3(512), 0, s, a);
-    else if (cpt == 1 && threads == 1024) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 1024, 1>), dim3(grid), dim3(1024), 0, s, a);
-    else if (cpt == 2 && threads == 256) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256, 2>), dim3(grid), dim3(256), 0, s, a);
-    else if (cpt == 2 && threads == 512) hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512, 2>), dim3(grid), dim3(512), 0, s, a);
-    else return hipErrorInvalidValue;
+    switch (threads) {
+    case 64: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 64>), dim3(grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 128>), dim3(grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256>), dim3(grid), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512>), dim3(grid), dim3(512), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 template <int SYS, int RS, bool ST>
-static int occupancy_sys(int threads, int cpt)
+static int occupancy_sys(int threads)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-#define KNC_OCC(T, C) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, T, C>, T, 0)
-    if (cpt == 1 && threads == 64) KNC_OCC(64, 1);
-    else if (cpt == 1 && threads == 128) KNC_OCC(128, 1);
-    else if (cpt == 1 && threads == 256) KNC_OCC(256, 1);
-    else if (cpt == 1 && threads == 512) KNC_OCC(512, 1);
-    else if (cpt == 1 && threads == 1024) KNC_OCC(1024, 1);
-    else if (cpt == 2 && threads == 256) KNC_OCC(256, 2);
-    else if (cpt == 2 && threads == 512) KNC_OCC(512, 2);
-#undef KNC_OCC
+    switch (threads) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 64>, 64, 0); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 128>, 128, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 256>, 256, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 512>, 512, 0); break;
+    default: break;
+    }
     return e == hipSuccess ? n : 0;
 }
 
-int stage_occupancy(int system, int rs, bool stats, int threads, int cpt)
+bool stage_variant_supported(int threads)
 {
-    if (system == SYS_SRHD) {
-        if (rs == RS_HLLC) return stats ? occupancy_sys<SYS_SRHD, RS_HLLC, true>(threads, cpt)
-                                        : occupancy_sys<SYS_SRHD, RS_HLLC, false>(threads, cpt);
-        return stats ? occupancy_sys<SYS_SRHD, RS_HLLE, true>(threads, cpt)
-                     : occupancy_sys<SYS_SRHD, RS_HLLE, false>(threads, cpt);
-    }
-    if (rs == RS_HLLC) return stats ? occupancy_sys<SYS_EULER, RS_HLLC, true>(threads, cpt)
-                                    : occupancy_sys<SYS_EULER, RS_HLLC, false>(threads, cpt);
-    return stats ? occupancy_sys<SYS_EULER, RS_HLLE, true>(threads, cpt)
-                 : occupancy_sys<SYS_EULER, RS_HLLE, false>(threads, cpt);
+    return threads == 64 || threads == 128 || threads == 256 || threads == 512;
 }
 
-bool stage_variant_supported(int threads, int cpt)
+int stage_max_rows() { return MAX_ROWS; }
+
+#define KNC_DISPATCH(FN, ...)                                                                          \
+    (system == SYS_SRHD                                                                                \
+         ? (rs == RS_HLLC ? (stats ? FN<SYS_SRHD, RS_HLLC, true>(__VA_ARGS__)                           \
+                                   : FN<SYS_SRHD, RS_HLLC, false>(__VA_ARGS__))                         \
+                          : (stats ? FN<SYS_SRHD, RS_HLLE, true>(__VA_ARGS__)                           \
+                                   : FN<SYS_SRHD, RS_HLLE, false>(__VA_ARGS__)))                        \
+         : (rs == RS_HLLC ? (stats ? FN<SYS_EULER, RS_HLLC, true>(__VA_ARGS__)                          \
+                                   : FN<SYS_EULER, RS_HLLC, false>(__VA_ARGS__))                        \
+                          : (stats ? FN<SYS_EULER, RS_HLLE, true>(__VA_ARGS__)                          \
+                                   : FN<SYS_EULER, RS_HLLE, false>(__VA_ARGS__))))
+
+int stage_occupancy(int system, int rs, bool stats, int threads)
 {
-    if (cpt == 1) return threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024;
-    if (cpt == 2) return threads == 256 || threads == 512;
-    return false;
+    return KNC_DISPATCH(occupancy_sys, threads);
 }
 
-hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, int cpt,
-                        hipStream_t s)
+hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, hipStream_t s)
 {
-    if (system == SYS_SRHD) {
-        if (rs == RS_HLLC) return stats ? launch_stage_sys<SYS_SRHD, RS_HLLC, true>(a, grid, threads, cpt, s)
-                                        : launch_stage_sys<SYS_SRHD, RS_HLLC, false>(a, grid, threads, cpt, s);
-        return stats ? launch_stage_sys<SYS_SRHD, RS_HLLE, true>(a, grid, threads, cpt, s)
-                     : launch_stage_sys<SYS_SRHD, RS_HLLE, false>(a, grid, threads, cpt, s);
-    }
-    if (rs == RS_HLLC) return stats ? launch_stage_sys<SYS_EULER, RS_HLLC, true>(a, grid, threads, cpt, s)
-                                    : launch_stage_sys<SYS_EULER, RS_HLLC, false>(a, grid, threads, cpt, s);
-    return stats ? launch_stage_sys<SYS_EULER, RS_HLLE, true>(a, grid, threads, cpt, s)
-                 : launch_stage_sys<SYS_EULER, RS_HLLE, false>(a, grid, threads, cpt, s);
+    return KNC_DISPATCH(launch_stage_sys, a, grid, threads, s);
 }
 
 hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s)

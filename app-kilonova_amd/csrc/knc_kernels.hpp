@@ -31,10 +31,10 @@ struct StageArgs {
     int32_t pad;
 };
 
-hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, int cpt,
-                        hipStream_t s);
-bool stage_variant_supported(int threads, int cpt);
-int stage_occupancy(int system, int rs, bool stats, int threads, int cpt);
+hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, hipStream_t s);
+bool stage_variant_supported(int threads);
+int stage_occupancy(int system, int rs, bool stats, int threads);
+int stage_max_rows();
 hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s);
 hipError_t launch_dt_finish(unsigned long long* min_bits, double cfl, double* dt_out, hipStream_t s);
 hipError_t launch_dt_reset(unsigned long long* min_bits, hipStream_t s);

@@ -45,7 +45,7 @@ class Options(ctypes.Structure):
     _fields_ = [("launch_mode", ctypes.c_int32), ("c2p_guess", ctypes.c_int32),
                 ("collect_stats", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("stage_threads", ctypes.c_int32), ("columns_per_thread", ctypes.c_int32)]
+                ("stage_threads", ctypes.c_int32)]
 
 
 class Error(ctypes.Structure):
@@ -146,10 +146,10 @@ class Context:
     """A KNC ``State`` resident on one GPU plus the scheme that advances it."""
 
     def __init__(self, hydro, launch_mode=PER_BLOCK, c2p_guess=GUESS_PLAIN, collect_stats=False,
-                 rows_per_item=0, device=0, use_graph=True, stage_threads=0, columns_per_thread=0):
+                 rows_per_item=0, device=0, use_graph=True, stage_threads=0):
         self.hydro = hydro
         self.options = Options(launch_mode, c2p_guess, int(collect_stats), rows_per_item, device, int(use_graph),
-                               stage_threads, columns_per_thread)
+                               stage_threads)
         h = ctypes.c_void_p()
         _check(lib().knc_context_create(ctypes.byref(hydro), ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h

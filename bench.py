@@ -111,7 +111,6 @@ def main():
     ap.add_argument("--nr", type=int, default=4096)
     ap.add_argument("--nq", type=int, default=1024)
     ap.add_argument("--threads", type=int, default=0, help="stage workgroup size (0: library choice)")
-    ap.add_argument("--cpt", type=int, default=0, help="polar columns per thread (0: library choice)")
     ap.add_argument("--rows", type=int, default=0, help="radial rows per work item (0: library choice)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -125,7 +124,7 @@ def main():
     log(f"bench: bomb_jet {args.nr}x{args.nq} zones={zones} blocks={len(setup.blocks)} "
         f"steps={args.steps} warmup={args.warmup} fold={args.fold}")
 
-    shape = dict(stage_threads=args.threads, columns_per_thread=args.cpt, rows_per_item=args.rows)
+    shape = dict(stage_threads=args.threads, rows_per_item=args.rows)
     ctx_b, wall_b, stage_ms_b = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, **shape)
     rate_b = zones * args.steps / wall_b
     log(f"bench: batched {rate_b / 1e9:.3f} Gzu/s, stage {stage_ms_b:.4f} ms")

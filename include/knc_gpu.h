@@ -80,11 +80,11 @@ typedef struct knc_options {
                                 over every block, fold steps replayed as a hipGraph */
     int32_t c2p_guess;       /* KNC_GUESS_PLAIN | KNC_GUESS_LEARNED */
     int32_t collect_stats;   /* 1: Newton iteration histogram and fallback counts */
-    int32_t rows_per_item;   /* radial rows per workgroup work item; 0 = autotune */
+    int32_t rows_per_item;   /* radial rows per work item (<= 64); 0 = chosen from the mesh */
     int32_t device;          /* HIP device ordinal */
     int32_t use_graph;       /* batched mode: 1 = capture fold steps in a hipGraph */
-    int32_t stage_threads;   /* stage workgroup size (64..1024); 0 = chosen from nq */
-    int32_t columns_per_thread; /* polar columns per stage thread (1 or 2); 0 = chosen from nq */
+    int32_t stage_threads;   /* stage workgroup size = polar columns per work item
+                                (64, 128, 256, 512); 0 = chosen from nq */
 } knc_options;
 
 /* HydroError (src/physics/mod.rs:109-117): code, the failing zone and its

@@ -150,3 +150,22 @@ def test_negative_density_is_reported_with_position():
         sim.advance(sim.time_step(), 1)
     assert (eo.value.block, eo.value.i, eo.value.j) == (1, 5, 7)
     assert e.value.r == eo.value.r and e.value.theta == eo.value.theta
+
+
+@pytest.mark.parametrize("nq,threads", [(256, 64), (100, 64), (65, 64), (130, 128)])
+def test_polar_tiles_match_untiled_and_oracle(nq, threads):
+    """Work items narrower than the mesh (halo columns recovered per item)
+    give the same bits as one item spanning every column, and as the oracle."""
+    s = knc.setups.bomb_jet(nr=96, nq=nq, nblocks=3, nshell=4)
+    tiled = knc.context_from_setup(s, launch_mode=knc.BATCHED, stage_threads=threads, rows_per_item=7)
+    whole = knc.context_from_setup(s, launch_mode=knc.PER_BLOCK, stage_threads=512)
+    sim = oracle_from_setup(s)
+    for _ in range(2):
+        dt = sim.time_step()
+        sim.advance(dt, 3)
+        assert tiled.advance(3) == dt
+        assert whole.advance(3) == dt
+    st, sw = gpu_state(tiled), gpu_state(whole)
+    for (x, m), (y, n) in zip(st, sw):
+        assert np.array_equal(x, y) and np.array_equal(m, n)
+    assert max_rel_l1(st, oracle_state(sim, s.blocks)) <= ONE_STEP
