@@ -24,6 +24,7 @@
 #include "../../include/knc_gpu.h"
 #include "knc_device.hpp"
 #include "knc_kernels.hpp"
+#include "knc_mlp_host.hpp"
 #include "knc_zone.hpp"
 
 using namespace knc;
@@ -161,6 +162,11 @@ struct knc_context {
     // graphs keyed by (starting slot, fold)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
 
+    // learned c2p guess: device copy of the network's operand fragments
+    void* d_mlp = nullptr;
+    MlpDev mlp{};
+    bool has_mlp = false;
+
     // timing
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -169,6 +175,8 @@ struct knc_context {
 };
 
 namespace {
+
+bool learned(const knc_context* c) { return c->o.c2p_guess == KNC_GUESS_LEARNED && c->h.system == KNC_SRHD; }
 
 void drop_graphs(knc_context* c)
 {
@@ -238,7 +246,8 @@ int choose_rows(knc_context* c)
     int64_t rows = 0;
     for (auto& kv : c->blocks) rows += kv.second.nr;
     const int tiles = (c->nq + c->threads - 1) / c->threads;
-    const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, c->threads));
+    const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, learned(c),
+                                                c->threads));
     const int64_t want = (int64_t)c->num_cu * occ;
     int64_t r = rows * tiles / std::max<int64_t>(1, want);
     return (int)std::max<int64_t>(4, std::min<int64_t>(32, r));
@@ -308,7 +317,7 @@ int rebuild_tables(knc_context* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     // Persistent grid: every CU holds as many stage workgroups as fit.
     const int per_cu = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
-                                                   c->threads));
+                                                   learned(c), c->threads));
     c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
     c->dirty = false;
     return KNC_OK;
@@ -336,6 +345,7 @@ StageArgs base_args(knc_context* c)
     a.err = c->d_err;
     a.stats = c->d_stats;
     a.step = 0;
+    a.mlp = c->mlp;
     return a;
 }
 
@@ -359,14 +369,15 @@ int launch_one_stage(knc_context* c, StageArgs a)
     if (c->o.launch_mode == KNC_LAUNCH_BATCHED) {
         a.items = c->d_items;
         a.nitems_ptr = c->d_nitems;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, c->persistent_grid, c->threads, c->stream));
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, learned(c), a, c->persistent_grid, c->threads,
+                             c->stream));
         c->last_launches += 1;
     } else {
         for (auto& bi : c->block_items) {
             a.items = c->d_items + bi.first;
             a.nitems_ptr = nullptr;
             a.nitems = bi.second;
-            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, a, bi.second, c->threads, c->stream));
+            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, learned(c), a, bi.second, c->threads, c->stream));
             c->last_launches += 1;
         }
     }
@@ -486,6 +497,8 @@ int require_blocks(knc_context* c)
 {
     if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
     if (c->blocks.empty()) return fail(KNC_ERR_NO_BLOCK, "no blocks");
+    if (learned(c) && !c->has_mlp)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "c2p_guess is KNC_GUESS_LEARNED but no network is attached");
     return rebuild_tables(c);
 }
 
@@ -588,9 +601,9 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
         c->o.launch_mode = KNC_LAUNCH_PER_BLOCK;
         c->o.use_graph = 0;
     }
-    if (c->o.c2p_guess != KNC_GUESS_PLAIN) {
+    if (c->o.c2p_guess != KNC_GUESS_PLAIN && c->o.c2p_guess != KNC_GUESS_LEARNED) {
         delete c;
-        return fail(KNC_ERR_UNSUPPORTED, "learned c2p guess requires trained weights (knc_mlp_*)");
+        return fail(KNC_ERR_INVALID_ARGUMENT, "unknown c2p guess mode");
     }
     c->zp = zone_params(c->h);
     int ndev = 0;
@@ -636,7 +649,7 @@ void knc_context_destroy(knc_context* c)
         (void)hipFree(kv.second.rgeo);
     }
     void* ptrs[] = {c->d_theta, c->d_bnd, c->d_blocks, c->d_items, c->d_nitems, c->d_err, c->d_stats,
-                    c->d_min_bits, c->d_dt};
+                    c->d_min_bits, c->d_dt, c->d_mlp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -804,6 +817,27 @@ int knc_advance(knc_context* c, int32_t fold, double* dt_used, knc_error* err)
 {
     if (fold < 1) return fail(KNC_ERR_INVALID_ARGUMENT, "fold must be >= 1");
     return run_steps(c, true, 0.0, fold, dt_used, err);
+}
+
+int knc_context_set_mlp(knc_context* c, const knc_mlp* mm)
+{
+    const MlpHost* m = reinterpret_cast<const MlpHost*>(mm);
+    if (!c || !m) return fail(KNC_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t bytes = MlpHost::frag_bytes();
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->d_mlp) HIP_TRY(hipMalloc(&c->d_mlp, bytes));
+    HIP_TRY(hipMemcpy(c->d_mlp, m->d_frag, bytes, hipMemcpyDeviceToDevice));
+    const char* src = (const char*)m->d_frag;
+    char* dst = (char*)c->d_mlp;
+    c->mlp = m->dev;
+    c->mlp.a1 = (const bf16x8*)(dst + ((const char*)m->dev.a1 - src));
+    c->mlp.a2 = (const bf16x8*)(dst + ((const char*)m->dev.a2 - src));
+    c->mlp.c1 = (const floatx16*)(dst + ((const char*)m->dev.c1 - src));
+    c->mlp.c2 = (const floatx16*)(dst + ((const char*)m->dev.c2 - src));
+    c->mlp.w3 = (const floatx16*)(dst + ((const char*)m->dev.w3 - src));
+    c->has_mlp = true;
+    drop_graphs(c);
+    return KNC_OK;
 }
 
 int knc_get_c2p_stats(knc_context* c, knc_c2p_stats* out)

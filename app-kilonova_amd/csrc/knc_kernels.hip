@@ -20,6 +20,7 @@
 
 #include "knc_device.hpp"
 #include "knc_kernels.hpp"
+#include "knc_mlp.hpp"
 #include "knc_zone.hpp"
 
 namespace knc {
@@ -48,10 +49,10 @@ __device__ inline void record_error(ErrLog* log, int step, int order, int row, i
 // c2p of one zone (BlockState::try_to_primitive, src/state.rs:76-94, and the
 // scalar concentration s = M/dV/Gamma of src/scheme.rs:35).  u is the
 // volume-integrated state; rdv the reciprocal volume factor.
-template <int SYS, bool STATS>
+template <int SYS, bool STATS, bool GUESS>
 __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, double usq, double utau,
                                     double um, double rdv, int order, int row, int col, int block_index,
-                                    unsigned int* lds_hist)
+                                    unsigned int* lds_hist, double guess)
 {
     const Cons u{ud * rdv, usr * rdv, usq * rdv, utau * rdv};
     PS out;
@@ -70,7 +71,22 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
         } else {
             const double S2 = u.sr * u.sr + u.sq * u.sq;
             double p = 0.0;
-            code = c2p_srhd_newton(u.d, S2, u.tau, a.zp, 0.0, &p, &it);
+            if (GUESS) {
+                // Newton from the learned guess; on failure restart from the
+                // plain guess (p = 0) and count the fallback.
+                code = c2p_srhd_newton(u.d, S2, u.tau, a.zp, guess, &p, &it);
+                if (code != E_OK) {
+                    int it2 = 0;
+                    code = c2p_srhd_newton(u.d, S2, u.tau, a.zp, 0.0, &p, &it2);
+                    if (STATS) {
+                        atomicAdd(&a.stats->fallbacks, 1ull);
+                        atomicAdd(&a.stats->learned_iters, (unsigned long long)it2);
+                    }
+                    it += it2;
+                }
+            } else {
+                code = c2p_srhd_newton(u.d, S2, u.tau, a.zp, 0.0, &p, &it);
+            }
             if (code == E_OK) {
                 code = c2p_srhd_finish(u, S2, p, &out.P, &rW);
                 if (code == E_NEGATIVE_PRESSURE) {
@@ -163,7 +179,9 @@ __device__ inline Raw5 load_raw(const RowSrc& r, int j)
     return x;
 }
 
-template <int SYS, bool STATS>
+// Primitive of a loaded row value.  Must be reached by every lane of the
+// wavefront (the learned guess is a wavefront-wide MFMA evaluation).
+template <int SYS, bool STATS, bool GUESS>
 __device__ inline PS raw_primitive(const StageArgs& a, const RowSrc& r, const Raw5& x, int j, unsigned int* lds_hist)
 {
     if (r.fixed) {
@@ -173,8 +191,13 @@ __device__ inline PS raw_primitive(const StageArgs& a, const RowSrc& r, const Ra
         return q;
     }
     const double rdv = r.rvolA * a.th.rdcost[j];
-    return zone_primitive<SYS, STATS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], rdv, r.order, r.row, j, r.index,
-                                      lds_hist);
+    double guess = 0.0;
+    if (GUESS) {
+        const double D = x.v[0] * rdv, sr = x.v[1] * rdv, sq = x.v[2] * rdv, tau = x.v[3] * rdv;
+        guess = mlp_guess_wave(a.mlp, D, sr * sr + sq * sq, tau, a.zp.gamma);
+    }
+    return zone_primitive<SYS, STATS, GUESS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], rdv, r.order, r.row, j,
+                                             r.index, lds_hist, guess);
 }
 
 __device__ inline void lds_put(double* L, int W, int j, const PS& q)
@@ -212,14 +235,14 @@ __device__ inline Flux5 scaled(const Cons& F, double Fs, double c, double A)
 constexpr int MAX_ROWS = 64;
 constexpr int NHALO = 6;   // j0-2, j0-1 | j1-2, j1-1, j1, j1+1
 
-template <int SYS, int RS, bool STATS>
+template <int SYS, bool STATS, bool GUESS>
 __device__ inline PS column_primitive(const StageArgs& a, const BlockDesc& B, int row, int col, unsigned int* lds_hist)
 {
     const RowSrc src = block_row(a, B, row);
-    return raw_primitive<SYS, STATS>(a, src, load_raw(src, col), col, lds_hist);
+    return raw_primitive<SYS, STATS, GUESS>(a, src, load_raw(src, col), col, lds_hist);
 }
 
-template <int SYS, int RS, bool STATS, int NT>
+template <int SYS, int RS, bool STATS, int NT, bool GUESS>
 __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 {
     constexpr int WP = NT + 3;                 // primitive row: 2 left halo, NT own, 1 right halo
@@ -261,14 +284,19 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
         if (left_halo || right_halo) {
             // Halo primitives of this item's rows, then the right-face fluxes.
             __syncthreads();
-            for (int z = tid; z < NHALO * nrow; z += NT) {
-                const int h = z % NHALO, rr = z / NHALO;
+            // uniform trip count: every lane reaches raw_primitive together
+            for (int z0 = 0; z0 < NHALO * nrow; z0 += NT) {
+                const int z = z0 + tid;
+                const int zz = z < NHALO * nrow ? z : 0;
+                const int h = zz % NHALO, rr = zz / NHALO;
                 const int col = h < 2 ? j0 - 2 + h : j1 - 4 + h;
-                const bool need = h < 2 ? left_halo : right_halo;
-                PS q = zero;
-                if (need && col >= 0 && col < nq) q = column_primitive<SYS, RS, STATS>(a, B, it.i0 + rr, col, lds_hist);
-                double* d = ldsH + (h * MAX_ROWS + rr) * 5;
-                d[0] = q.P.rho; d[1] = q.P.ur; d[2] = q.P.uq; d[3] = q.P.p; d[4] = q.s;
+                const bool need = z < NHALO * nrow && (h < 2 ? left_halo : right_halo) && col >= 0 && col < nq;
+                const PS q0 = column_primitive<SYS, STATS, GUESS>(a, B, it.i0 + rr, need ? col : j, lds_hist);
+                if (z < NHALO * nrow) {
+                    const PS q = need ? q0 : zero;
+                    double* d = ldsH + (h * MAX_ROWS + rr) * 5;
+                    d[0] = q.P.rho; d[1] = q.P.ur; d[2] = q.P.uq; d[3] = q.P.p; d[4] = q.s;
+                }
             }
             __syncthreads();
             if (right_halo) {
@@ -317,7 +345,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     for (int f = 0; f < 5; ++f) u0.v[f] = U0[o + f * np];
                 }
             }
-            const PS Pk = raw_primitive<SYS, STATS>(a, src, cur, j, lds_hist);
+            const PS Pk = raw_primitive<SYS, STATS, GUESS>(a, src, cur, j, lds_hist);
             Flux5 Fk;
             if (k >= it.i0) {
                 const PS g = plm5(th, Pm2, Pm1, Pk);                 // gx[k-1]
@@ -451,8 +479,8 @@ __global__ void __launch_bounds__(256) k_dt(StageArgs a, unsigned long long* min
             const int j = z % nq;
             const double* s = slot_of(B, a.in_slot) + (int64_t)i * nq + j;
             const double rdv = rg_rvolA(B)[i] * a.th.rdcost[j];
-            const PS q = zone_primitive<SYS, false>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, B.order, i, j,
-                                                    B.index, nullptr);
+            const PS q = zone_primitive<SYS, false, false>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, B.order,
+                                                           i, j, B.index, nullptr, 0.0);
             const double dr = rg_dr(B)[i];
             const double dl = dmin(dr, a.th.dq[j] * rg_rv(B)[i]);
             m = dmin(m, dl / (max_signal_speed<SYS>(q.P, a.zp) * a.zp.c));
@@ -590,29 +618,29 @@ static int grid_for(int64_t n, int block)
 }
 
 // Stage kernel variants by workgroup size.
-template <int SYS, int RS, bool ST>
-static hipError_t launch_stage_sys(const StageArgs& a, int grid, int threads, hipStream_t s)
+template <int SYS, int RS, bool ST, bool G>
+static hipError_t launch_stage_v(const StageArgs& a, int grid, int threads, hipStream_t s)
 {
     switch (threads) {
-    case 64: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 64>), dim3(grid), dim3(64), 0, s, a); break;
-    case 128: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 128>), dim3(grid), dim3(128), 0, s, a); break;
-    case 256: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256>), dim3(grid), dim3(256), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512>), dim3(grid), dim3(512), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 64, G>), dim3(grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 128, G>), dim3(grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256, G>), dim3(grid), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512, G>), dim3(grid), dim3(512), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int SYS, int RS, bool ST>
-static int occupancy_sys(int threads)
+template <int SYS, int RS, bool ST, bool G>
+static int occupancy_v(int threads)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (threads) {
-    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 64>, 64, 0); break;
-    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 128>, 128, 0); break;
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 256>, 256, 0); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 512>, 512, 0); break;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 64, G>, 64, 0); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 128, G>, 128, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 256, G>, 256, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 512, G>, 512, 0); break;
     default: break;
     }
     return e == hipSuccess ? n : 0;
@@ -625,25 +653,32 @@ bool stage_variant_supported(int threads)
 
 int stage_max_rows() { return MAX_ROWS; }
 
-#define KNC_DISPATCH(FN, ...)                                                                          \
-    (system == SYS_SRHD                                                                                \
-         ? (rs == RS_HLLC ? (stats ? FN<SYS_SRHD, RS_HLLC, true>(__VA_ARGS__)                           \
-                                   : FN<SYS_SRHD, RS_HLLC, false>(__VA_ARGS__))                         \
-                          : (stats ? FN<SYS_SRHD, RS_HLLE, true>(__VA_ARGS__)                           \
-                                   : FN<SYS_SRHD, RS_HLLE, false>(__VA_ARGS__)))                        \
-         : (rs == RS_HLLC ? (stats ? FN<SYS_EULER, RS_HLLC, true>(__VA_ARGS__)                          \
-                                   : FN<SYS_EULER, RS_HLLC, false>(__VA_ARGS__))                        \
-                          : (stats ? FN<SYS_EULER, RS_HLLE, true>(__VA_ARGS__)                          \
-                                   : FN<SYS_EULER, RS_HLLE, false>(__VA_ARGS__))))
+// Dispatch on (system, solver, stats, learned); the learned guess exists for
+// SRHD only (the Euler c2p is closed form).
+#define KNC_DISPATCH(FN, ...)                                                                              \
+    (system == SYS_SRHD                                                                                    \
+         ? (learned ? (rs == RS_HLLC ? (stats ? FN<SYS_SRHD, RS_HLLC, true, true>(__VA_ARGS__)             \
+                                              : FN<SYS_SRHD, RS_HLLC, false, true>(__VA_ARGS__))           \
+                                     : (stats ? FN<SYS_SRHD, RS_HLLE, true, true>(__VA_ARGS__)             \
+                                              : FN<SYS_SRHD, RS_HLLE, false, true>(__VA_ARGS__)))          \
+                    : (rs == RS_HLLC ? (stats ? FN<SYS_SRHD, RS_HLLC, true, false>(__VA_ARGS__)            \
+                                              : FN<SYS_SRHD, RS_HLLC, false, false>(__VA_ARGS__))          \
+                                     : (stats ? FN<SYS_SRHD, RS_HLLE, true, false>(__VA_ARGS__)            \
+                                              : FN<SYS_SRHD, RS_HLLE, false, false>(__VA_ARGS__))))        \
+         : (rs == RS_HLLC ? (stats ? FN<SYS_EULER, RS_HLLC, true, false>(__VA_ARGS__)                      \
+                                   : FN<SYS_EULER, RS_HLLC, false, false>(__VA_ARGS__))                    \
+                          : (stats ? FN<SYS_EULER, RS_HLLE, true, false>(__VA_ARGS__)                      \
+                                   : FN<SYS_EULER, RS_HLLE, false, false>(__VA_ARGS__))))
 
-int stage_occupancy(int system, int rs, bool stats, int threads)
+int stage_occupancy(int system, int rs, bool stats, bool learned, int threads)
 {
-    return KNC_DISPATCH(occupancy_sys, threads);
+    return KNC_DISPATCH(occupancy_v, threads);
 }
 
-hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, hipStream_t s)
+hipError_t launch_stage(int system, int rs, bool stats, bool learned, const StageArgs& a, int grid, int threads,
+                        hipStream_t s)
 {
-    return KNC_DISPATCH(launch_stage_sys, a, grid, threads, s);
+    return KNC_DISPATCH(launch_stage_v, a, grid, threads, s);
 }
 
 hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s)

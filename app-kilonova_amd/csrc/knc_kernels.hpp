@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "knc_device.hpp"
+#include "knc_mlp.hpp"
 #include "knc_zone.hpp"
 
 namespace knc {
@@ -29,11 +30,13 @@ struct StageArgs {
     C2PStats* stats;
     int32_t step;                // error-key step number (0 = dt pass)
     int32_t pad;
+    MlpDev mlp;                  // learned c2p guess (learned-guess kernels only)
 };
 
-hipError_t launch_stage(int system, int rs, bool stats, const StageArgs& a, int grid, int threads, hipStream_t s);
+hipError_t launch_stage(int system, int rs, bool stats, bool learned, const StageArgs& a, int grid, int threads,
+                        hipStream_t s);
 bool stage_variant_supported(int threads);
-int stage_occupancy(int system, int rs, bool stats, int threads);
+int stage_occupancy(int system, int rs, bool stats, bool learned, int threads);
 int stage_max_rows();
 hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s);
 hipError_t launch_dt_finish(unsigned long long* min_bits, double cfl, double* dt_out, hipStream_t s);

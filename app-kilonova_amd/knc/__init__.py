@@ -7,7 +7,7 @@ from .capi import (BATCHED, EULER, FIXED, GUESS_LEARNED, GUESS_PLAIN, HLLC, HLLE
                    PER_BLOCK, POLAR, RADIAL, SRHD, Context, Hydro, HydroError, fp64_probe,
                    geometrical_source_terms, intercell_flux, lib, max_signal_speed, newtonian_hydro, plm_gradient,
                    relativistic_hydro, to_conserved, try_to_primitive)
-from . import setups
+from . import mlp, setups
 
 
 def hydro_from_spec(spec):

@@ -154,6 +154,7 @@ class Context:
         _check(lib().knc_context_create(ctypes.byref(hydro), ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h
         self.shapes = {}
+        self.vertices = {}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -177,10 +178,12 @@ class Context:
         sm = _f64(scalar_mass).reshape(nr, nq)
         _check(lib().knc_block_insert(self._h, int(index), nr, nq, _p(rv), _p(qv), _p(cons), _p(sm)))
         self.shapes[int(index)] = (nr, nq)
+        self.vertices[int(index)] = (rv.copy(), qv.copy())
 
     def remove_block(self, index):
         _check(lib().knc_block_remove(self._h, int(index)))
         self.shapes.pop(int(index), None)
+        self.vertices.pop(int(index), None)
 
     def download_block(self, index):
         nr, nq = self.shapes[int(index)]

@@ -180,6 +180,39 @@ int knc_hydro_max_signal_speed(const knc_hydro* h, int64_t n, const double* prim
 int knc_hydro_plm_gradient(double plm_theta, int64_t n, const double* a, const double* b, const double* c,
                            double* out);
 
+/* ---- learned c2p guess (no counterpart in the reference) --------------- */
+/* A 4 -> 64 -> 64 -> 1 tanh MLP mapping log2-scaled features of a conserved
+ * state (D, |S|/D, thermal part of tau/D, gamma) to log2(p/D), used as the
+ * initial pressure of the SRHD Newton recovery when a context's c2p_guess is
+ * KNC_GUESS_LEARNED; Newton always refines it and restarts from the plain
+ * guess if it does not converge.  Parameters: 4545 FP32 values
+ * W1[64][4] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3; norm: 10 FP32
+ * values shift[4] | scale[4] | tshift | tscale (x = (f - shift) * scale,
+ * target = (log2(p/D) - tshift) * tscale). */
+typedef struct knc_mlp knc_mlp;
+const char* knc_mlp_last_error_message(void);
+int knc_mlp_num_params(void);
+int knc_mlp_create(int32_t device, knc_mlp** out);
+void knc_mlp_destroy(knc_mlp* mlp);
+int knc_mlp_init(knc_mlp* mlp, uint64_t seed);
+int knc_mlp_set_params(knc_mlp* mlp, const float* params, const float* norm);
+int knc_mlp_get_params(knc_mlp* mlp, float* params, float* norm);
+/* Mean squared error and its gradient over n normalised samples
+ * (x: n*4, t: n), computed by the training kernels. */
+int knc_mlp_loss_grad(knc_mlp* mlp, int64_t n, const float* x, const float* t, float* loss, float* grad);
+/* Adam training on the GPU; epoch_loss (may be NULL) gets one value per epoch. */
+int knc_mlp_train(knc_mlp* mlp, int64_t n, const float* x, const float* t, int32_t epochs, int32_t batch, float lr,
+                  uint64_t seed, float* epoch_loss);
+/* Raw (unnormalised) features of per-volume conserved states, n*4. */
+int knc_mlp_features(int64_t n, const double* conserved, const double* gamma, float* features);
+/* BF16-MFMA inference: normalised output y (may be NULL) and the pressure
+ * guess (may be NULL). */
+int knc_mlp_predict(knc_mlp* mlp, int64_t n, const double* conserved, const double* gamma, float* y, double* guess);
+/* Attaches a trained network to a context (copied; the network may be
+ * destroyed afterwards).  Required before advancing a context created with
+ * c2p_guess = KNC_GUESS_LEARNED. */
+int knc_context_set_mlp(knc_context* ctx, const knc_mlp* mlp);
+
 /* FP64 arithmetic probe of the device (a/b, sqrt|a|, a*b+c with two
  * roundings), used to check that device arithmetic rounds like the host's. */
 int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sqrt_abs_a,

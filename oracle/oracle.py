@@ -260,3 +260,24 @@ class Sim:
         z, it, fl = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
         lib().ko_sim_c2p_stats(self._p, ctypes.byref(z), ctypes.byref(it), ctypes.byref(fl))
         return z.value, it.value, fl.value
+
+
+# ---- learned-guess MLP (FP64 reference) -------------------------------------
+def mlp_forward(params, x):
+    """y = W3 tanh(W2 tanh(W1 x + b1) + b2) + b3 in FP64 (knc_mlp_oracle.cpp)."""
+    P = _f64(params)
+    x = _f64(x).reshape(-1, 4)
+    y = np.zeros(x.shape[0])
+    lib().ko_mlp_forward(_p(P), ctypes.c_int64(x.shape[0]), _p(x), _p(y))
+    return y
+
+
+def mlp_loss_grad(params, x, t):
+    """Mean squared error and its gradient in FP64."""
+    P = _f64(params)
+    x = _f64(x).reshape(-1, 4)
+    t = _f64(t).reshape(-1)
+    loss = ctypes.c_double()
+    grad = np.zeros(P.size)
+    lib().ko_mlp_loss_grad(_p(P), ctypes.c_int64(x.shape[0]), _p(x), _p(t), ctypes.byref(loss), _p(grad))
+    return loss.value, grad
