@@ -55,9 +55,27 @@ __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b 
 //   f = (gamma-1)(rho h - rho - p) - p,  f' = v^2 c_s^2 - 1,  c_s^2 = gamma p/(rho h).
 // With F = Q f = (gamma-1)(den - D sqrt(den) - pQ) - pQ the step is
 // dp = F den / (gamma p S^2 - den Q): one division per iteration.
-// Converged when |dp| <= 1e-12 (tau + D + p).  Error mapping follows
-// relativistic_hydro.rs:68-88.
+// Converged when |dp| <= 1e-12 (tau + D + p), then c2p_srhd_polish.  Error
+// mapping follows relativistic_hydro.rs:68-88.
 // ---------------------------------------------------------------------------
+// Canonical final step: round the converged iterate to the power-of-two grid
+// 2^(ilogb(tau + D) - 33), which depends on the state only, and take one more
+// Newton step, so the recovered pressure does not depend on the starting
+// guess (the iterate is only defined to ~eps (tau + D + p)).
+__device__ inline double c2p_srhd_polish(double D, double S2, double E, const ZoneParams& zp, double p)
+{
+    if (!(E > 0.0)) return p;
+    const int e = ilogb(E) - 33;
+    p = ldexp(rint(ldexp(p, -e)), e);
+    const double Q = E + p;
+    const double den = Q * Q - S2;
+    if (!(den > 0.0)) return p;
+    const double sq = sqrt(den);
+    const double pQ = p * Q;
+    const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
+    return p - F * den / (zp.gamma * p * S2 - den * Q);
+}
+
 __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const ZoneParams& zp,
                                       double p, double* p_out, int* iters)
 {
@@ -72,7 +90,11 @@ __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const Zon
         const double dp = F * den / (zp.gamma * p * S2 - den * Q);
         p = p - dp;
         if (!(fabs(p) < 1e300)) { *iters = n; return E_ROOT_FINDER_FAILED; }
-        if (fabs(dp) <= C2P_TOL * (E + p)) { *iters = n; *p_out = p; return E_OK; }
+        if (fabs(dp) <= C2P_TOL * (E + p)) {
+            *iters = n + 1;
+            *p_out = c2p_srhd_polish(D, S2, E, zp, p);
+            return E_OK;
+        }
     }
     *iters = C2P_MAX_ITER;
     return E_ROOT_FINDER_FAILED;

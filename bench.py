@@ -8,11 +8,15 @@ reduction run once per fold of 10 steps.  A step is one full RK2 step
 (src/main.rs:42-48).
 
 Paths timed on the same initial data:
-  plain    one stage launch per block per stage (KNC_LAUNCH_PER_BLOCK)
+  plain    one stage launch per block per stage (KNC_LAUNCH_PER_BLOCK),
+           plain c2p guess
   batched  one persistent launch per stage over every block, the fold's
-           steps replayed as a hipGraph (KNC_LAUNCH_BATCHED)
-The reported value is the batched path; the plain path's rate and the
-rel-L1 between the two final states are reported beside it.
+           steps replayed as a hipGraph (KNC_LAUNCH_BATCHED), plain guess
+  learned  batched + the learned c2p pressure guess (BF16-MFMA network,
+           weights from knc/guess_net.npz, trained by knc.train_guess)
+The reported value is the learned + batched path (batched when no trained
+weights exist); the other rates, the rel-L1 of every final state against
+the plain path, and Newton-iteration statistics are reported beside it.
 
 Roofline of the stage kernel: algorithmic HBM bytes per zone-update are
 40 B (5 FP64 fields) read + 40 B written on stage 1 and 80 B read + 40 B
@@ -44,9 +48,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(setup, launch_mode, steps, warmup, fold, **shape):
+def run_gpu(setup, launch_mode, steps, warmup, fold, net=None, **shape):
     import knc
-    ctx = knc.context_from_setup(setup, launch_mode=launch_mode, use_graph=True, **shape)
+    guess = knc.GUESS_LEARNED if net is not None else knc.GUESS_PLAIN
+    ctx = knc.context_from_setup(setup, launch_mode=launch_mode, use_graph=True, c2p_guess=guess, **shape)
+    if net is not None:
+        net.attach(ctx)
     ctx.set_timing(True)
     done = 0
     while done < warmup:
@@ -82,6 +89,29 @@ def cpu_baseline(setup, nsteps, nthreads):
     return setup.total_zones * nsteps / wall, wall
 
 
+def newton_stats(setup, fold, net, steps=20):
+    """Newton iteration histogram and fallback rate of the plain and learned
+    guesses over a short instrumented run (not timed)."""
+    import knc
+    out = {}
+    for name, n in (("plain", None), ("learned", net)):
+        if name == "learned" and n is None:
+            continue
+        guess = knc.GUESS_LEARNED if n is not None else knc.GUESS_PLAIN
+        ctx = knc.context_from_setup(setup, launch_mode=knc.BATCHED, c2p_guess=guess, collect_stats=True)
+        if n is not None:
+            n.attach(ctx)
+        for _ in range(steps // fold):
+            ctx.advance(fold)
+        st = ctx.c2p_stats()
+        h = st["hist"]
+        z = max(1, int(h.sum()))
+        out[name] = {"mean_iterations": float((h * np.arange(h.size)).sum() / z),
+                     "histogram": {int(k): int(v) for k, v in enumerate(h) if v},
+                     "fallback_rate": st["fallbacks"] / z, "floors": st["floors"], "zones": z}
+    return out
+
+
 def read_traffic():
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary,
     or None."""
@@ -112,6 +142,8 @@ def main():
     ap.add_argument("--nq", type=int, default=1024)
     ap.add_argument("--threads", type=int, default=0, help="stage workgroup size (0: library choice)")
     ap.add_argument("--rows", type=int, default=0, help="radial rows per work item (0: library choice)")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-stats", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; multi-GPU sharding of the "
@@ -129,17 +161,38 @@ def main():
     rate_b = zones * args.steps / wall_b
     log(f"bench: batched {rate_b / 1e9:.3f} Gzu/s, stage {stage_ms_b:.4f} ms")
 
+    net = None
+    if not args.no_learned:
+        from knc import train_guess
+        try:
+            net = train_guess.load()
+        except RuntimeError as e:
+            log(f"bench: learned path skipped: {e}")
+    learned = None
+    ctx_l = None
+    if net is not None:
+        ctx_l, wall_l, stage_ms_l = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, net=net, **shape)
+        rate_l = zones * args.steps / wall_l
+        log(f"bench: learned+batched {rate_l / 1e9:.3f} Gzu/s, stage {stage_ms_l:.4f} ms")
+        learned = {"value": rate_l, "ms_per_step": 1e3 * wall_l / args.steps, "stage_ms": stage_ms_l}
+
     plain = None
-    parity = None
+    parity = {}
     if not args.no_plain:
         ctx_p, wall_p, stage_ms_p = run_gpu(setup, knc.PER_BLOCK, args.steps, args.warmup, args.fold)
         rate_p = zones * args.steps / wall_p
         log(f"bench: plain {rate_p / 1e9:.3f} Gzu/s, stage {stage_ms_p:.4f} ms")
         from util import gpu_state, rel_l1  # noqa: E402
-        parity = rel_l1(gpu_state(ctx_b), gpu_state(ctx_p))
+        ref = gpu_state(ctx_p)
+        parity["batched_vs_plain"] = rel_l1(gpu_state(ctx_b), ref)
+        if ctx_l is not None:
+            parity["learned_batched_vs_plain"] = rel_l1(gpu_state(ctx_l), ref)
         plain = {"value": rate_p, "ms_per_step": 1e3 * wall_p / args.steps, "stage_ms": stage_ms_p}
 
-    achieved = BYTES_PER_ZONE_STAGE * zones / (stage_ms_b * 1e-3) / 1e9
+    newton = newton_stats(setup, args.fold, net) if not args.no_stats else None
+
+    head_rate, head_stage = (learned["value"], learned["stage_ms"]) if learned else (rate_b, stage_ms_b)
+    achieved = BYTES_PER_ZONE_STAGE * zones / (head_stage * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
         ncores = min(16, os.cpu_count() or 1)
@@ -151,12 +204,12 @@ def main():
 
     out = {
         "metric": "FP64 zone-updates/s (RK2 step of the per-block stage)",
-        "value": rate_b,
+        "value": head_rate,
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall_b / args.steps,
+        "ms_per_step": 1e3 * zones / head_rate,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -164,14 +217,21 @@ def main():
         "data": "synthetic (seeded)",
         "config": {"workload": f"bomb_jet SRHD kinetic bomb + jet, {args.nr}x{args.nq} zones, "
                                f"{len(setup.blocks)} uneven blocks, HLLE, RK2, fold {args.fold}",
-                   "path": "batched persistent stage launches, hipGraph per fold, plain c2p guess",
+                   "path": ("learned c2p guess + batched persistent stage launches, hipGraph per fold"
+                            if learned else "batched persistent stage launches, hipGraph per fold, plain c2p guess"),
                    "zones": zones},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
         "cpu_baseline": cpu,
-        "plain_per_block": plain,
-        "batched_vs_plain_rel_l1": parity,
-        "stage_ms": stage_ms_b,
+        "paths": {"plain_per_block": plain,
+                  "batched": {"value": rate_b, "ms_per_step": 1e3 * wall_b / args.steps, "stage_ms": stage_ms_b},
+                  "learned_batched": learned},
+        "speedup_vs_plain": ({"batched": rate_b / plain["value"],
+                              "learned_batched": (learned["value"] / plain["value"]) if learned else None}
+                             if plain else None),
+        "rel_l1_vs_plain": parity or None,
+        "newton": newton,
+        "stage_ms": head_stage,
     }
     print(json.dumps(out), flush=True)
 

@@ -77,9 +77,30 @@ struct Cons { double d, sr, sq, tau; };   // SRHD (D,S_r,S_q,tau); Euler (rho,m_
 //   dp = f/f' = F den / (gamma p S^2 - den Q).
 // Converged when |dp| <= 1e-12 (tau + D + p): the rounding noise of f scales
 // with Q, so a test relative to p alone stalls for cold or Gamma~100 gas.
+// The converged iterate then goes through c2p_srhd_polish.
 // Error mapping follows relativistic_hydro.rs:68-88: D<0 and tau<0 are
 // errors, a negative recovered pressure is floored to 1e-3 rho.
 // ---------------------------------------------------------------------------
+// Canonical final step.  A converged iterate is only defined to within the
+// rounding noise of f, ~eps (tau + D + p); different starting guesses land at
+// different points of that band.  Rounding the iterate to the power-of-two
+// grid 2^(ilogb(tau + D) - 33) (which depends on the state only) and taking
+// one more Newton step makes the result a function of the state alone,
+// except when two iterates straddle a grid boundary.
+double c2p_srhd_polish(double D, double S2, double E, double gamma, double p)
+{
+    if (!(E > 0.0)) return p;
+    const int e = std::ilogb(E) - 33;
+    p = std::ldexp(std::rint(std::ldexp(p, -e)), e);
+    const double Q = E + p;
+    const double den = Q * Q - S2;
+    if (!(den > 0.0)) return p;
+    const double sq = std::sqrt(den);
+    const double pQ = p * Q;
+    const double F = (gamma - 1.0) * (den - D * sq - pQ) - pQ;
+    return p - F * den / (gamma * p * S2 - den * Q);
+}
+
 int c2p_srhd_newton(double D, double S2, double tau, double gamma, double p,
                     double* p_out, int* iters)
 {
@@ -96,7 +117,9 @@ int c2p_srhd_newton(double D, double S2, double tau, double gamma, double p,
         p = p - dp;
         if (!(std::fabs(p) < 1e300)) { *iters = n; return ROOT_FINDER_FAILED; }
         if (std::fabs(dp) <= C2P_TOL * (E + p)) {
-            *iters = n; *p_out = p; return OK;
+            *iters = n + 1;
+            *p_out = c2p_srhd_polish(D, S2, E, gamma, p);
+            return OK;
         }
     }
     *iters = C2P_MAX_ITER;
