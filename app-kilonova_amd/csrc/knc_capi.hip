@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -113,6 +114,8 @@ struct HostBlock {
     int nr = 0, nq = 0;
     double* mem = nullptr;      // NSLOT * 5 * nr * nq doubles
     double* rgeo = nullptr;     // device radial table
+    float* guess = nullptr;     // learned-guess plane, nr * nq floats
+    double* prim = nullptr;     // primitive planes of the two-pass stage, 5 * nr * nq
     std::vector<double> rv;
 };
 
@@ -156,7 +159,7 @@ struct knc_context {
     // per-call device state
     ErrLog* d_err = nullptr;
     C2PStats* d_stats = nullptr;
-    unsigned long long* d_min_bits = nullptr;
+    double* d_dt_partial = nullptr;       // per-workgroup minima of the CFL pass
     double* d_dt = nullptr;
 
     // graphs keyed by (starting slot, fold)
@@ -166,6 +169,11 @@ struct knc_context {
     void* d_mlp = nullptr;
     MlpDev mlp{};
     bool has_mlp = false;
+
+    // moving excision surfaces
+    bool exc_on = false;
+    knc_excision exc{};
+    int64_t exc_removed = 0, exc_added = 0;
 
     // timing
     bool timing = false;
@@ -177,6 +185,7 @@ struct knc_context {
 namespace {
 
 bool learned(const knc_context* c) { return c->o.c2p_guess == KNC_GUESS_LEARNED && c->h.system == KNC_SRHD; }
+bool two_pass(const knc_context* c) { return c->o.stage_passes == 2; }
 
 void drop_graphs(knc_context* c)
 {
@@ -247,7 +256,7 @@ int choose_rows(knc_context* c)
     for (auto& kv : c->blocks) rows += kv.second.nr;
     const int tiles = (c->nq + c->threads - 1) / c->threads;
     const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, learned(c),
-                                                c->threads));
+                                                two_pass(c), c->threads));
     const int64_t want = (int64_t)c->num_cu * occ;
     int64_t r = rows * tiles / std::max<int64_t>(1, want);
     return (int)std::max<int64_t>(4, std::min<int64_t>(32, r));
@@ -257,6 +266,8 @@ int choose_rows(knc_context* c)
 int rebuild_tables(knc_context* c)
 {
     if (!c->dirty) return KNC_OK;
+    // captured launches hold the grid size and work count of the old tables
+    drop_graphs(c);
     const int nb = (int)c->blocks.size();
     c->rows_per_item = choose_rows(c);
     std::vector<BlockDesc> desc;
@@ -273,6 +284,8 @@ int rebuild_tables(knc_context* c)
         const size_t plane = (size_t)5 * b.nr * b.nq;
         for (int s = 0; s < NSLOT; ++s) d.slot[s] = b.mem + s * plane;
         d.rgeo = b.rgeo;
+        d.guess = b.guess;
+        d.prim = b.prim;
         d.nr = b.nr;
         d.nq = b.nq;
         d.left = order > 0 ? order - 1 : -1;
@@ -317,7 +330,7 @@ int rebuild_tables(knc_context* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     // Persistent grid: every CU holds as many stage workgroups as fit.
     const int per_cu = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
-                                                   learned(c), c->threads));
+                                                   learned(c), two_pass(c), c->threads));
     c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
     c->dirty = false;
     return KNC_OK;
@@ -345,6 +358,7 @@ StageArgs base_args(knc_context* c)
     a.err = c->d_err;
     a.stats = c->d_stats;
     a.step = 0;
+    a.rows = c->rows_per_item;
     a.mlp = c->mlp;
     return a;
 }
@@ -362,24 +376,57 @@ __global__ void k_set_double(double* p, double v)
     if (threadIdx.x == 0) *p = v;
 }
 
-// Launches one stage over every block: per-block launches or one batched one.
+// Grid of a row-parallel pass over `nitems` work items: one workgroup per
+// (item, row) unit up to eight per CU, which then stride over the units.
+int row_pass_grid(const knc_context* c, int nitems)
+{
+    const int64_t units = (int64_t)nitems * c->rows_per_item;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(units, std::min(DT_MAX_GRID, 8 * c->num_cu)));
+}
+
+// Launches one stage over every block: per-block launches or one batched
+// one.  The passes before the stage kernels (the learned-guess pass, the c2p
+// pass of the two-pass stage) cover every block first, since a block's stage
+// also reads its neighbours' edge rows.
 int launch_one_stage(knc_context* c, StageArgs a)
 {
     const bool stats = c->o.collect_stats != 0;
+    const bool prims = two_pass(c);
+    const bool lg = learned(c);
+    auto pre_pass = [&](const StageArgs& b, int nitems) -> int {
+        const int grid = row_pass_grid(c, nitems);
+        if (lg) {
+            HIP_TRY(launch_guess(b, grid, c->threads, c->stream));
+            c->last_launches += 1;
+        }
+        if (prims) {
+            HIP_TRY(launch_prim(c->h.system, stats, lg, b, grid, c->threads, c->stream));
+            c->last_launches += 1;
+        }
+        return KNC_OK;
+    };
+    int rc;
     if (c->o.launch_mode == KNC_LAUNCH_BATCHED) {
         a.items = c->d_items;
         a.nitems_ptr = c->d_nitems;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, learned(c), a, c->persistent_grid, c->threads,
+        if ((rc = pre_pass(a, c->nitems))) return rc;
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, prims, a, c->persistent_grid, c->threads,
                              c->stream));
         c->last_launches += 1;
-    } else {
-        for (auto& bi : c->block_items) {
-            a.items = c->d_items + bi.first;
-            a.nitems_ptr = nullptr;
-            a.nitems = bi.second;
-            HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, learned(c), a, bi.second, c->threads, c->stream));
-            c->last_launches += 1;
-        }
+        return KNC_OK;
+    }
+    for (auto& bi : c->block_items) {
+        a.items = c->d_items + bi.first;
+        a.nitems_ptr = nullptr;
+        a.nitems = bi.second;
+        if ((rc = pre_pass(a, bi.second))) return rc;
+    }
+    for (auto& bi : c->block_items) {
+        a.items = c->d_items + bi.first;
+        a.nitems_ptr = nullptr;
+        a.nitems = bi.second;
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, prims, a, bi.second, c->threads, c->stream));
+        c->last_launches += 1;
     }
     return KNC_OK;
 }
@@ -447,9 +494,9 @@ int enqueue_time_step(knc_context* c)
     a.in_slot = c->cur;
     a.step = 0;
     a.nitems_ptr = c->d_nitems;
-    const int grid = std::min(c->nitems, c->num_cu * 4);
-    HIP_TRY(launch_dt(c->h.system, a, c->d_min_bits, std::max(1, grid), c->stream));
-    HIP_TRY(launch_dt_finish(c->d_min_bits, c->h.cfl_number, c->d_dt, c->stream));
+    const int grid = row_pass_grid(c, c->nitems);
+    HIP_TRY(launch_dt(c->h.system, a, c->d_dt_partial, grid, c->stream));
+    HIP_TRY(launch_dt_finish(c->d_dt_partial, grid, c->h.cfl_number, c->d_dt, c->stream));
     return KNC_OK;
 }
 
@@ -502,8 +549,97 @@ int require_blocks(knc_context* c)
     return rebuild_tables(c);
 }
 
-// Runs `fold` RK steps after an optional dt pass; returns after the stream
-// drained and the error log was read.
+// Enqueues n RK steps from slot c->cur: replayed from a hipGraph captured
+// per (starting slot, n) in batched mode, launched one by one otherwise.
+int enqueue_steps(knc_context* c, int n)
+{
+    if (n <= 0) return KNC_OK;
+    const int start_slot = c->cur;
+    const bool graph = c->o.launch_mode == KNC_LAUNCH_BATCHED && c->o.use_graph && n > 1;
+    if (!graph) {
+        int rc;
+        for (int s = 0; s < n; ++s)
+            if ((rc = enqueue_rk_step(c, s + 1))) return rc;
+        return KNC_OK;
+    }
+    auto key = std::make_pair(start_slot, n);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        hipGraph_t g;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int crc = KNC_OK;
+        const int launches = c->last_launches;
+        for (int s = 0; s < n && crc == KNC_OK; ++s) crc = enqueue_rk_step(c, s + 1);
+        hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        c->cur = start_slot;
+        c->last_launches = launches;
+        if (crc) return crc;
+        HIP_TRY(ce);
+        hipGraphExec_t ex;
+        HIP_TRY(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        HIP_TRY(hipGraphDestroy(g));
+        it = c->graphs.emplace(key, ex).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, c->stream));
+    // advance the slot bookkeeping as the captured steps did
+    for (int s = 0; s < n; ++s) {
+        const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
+        c->cur = c->h.runge_kutta_order == 2 ? f2 : f1;
+    }
+    c->last_launches += n * c->h.runge_kutta_order;
+    return KNC_OK;
+}
+
+bool excision_moving(const knc_context* c)
+{
+    return c->exc_on && (c->exc.inner_speed > 0.0 || c->exc.outer_speed > 0.0);
+}
+
+// Mesh::inner_excision_surface / outer_excision_surface (src/mesh.rs:318-336).
+double excision_surface(double r0, double speed, double delay, double t)
+{
+    return r0 + std::max(t - delay, 0.0) * speed;
+}
+
+// add_remove_blocks (src/scheme.rs:179-206) at state time t.  With apply
+// false only reports whether the block set would change.
+int excise(knc_context* c, double t, bool apply, bool* change)
+{
+    *change = false;
+    const HostBlock& bi = c->blocks.begin()->second;
+    const HostBlock& bo = c->blocks.rbegin()->second;
+    const int32_t inner_index = bi.index, outer_index = bo.index;
+    const bool remove = c->blocks.size() > 1 &&
+                        bi.rv[bi.nr] < excision_surface(c->exc.inner_radius, c->exc.inner_speed, c->exc.delay, t);
+    const bool add = c->exc.source != nullptr &&
+                     bo.rv[bo.nr] < excision_surface(c->exc.outer_radius, c->exc.outer_speed, c->exc.delay, t);
+    *change = remove || add;
+    if (!apply) return KNC_OK;
+    int rc;
+    if (remove) {
+        if ((rc = knc_block_remove(c, inner_index))) return rc;
+        c->exc_removed += 1;
+    }
+    if (add) {
+        knc_block_data d{};
+        const int32_t idx = outer_index + 1;
+        const int32_t src_rc = c->exc.source(c->exc.user, idx, t, &d);
+        if (src_rc) return fail(KNC_ERR_INVALID_ARGUMENT, "block source returned %d for block %d", src_rc, idx);
+        if (!d.radial_vertices || !d.conserved || !d.scalar_mass)
+            return fail(KNC_ERR_INVALID_ARGUMENT, "block source left a NULL buffer for block %d", idx);
+        const std::vector<double> qv = c->qv;
+        if ((rc = knc_block_insert(c, idx, d.nr, c->nq, d.radial_vertices, qv.data(), d.conserved, d.scalar_mass)))
+            return rc;
+        c->exc_added += 1;
+    }
+    return KNC_OK;
+}
+
+// Runs `fold` RK steps after an optional dt pass (scheme::advance,
+// src/scheme.rs:212-240); returns after the stream drained and the error log
+// was read.  With moving excision surfaces the steps run in segments between
+// block-set changes: dt is read back first, so the state time of every step
+// is known on the host.
 int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* dt_used, knc_error* err)
 {
     int rc;
@@ -517,41 +653,41 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
         hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->d_dt, dt_fixed);
         HIP_TRY(hipGetLastError());
     }
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    const int start_slot = c->cur;
-    const bool graph = c->o.launch_mode == KNC_LAUNCH_BATCHED && c->o.use_graph && fold > 1;
-    if (graph) {
-        auto key = std::make_pair(start_slot, fold);
-        auto it = c->graphs.find(key);
-        if (it == c->graphs.end()) {
-            hipGraph_t g;
-            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int crc = KNC_OK;
-            for (int s = 0; s < fold && crc == KNC_OK; ++s) crc = enqueue_rk_step(c, s + 1);
-            hipError_t ce = hipStreamEndCapture(c->stream, &g);
-            c->cur = start_slot;
-            if (crc) return crc;
-            HIP_TRY(ce);
-            hipGraphExec_t ex;
-            HIP_TRY(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-            HIP_TRY(hipGraphDestroy(g));
-            it = c->graphs.emplace(key, ex).first;
-        }
-        HIP_TRY(hipGraphLaunch(it->second, c->stream));
-        // advance the slot bookkeeping as the captured steps did
-        for (int s = 0; s < fold; ++s) {
-            const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
-            c->cur = c->h.runge_kutta_order == 2 ? f2 : f1;
-        }
-        c->last_launches = fold * c->h.runge_kutta_order;
-    } else {
-        for (int s = 0; s < fold; ++s)
-            if ((rc = enqueue_rk_step(c, s + 1))) return rc;
+    const bool moving = with_dt && excision_moving(c);
+    double dt = dt_fixed;
+    if (moving) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(&dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost));
+        if ((rc = collect_error(c, err))) return rc;
     }
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    int pending = 0;
+    double t = c->time;
+    int64_t iter = c->iteration;
+    const int every = std::max(1, c->exc.check_every);
+    for (int s = 0; s < fold; ++s) {
+        if (moving && iter % every == 0) {
+            bool change = false;
+            if ((rc = excise(c, t, false, &change))) return rc;
+            if (change) {
+                if ((rc = enqueue_steps(c, pending))) return rc;
+                pending = 0;
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                if ((rc = collect_error(c, err))) return rc;
+                c->time = t;
+                c->iteration = iter;
+                if ((rc = excise(c, t, true, &change))) return rc;
+                if ((rc = require_blocks(c))) return rc;
+            }
+        }
+        ++pending;
+        t = advance_time(c->h.runge_kutta_order, t, dt);
+        ++iter;
+    }
+    if ((rc = enqueue_steps(c, pending))) return rc;
     if (c->timing) HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    double dt = dt_fixed;
-    HIP_TRY(hipMemcpy(&dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost));
+    if (!moving) HIP_TRY(hipMemcpy(&dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost));
     if (c->timing) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -560,10 +696,138 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
     }
     if (dt_used) *dt_used = dt;
     if ((rc = collect_error(c, err))) return rc;
-    for (int s = 0; s < fold; ++s) {
-        c->time = advance_time(c->h.runge_kutta_order, c->time, dt);
-        c->iteration += 1;
+    if (moving) {
+        c->time = t;
+        c->iteration = iter;
+    } else {
+        for (int s = 0; s < fold; ++s) {
+            c->time = advance_time(c->h.runge_kutta_order, c->time, dt);
+            c->iteration += 1;
+        }
     }
+    return KNC_OK;
+}
+
+// ---- autotuner ----------------------------------------------------------
+// Shapes already measured in this process, keyed by everything the stage
+// launches depend on: configuration, device, polar size and block sizes.
+std::mutex g_tune_mutex;
+std::map<std::vector<int64_t>, std::vector<int>> g_tune_cache;
+
+std::vector<int64_t> tune_key(const knc_context* c)
+{
+    std::vector<int64_t> k = {c->h.system, c->h.riemann_solver, c->h.runge_kutta_order, c->o.collect_stats,
+                              learned(c), c->o.launch_mode, c->o.device, c->num_cu, c->nq};
+    for (auto& kv : c->blocks) k.push_back(kv.second.nr);
+    return k;
+}
+
+// Average device time of one stage over `trials` RK steps with the given
+// shape.  The steps read slot c->cur and write the two other slots only, so
+// the state is unchanged.
+int time_shape(knc_context* c, int threads, int rows, int passes, int trials, double* ms)
+{
+    c->o.stage_threads = threads;
+    c->o.rows_per_item = rows;
+    c->o.stage_passes = passes;
+    int rc;
+    if ((rc = choose_stage_shape(c))) return rc;
+    c->dirty = true;
+    if ((rc = rebuild_tables(c))) return rc;
+    const int saved = c->cur;
+    for (int w = 0; w < trials + 1; ++w) {
+        if (w == 1) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        if ((rc = enqueue_rk_step(c, 1))) return rc;
+        c->cur = saved;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = (double)t / (trials * c->h.runge_kutta_order);
+    return KNC_OK;
+}
+
+int autotune(knc_context* c, int trials, bool force, knc_tune_result* out)
+{
+    int rc;
+    if ((rc = require_blocks(c))) return rc;
+    knc_tune_result r{};
+    const std::vector<int64_t> key = tune_key(c);
+    if (!force) {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            c->o.stage_threads = it->second[0];
+            c->o.rows_per_item = it->second[1];
+            c->o.stage_passes = it->second[2];
+            if ((rc = choose_stage_shape(c))) return rc;
+            c->dirty = true;
+            if ((rc = rebuild_tables(c))) return rc;
+            r.stage_threads = c->threads;
+            r.rows_per_item = c->rows_per_item;
+            r.stage_passes = c->o.stage_passes;
+            r.from_cache = 1;
+            if (out) *out = r;
+            return KNC_OK;
+        }
+    }
+    if (trials <= 0) trials = 3;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // keep the statistics and the error log as they were
+    C2PStats saved_stats;
+    HIP_TRY(hipMemcpy(&saved_stats, c->d_stats, sizeof saved_stats, hipMemcpyDeviceToHost));
+    const knc_options saved = c->o;
+    // the trial stages use a fixed dt of zero, so every trial reads the same data
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->d_dt, 0.0);
+    HIP_TRY(hipGetLastError());
+    double best = 0.0;
+    int best_t = 0, best_r = 0, best_p = 1;
+    rc = time_shape(c, saved.stage_threads, saved.rows_per_item, saved.stage_passes, trials, &r.default_ms_per_stage);
+    int ncand = 0;
+    const int widths[] = {64, 128, 256, 512};
+    const int rows[] = {4, 8, 16, 32, 64};
+    for (int passes = 1; passes <= 2 && rc == KNC_OK; ++passes)
+        for (int t : widths) {
+            if (rc) break;
+            if (!stage_variant_supported(t)) continue;
+            if (t > 64 && t / 2 >= c->nq) continue;   // wider than the mesh
+            for (int nr : rows) {
+                if (nr > stage_max_rows()) continue;
+                double ms = 0.0;
+                if ((rc = time_shape(c, t, nr, passes, trials, &ms))) break;
+                ++ncand;
+                if (best_t == 0 || ms < best) {
+                    best = ms;
+                    best_t = t;
+                    best_r = nr;
+                    best_p = passes;
+                }
+            }
+        }
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
+    (void)hipMemcpy(c->d_stats, &saved_stats, sizeof saved_stats, hipMemcpyHostToDevice);
+    c->o = saved;
+    if (rc == KNC_OK && best_t) {
+        c->o.stage_threads = best_t;
+        c->o.rows_per_item = best_r;
+        c->o.stage_passes = best_p;
+    }
+    int rc2 = choose_stage_shape(c);
+    c->dirty = true;
+    if (rc2 == KNC_OK) rc2 = rebuild_tables(c);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_cache[key] = {best_t, best_r, best_p};
+    }
+    r.stage_threads = best_t;
+    r.rows_per_item = best_r;
+    r.stage_passes = best_p;
+    r.candidates = ncand;
+    r.ms_per_stage = best;
+    if (out) *out = r;
     return KNC_OK;
 }
 
@@ -605,6 +869,10 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
         delete c;
         return fail(KNC_ERR_INVALID_ARGUMENT, "unknown c2p guess mode");
     }
+    if (c->o.stage_passes < 0 || c->o.stage_passes > 2) {
+        delete c;
+        return fail(KNC_ERR_INVALID_ARGUMENT, "stage_passes must be 0, 1 or 2");
+    }
     c->zp = zone_params(c->h);
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -625,13 +893,12 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(KNC_ERR_HIP, "hipStreamCreate failed"));
     if (hipMalloc(&c->d_err, sizeof(ErrLog)) != hipSuccess || hipMalloc(&c->d_stats, sizeof(C2PStats)) != hipSuccess ||
-        hipMalloc(&c->d_min_bits, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_dt_partial, DT_MAX_GRID * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->d_dt, sizeof(double)) != hipSuccess || hipMalloc(&c->d_nitems, sizeof(int32_t)) != hipSuccess)
         return cleanup(fail(KNC_ERR_HIP, "hipMalloc failed"));
     (void)hipMemset(c->d_stats, 0, sizeof(C2PStats));
     (void)hipMemset(c->d_dt, 0, sizeof(double));
     hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
-    if (launch_dt_reset(c->d_min_bits, c->stream) != hipSuccess) return cleanup(fail(KNC_ERR_HIP, "reset failed"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return cleanup(fail(KNC_ERR_HIP, "hipEventCreate failed"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(KNC_ERR_HIP, "init failed"));
@@ -647,9 +914,11 @@ void knc_context_destroy(knc_context* c)
     for (auto& kv : c->blocks) {
         (void)hipFree(kv.second.mem);
         (void)hipFree(kv.second.rgeo);
+        (void)hipFree(kv.second.guess);
+        (void)hipFree(kv.second.prim);
     }
     void* ptrs[] = {c->d_theta, c->d_bnd, c->d_blocks, c->d_items, c->d_nitems, c->d_err, c->d_stats,
-                    c->d_min_bits, c->d_dt, c->d_mlp};
+                    c->d_dt_partial, c->d_dt, c->d_mlp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -690,6 +959,17 @@ int knc_block_insert(knc_context* c, int32_t index, int32_t nr, int32_t nq, cons
         (void)hipFree(b.mem);
         return fail(KNC_ERR_HIP, "hipMalloc failed");
     }
+    if (hipMalloc(&b.guess, n * sizeof(float)) != hipSuccess) {
+        (void)hipFree(b.mem);
+        (void)hipFree(b.rgeo);
+        return fail(KNC_ERR_HIP, "hipMalloc failed");
+    }
+    if (hipMalloc(&b.prim, 5 * n * sizeof(double)) != hipSuccess) {
+        (void)hipFree(b.mem);
+        (void)hipFree(b.rgeo);
+        (void)hipFree(b.guess);
+        return fail(KNC_ERR_HIP, "hipMalloc failed");
+    }
     std::vector<double> soa(5 * n);
     for (size_t k = 0; k < n; ++k) {
         soa[k] = conserved[4 * k];
@@ -716,6 +996,8 @@ int knc_block_remove(knc_context* c, int32_t index)
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(it->second.mem));
     HIP_TRY(hipFree(it->second.rgeo));
+    HIP_TRY(hipFree(it->second.guess));
+    HIP_TRY(hipFree(it->second.prim));
     c->blocks.erase(it);
     c->dirty = true;
     return KNC_OK;
@@ -817,6 +1099,36 @@ int knc_advance(knc_context* c, int32_t fold, double* dt_used, knc_error* err)
 {
     if (fold < 1) return fail(KNC_ERR_INVALID_ARGUMENT, "fold must be >= 1");
     return run_steps(c, true, 0.0, fold, dt_used, err);
+}
+
+int knc_set_excision(knc_context* c, const knc_excision* ex)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!ex) {
+        c->exc_on = false;
+        c->exc = knc_excision{};
+        return KNC_OK;
+    }
+    if (ex->check_every < 1) return fail(KNC_ERR_INVALID_ARGUMENT, "check_every must be >= 1");
+    if (!(ex->inner_speed >= 0.0) || !(ex->outer_speed >= 0.0))
+        return fail(KNC_ERR_INVALID_ARGUMENT, "excision speeds must be >= 0");
+    c->exc = *ex;
+    c->exc_on = true;
+    return KNC_OK;
+}
+
+int knc_excision_counts(knc_context* c, int64_t* removed, int64_t* added)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (removed) *removed = c->exc_removed;
+    if (added) *added = c->exc_added;
+    return KNC_OK;
+}
+
+int knc_autotune(knc_context* c, int32_t trials, int32_t force, knc_tune_result* out)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    return autotune(c, trials, force != 0, out);
 }
 
 int knc_context_set_mlp(knc_context* c, const knc_mlp* mm)

@@ -32,6 +32,8 @@ constexpr int MAX_NQ = 1024;
 struct BlockDesc {
     double* slot[NSLOT];
     const double* rgeo;
+    float* guess;    // learned c2p guess log2(p/D) per zone (nr x nq), written by k_guess
+    double* prim;    // primitive planes rho | u_r | u_q | p | rho X (5 x nr x nq), written by k_prim
     int32_t nr;
     int32_t nq;
     int32_t left;    // table index of the inner neighbour, -1 at the inner boundary

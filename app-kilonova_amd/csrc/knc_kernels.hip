@@ -13,8 +13,8 @@
 // HBM.
 //
 // k_dt: the CFL reduction of State::time_step (src/state.rs:143-162): c2p,
-// dl / (max_signal_speed * c), wavefront min by cross-lane shuffles, LDS, one
-// 64-bit atomicMin per workgroup on the bit pattern of a positive double.
+// dl / (max_signal_speed * c), wavefront min by cross-lane shuffles, one
+// partial minimum per workgroup, reduced by k_dt_finish.
 #include <hip/hip_runtime.h>
 #include <float.h>
 
@@ -115,6 +115,7 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
 // workgroup, so the resolution and every branch on it are uniform.
 struct RowSrc {
     const double* base;   // first plane at (row, 0): U planes, or boundary prim planes
+    const float* guess;   // learned guess row (block rows of learned-guess stages)
     int64_t np;           // plane stride
     double rvolA;         // 1/volA of the row (conserved rows)
     int32_t fixed;        // 1: boundary primitive rows (no c2p)
@@ -126,6 +127,7 @@ __device__ inline RowSrc fixed_rows(const double* rows, int k2, int nq)
     RowSrc r;
     r.fixed = 1;
     r.base = rows + (int64_t)k2 * nq;
+    r.guess = nullptr;
     r.np = 2 * (int64_t)nq;
     r.rvolA = 1.0;
     r.order = 0;
@@ -134,11 +136,15 @@ __device__ inline RowSrc fixed_rows(const double* rows, int k2, int nq)
     return r;
 }
 
+// Row k of block S: its conserved planes (c2p in the stage), or with PRIMS
+// the primitive planes a c2p pass already wrote (no c2p in the stage).
+template <bool PRIMS>
 __device__ inline RowSrc block_row(const StageArgs& a, const BlockDesc& S, int k)
 {
     RowSrc r;
-    r.fixed = 0;
-    r.base = slot_of(S, a.in_slot) + (int64_t)k * S.nq;
+    r.fixed = PRIMS ? 1 : 0;
+    r.base = (PRIMS ? S.prim : slot_of(S, a.in_slot)) + (int64_t)k * S.nq;
+    r.guess = S.guess + (int64_t)k * S.nq;
     r.np = (int64_t)S.nr * S.nq;
     r.rvolA = rg_rvolA(S)[k];
     r.order = S.order;
@@ -147,40 +153,47 @@ __device__ inline RowSrc block_row(const StageArgs& a, const BlockDesc& S, int k
     return r;
 }
 
+template <bool PRIMS>
 __device__ inline RowSrc resolve_row(const StageArgs& a, const BlockDesc& B, int k)
 {
     if (k < 0) {
         if (B.left >= 0) {
             const BlockDesc N = a.blocks[B.left];
-            return block_row(a, N, N.nr + k);
+            return block_row<PRIMS>(a, N, N.nr + k);
         }
         if (a.bnd.inner_mode == BND_FIXED) return fixed_rows(a.bnd.inner_rows, k + 2, B.nq);
-        return block_row(a, B, 0);
+        return block_row<PRIMS>(a, B, 0);
     }
     if (k >= B.nr) {
         if (B.right >= 0) {
             const BlockDesc N = a.blocks[B.right];
-            return block_row(a, N, k - B.nr);
+            return block_row<PRIMS>(a, N, k - B.nr);
         }
         if (a.bnd.outer_mode == BND_FIXED) return fixed_rows(a.bnd.outer_rows, k - B.nr, B.nq);
-        return block_row(a, B, B.nr - 1);
+        return block_row<PRIMS>(a, B, B.nr - 1);
     }
-    return block_row(a, B, k);
+    return block_row<PRIMS>(a, B, k);
 }
 
-struct Raw5 { double v[5]; };
+// One zone as loaded from a row: the five planes and, for learned-guess
+// stages, the guessed log2(p/D).
+struct Raw5 {
+    double v[5];
+    float g;
+};
 
+template <bool GUESS>
 __device__ inline Raw5 load_raw(const RowSrc& r, int j)
 {
     const double* p = r.base + j;
     Raw5 x;
 #pragma unroll
     for (int f = 0; f < 5; ++f) x.v[f] = p[f * r.np];
+    x.g = GUESS && !r.fixed ? r.guess[j] : 0.0f;
     return x;
 }
 
-// Primitive of a loaded row value.  Must be reached by every lane of the
-// wavefront (the learned guess is a wavefront-wide MFMA evaluation).
+// Primitive of a loaded row value.
 template <int SYS, bool STATS, bool GUESS>
 __device__ inline PS raw_primitive(const StageArgs& a, const RowSrc& r, const Raw5& x, int j, unsigned int* lds_hist)
 {
@@ -191,11 +204,7 @@ __device__ inline PS raw_primitive(const StageArgs& a, const RowSrc& r, const Ra
         return q;
     }
     const double rdv = r.rvolA * a.th.rdcost[j];
-    double guess = 0.0;
-    if (GUESS) {
-        const double D = x.v[0] * rdv, sr = x.v[1] * rdv, sq = x.v[2] * rdv, tau = x.v[3] * rdv;
-        guess = mlp_guess_wave(a.mlp, D, sr * sr + sq * sq, tau, a.zp.gamma);
-    }
+    const double guess = GUESS ? mlp_guess_pressure(x.v[0] * rdv, x.g) : 0.0;
     return zone_primitive<SYS, STATS, GUESS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], rdv, r.order, r.row, j,
                                              r.index, lds_hist, guess);
 }
@@ -235,14 +244,16 @@ __device__ inline Flux5 scaled(const Cons& F, double Fs, double c, double A)
 constexpr int MAX_ROWS = 64;
 constexpr int NHALO = 6;   // j0-2, j0-1 | j1-2, j1-1, j1, j1+1
 
-template <int SYS, bool STATS, bool GUESS>
+template <int SYS, bool STATS, bool GUESS, bool PRIMS>
 __device__ inline PS column_primitive(const StageArgs& a, const BlockDesc& B, int row, int col, unsigned int* lds_hist)
 {
-    const RowSrc src = block_row(a, B, row);
-    return raw_primitive<SYS, STATS, GUESS>(a, src, load_raw(src, col), col, lds_hist);
+    const RowSrc src = block_row<PRIMS>(a, B, row);
+    return raw_primitive<SYS, STATS, GUESS>(a, src, load_raw<GUESS && !PRIMS>(src, col), col, lds_hist);
 }
 
-template <int SYS, int RS, bool STATS, int NT, bool GUESS>
+// PRIMS: the primitives come from the planes written by k_prim (the two-pass
+// stage); otherwise the stage recovers them itself (fused stage).
+template <int SYS, int RS, bool STATS, int NT, bool GUESS, bool PRIMS>
 __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 {
     constexpr int WP = NT + 3;                 // primitive row: 2 left halo, NT own, 1 right halo
@@ -291,7 +302,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                 const int h = zz % NHALO, rr = zz / NHALO;
                 const int col = h < 2 ? j0 - 2 + h : j1 - 4 + h;
                 const bool need = z < NHALO * nrow && (h < 2 ? left_halo : right_halo) && col >= 0 && col < nq;
-                const PS q0 = column_primitive<SYS, STATS, GUESS>(a, B, it.i0 + rr, need ? col : j, lds_hist);
+                const PS q0 = column_primitive<SYS, STATS, GUESS, PRIMS>(a, B, it.i0 + rr, need ? col : j, lds_hist);
                 if (z < NHALO * nrow) {
                     const PS q = need ? q0 : zero;
                     double* d = ldsH + (h * MAX_ROWS + rr) * 5;
@@ -324,15 +335,15 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 
         PS Pm2, Pm1, gm2;
         Flux5 Fm;
-        RowSrc src = resolve_row(a, B, it.i0 - 2);
-        Raw5 cur = load_raw(src, j), nxt;
+        RowSrc src = resolve_row<PRIMS>(a, B, it.i0 - 2);
+        Raw5 cur = load_raw<GUESS && !PRIMS>(src, j), nxt;
 
         for (int k = it.i0 - 2; k < it.i1 + 2; ++k) {
             // Prefetch the next row and this step's update operands.
             RowSrc nsrc = src;
             if (k + 1 < it.i1 + 2) {
-                nsrc = resolve_row(a, B, k + 1);
-                nxt = load_raw(nsrc, j);
+                nsrc = resolve_row<PRIMS>(a, B, k + 1);
+                nxt = load_raw<GUESS && !PRIMS>(nsrc, j);
             }
             const bool update = k >= it.i0 + 2;
             Raw5 ui, u0;
@@ -451,6 +462,88 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// c2p pass of the two-pass stage: the primitives and scalar concentration of
+// every zone of the stage's input into the blocks' primitive planes, each
+// zone recovered once.  One (work item, row) unit per workgroup iteration,
+// NT columns at a time.  With GUESS, Newton starts from the guess plane
+// written by k_guess.
+// ---------------------------------------------------------------------------
+template <int SYS, bool STATS, bool GUESS, int NT>
+__global__ void __launch_bounds__(NT) k_prim(StageArgs a)
+{
+    __shared__ unsigned int lds_hist[HIST_BINS];
+    const int tid = threadIdx.x;
+    if (STATS) {
+        for (int b = tid; b < HIST_BINS; b += NT) lds_hist[b] = 0u;
+        __syncthreads();
+    }
+    const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const WorkItem it = a.items[u / a.rows];
+        const int i = it.i0 + u % a.rows;
+        if (i >= it.i1) continue;
+        const BlockDesc B = a.blocks[it.block];
+        const int nq = B.nq;
+        const int64_t np = (int64_t)B.nr * nq;
+        const double* U = slot_of(B, a.in_slot) + (int64_t)i * nq;
+        double* P = B.prim + (int64_t)i * nq;
+        const double rvolA = rg_rvolA(B)[i];
+        for (int j = it.j0 + tid; j < it.j1; j += NT) {
+            const double rdv = rvolA * a.th.rdcost[j];
+            const double u0 = U[j], u1 = U[np + j], u2 = U[2 * np + j], u3 = U[3 * np + j], u4 = U[4 * np + j];
+            const double guess = GUESS ? mlp_guess_pressure(u0 * rdv, B.guess[(int64_t)i * nq + j]) : 0.0;
+            {
+                const PS q = zone_primitive<SYS, STATS, GUESS>(a, u0, u1, u2, u3, u4, rdv, B.order, i, j, B.index,
+                                                               lds_hist, guess);
+                P[j] = q.P.rho;
+                P[np + j] = q.P.ur;
+                P[2 * np + j] = q.P.uq;
+                P[3 * np + j] = q.P.p;
+                P[4 * np + j] = q.s;
+            }
+        }
+    }
+    if (STATS) {
+        __syncthreads();
+        for (int b = tid; b < HIST_BINS; b += NT)
+            if (lds_hist[b]) atomicAdd(&a.stats->hist[b], (unsigned long long)lds_hist[b]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Learned c2p guess pass: before a learned-guess stage, the network's
+// log2(p/D) for every zone of the stage's input into the blocks' guess
+// planes.  One workgroup per (work item, row), NT columns per pass; the
+// inference is wavefront-wide (BF16 MFMA), so every lane runs it and lanes
+// past the item's last column evaluate that column without storing.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) k_guess(StageArgs a)
+{
+    const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const WorkItem it = a.items[u / a.rows];
+        const int i = it.i0 + u % a.rows;
+        if (i >= it.i1) continue;
+        const BlockDesc B = a.blocks[it.block];
+        const int nq = B.nq;
+        const int64_t np = (int64_t)B.nr * nq;
+        const double* U = slot_of(B, a.in_slot) + (int64_t)i * nq;
+        const double rvolA = rg_rvolA(B)[i];
+        for (int c0 = it.j0; c0 < it.j1; c0 += NT) {
+            const int j = c0 + (int)threadIdx.x;
+            const bool act = j < it.j1;
+            const int jj = act ? j : it.j1 - 1;
+            const double rdv = rvolA * a.th.rdcost[jj];
+            const double D = U[jj] * rdv, sr = U[np + jj] * rdv, sq = U[2 * np + jj] * rdv;
+            const double tau = U[3 * np + jj] * rdv;
+            const float l2 = mlp_log2_guess_wave(a.mlp, D, sr * sr + sq * sq, tau, a.zp.gamma);
+            if (act) B.guess[(int64_t)i * nq + j] = l2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // CFL reduction.
 // ---------------------------------------------------------------------------
 __device__ inline double wave_min(double v)
@@ -463,26 +556,27 @@ __device__ inline double wave_min(double v)
 }
 
 template <int SYS>
-__global__ void __launch_bounds__(256) k_dt(StageArgs a, unsigned long long* min_bits)
+__global__ void __launch_bounds__(256) k_dt(StageArgs a, double* partial)
 {
-    __shared__ double wmin[8];
-    const int nitems = a.nitems_ptr ? *a.nitems_ptr : a.nitems;
+    __shared__ double wmin[4];
+    const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
     double m = DBL_MAX;
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-        const WorkItem it = a.items[w];
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const WorkItem it = a.items[u / a.rows];
+        const int i = it.i0 + u % a.rows;
+        if (i >= it.i1) continue;
         const BlockDesc B = a.blocks[it.block];
         const int nq = B.nq;
         const int64_t np = (int64_t)B.nr * nq;
-        const int nz = (it.i1 - it.i0) * nq;
-        for (int z = threadIdx.x; z < nz; z += blockDim.x) {
-            const int i = it.i0 + z / nq;
-            const int j = z % nq;
+        const double rvolA = rg_rvolA(B)[i];
+        const double dr = rg_dr(B)[i];
+        const double r0 = rg_rv(B)[i];
+        for (int j = it.j0 + (int)threadIdx.x; j < it.j1; j += blockDim.x) {
             const double* s = slot_of(B, a.in_slot) + (int64_t)i * nq + j;
-            const double rdv = rg_rvolA(B)[i] * a.th.rdcost[j];
+            const double rdv = rvolA * a.th.rdcost[j];
             const PS q = zone_primitive<SYS, false, false>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, B.order,
                                                            i, j, B.index, nullptr, 0.0);
-            const double dr = rg_dr(B)[i];
-            const double dl = dmin(dr, a.th.dq[j] * rg_rv(B)[i]);
+            const double dl = dmin(dr, a.th.dq[j] * r0);
             m = dmin(m, dl / (max_signal_speed<SYS>(q.P, a.zp) * a.zp.c));
         }
     }
@@ -493,23 +587,26 @@ __global__ void __launch_bounds__(256) k_dt(StageArgs a, unsigned long long* min
     if (threadIdx.x == 0) {
         double r = wmin[0];
         for (int k = 1; k < (int)(blockDim.x + 63) / 64; ++k) r = dmin(r, wmin[k]);
-        atomicMin(min_bits, (unsigned long long)__double_as_longlong(r));
+        partial[blockIdx.x] = r;
     }
 }
 
-// dt = min * cfl (state.rs:160); resets the accumulator for the next pass.
-__global__ void k_dt_finish(unsigned long long* min_bits, double cfl, double* dt_out)
+// dt = min * cfl (state.rs:160) over the workgroup minima of k_dt; the
+// minimum is exact, so the reduction order does not matter.
+__global__ void __launch_bounds__(256) k_dt_finish(const double* partial, int n, double cfl, double* dt_out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double m = __longlong_as_double((long long)*min_bits);
-        *dt_out = m * cfl;
-        *min_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
+    __shared__ double wmin[4];
+    double m = DBL_MAX;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) m = dmin(m, partial[k]);
+    m = wave_min(m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) wmin[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = wmin[0];
+        for (int k = 1; k < (int)(blockDim.x + 63) / 64; ++k) r = dmin(r, wmin[k]);
+        *dt_out = r * cfl;
     }
-}
-
-__global__ void k_dt_reset(unsigned long long* min_bits)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *min_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
 }
 
 // ---------------------------------------------------------------------------
@@ -617,33 +714,62 @@ static int grid_for(int64_t n, int block)
     return (int)g;
 }
 
-// Stage kernel variants by workgroup size.
-template <int SYS, int RS, bool ST, bool G>
-static hipError_t launch_stage_v(const StageArgs& a, int grid, int threads, hipStream_t s)
+// Stage kernel variants by workgroup size.  The flux pass of the two-pass
+// stage does no c2p, so it has no statistics or guess variants.
+template <int SYS, int RS, bool ST, bool G, bool P>
+static hipError_t launch_stage_p(const StageArgs& a, int grid, int threads, hipStream_t s)
 {
     switch (threads) {
-    case 64: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 64, G>), dim3(grid), dim3(64), 0, s, a); break;
-    case 128: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 128, G>), dim3(grid), dim3(128), 0, s, a); break;
-    case 256: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256, G>), dim3(grid), dim3(256), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512, G>), dim3(grid), dim3(512), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 64, G, P>), dim3(grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 128, G, P>), dim3(grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 256, G, P>), dim3(grid), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_stage<SYS, RS, ST, 512, G, P>), dim3(grid), dim3(512), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int SYS, int RS, bool ST, bool G>
-static int occupancy_v(int threads)
+template <int SYS, int RS, bool ST, bool G, bool P>
+static int occupancy_p(int threads)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (threads) {
-    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 64, G>, 64, 0); break;
-    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 128, G>, 128, 0); break;
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 256, G>, 256, 0); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 512, G>, 512, 0); break;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 64, G, P>, 64, 0); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 128, G, P>, 128, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 256, G, P>, 256, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SYS, RS, ST, 512, G, P>, 512, 0); break;
     default: break;
     }
     return e == hipSuccess ? n : 0;
+}
+
+template <int SYS, int RS, bool ST, bool G>
+static hipError_t launch_stage_v(const StageArgs& a, int grid, int threads, bool prims, hipStream_t s)
+{
+    return prims ? launch_stage_p<SYS, RS, false, false, true>(a, grid, threads, s)
+                 : launch_stage_p<SYS, RS, ST, G, false>(a, grid, threads, s);
+}
+
+template <int SYS, int RS, bool ST, bool G>
+static int occupancy_v(int threads, bool prims)
+{
+    return prims ? occupancy_p<SYS, RS, false, false, true>(threads) : occupancy_p<SYS, RS, ST, G, false>(threads);
+}
+
+// c2p pass variants by workgroup size.
+template <int SYS, int RS, bool ST, bool G>
+static hipError_t launch_prim_v(const StageArgs& a, int grid, int threads, hipStream_t s)
+{
+    const dim3 g(grid);
+    switch (threads) {
+    case 64: hipLaunchKernelGGL((k_prim<SYS, ST, G, 64>), g, dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((k_prim<SYS, ST, G, 128>), g, dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_prim<SYS, ST, G, 256>), g, dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_prim<SYS, ST, G, 512>), g, dim3(512), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 bool stage_variant_supported(int threads)
@@ -670,33 +796,48 @@ int stage_max_rows() { return MAX_ROWS; }
                           : (stats ? FN<SYS_EULER, RS_HLLE, true, false>(__VA_ARGS__)                      \
                                    : FN<SYS_EULER, RS_HLLE, false, false>(__VA_ARGS__))))
 
-int stage_occupancy(int system, int rs, bool stats, bool learned, int threads)
+int stage_occupancy(int system, int rs, bool stats, bool learned, bool prims, int threads)
 {
-    return KNC_DISPATCH(occupancy_v, threads);
+    return KNC_DISPATCH(occupancy_v, threads, prims);
 }
 
-hipError_t launch_stage(int system, int rs, bool stats, bool learned, const StageArgs& a, int grid, int threads,
-                        hipStream_t s)
+hipError_t launch_stage(int system, int rs, bool stats, bool learned, bool prims, const StageArgs& a, int grid,
+                        int threads, hipStream_t s)
 {
-    return KNC_DISPATCH(launch_stage_v, a, grid, threads, s);
+    return KNC_DISPATCH(launch_stage_v, a, grid, threads, prims, s);
 }
 
-hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s)
+hipError_t launch_prim(int system, bool stats, bool learned, const StageArgs& a, int grid, int threads,
+                       hipStream_t s)
 {
-    if (system == SYS_SRHD) hipLaunchKernelGGL((k_dt<SYS_SRHD>), dim3(grid), dim3(256), 0, s, a, min_bits);
-    else hipLaunchKernelGGL((k_dt<SYS_EULER>), dim3(grid), dim3(256), 0, s, a, min_bits);
+    const int rs = RS_HLLE;
+    return KNC_DISPATCH(launch_prim_v, a, grid, threads, s);
+}
+
+hipError_t launch_dt(int system, const StageArgs& a, double* partial, int grid, hipStream_t s)
+{
+    if (grid < 1 || grid > DT_MAX_GRID) return hipErrorInvalidValue;
+    if (system == SYS_SRHD) hipLaunchKernelGGL((k_dt<SYS_SRHD>), dim3(grid), dim3(256), 0, s, a, partial);
+    else hipLaunchKernelGGL((k_dt<SYS_EULER>), dim3(grid), dim3(256), 0, s, a, partial);
     return hipGetLastError();
 }
 
-hipError_t launch_dt_finish(unsigned long long* min_bits, double cfl, double* dt_out, hipStream_t s)
+hipError_t launch_guess(const StageArgs& a, int grid, int threads, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, s, min_bits, cfl, dt_out);
+    const dim3 g(grid);
+    switch (threads) {
+    case 64: hipLaunchKernelGGL(k_guess<64>, g, dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL(k_guess<128>, g, dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL(k_guess<256>, g, dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL(k_guess<512>, g, dim3(512), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_dt_reset(unsigned long long* min_bits, hipStream_t s)
+hipError_t launch_dt_finish(const double* partial, int n, double cfl, double* dt_out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, s, min_bits);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, s, partial, n, cfl, dt_out);
     return hipGetLastError();
 }
 

@@ -29,18 +29,31 @@ struct StageArgs {
     ErrLog* err;
     C2PStats* stats;
     int32_t step;                // error-key step number (0 = dt pass)
-    int32_t pad;
+    int32_t rows;                // rows per work item (row-parallel passes: k_prim, k_guess, k_dt)
     MlpDev mlp;                  // learned c2p guess (learned-guess kernels only)
 };
 
-hipError_t launch_stage(int system, int rs, bool stats, bool learned, const StageArgs& a, int grid, int threads,
-                        hipStream_t s);
+// One stage: fused (c2p inside the stage kernel) or, with prims, the flux
+// pass of the two-pass stage, which reads the primitive planes k_prim wrote.
+hipError_t launch_stage(int system, int rs, bool stats, bool learned, bool prims, const StageArgs& a, int grid,
+                        int threads, hipStream_t s);
+// The c2p pass of the two-pass stage (grid x over work items, y over rows).
+hipError_t launch_prim(int system, bool stats, bool learned, const StageArgs& a, int grid, int threads,
+                       hipStream_t s);
 bool stage_variant_supported(int threads);
-int stage_occupancy(int system, int rs, bool stats, bool learned, int threads);
+int stage_occupancy(int system, int rs, bool stats, bool learned, bool prims, int threads);
 int stage_max_rows();
-hipError_t launch_dt(int system, const StageArgs& a, unsigned long long* min_bits, int grid, hipStream_t s);
-hipError_t launch_dt_finish(unsigned long long* min_bits, double cfl, double* dt_out, hipStream_t s);
-hipError_t launch_dt_reset(unsigned long long* min_bits, hipStream_t s);
+// The learned c2p guess of every zone of a.in_slot into the blocks' guess
+// planes.  Row-parallel passes (k_guess, k_prim, k_dt) take one (work item,
+// row) unit per workgroup iteration, a.rows rows per item, with a 1-D grid
+// striding over the units.
+hipError_t launch_guess(const StageArgs& a, int grid, int threads, hipStream_t s);
+// CFL reduction: each workgroup writes the minimum of dl / (max signal
+// speed x c) over its zones to partial[blockIdx.x] (grid <= DT_MAX_GRID);
+// k_dt_finish reduces the partials to dt = cfl x min.
+constexpr int DT_MAX_GRID = 4096;
+hipError_t launch_dt(int system, const StageArgs& a, double* partial, int grid, hipStream_t s);
+hipError_t launch_dt_finish(const double* partial, int n, double cfl, double* dt_out, hipStream_t s);
 
 hipError_t launch_try_to_primitive(int system, const ZoneParams& zp, int64_t n, const double* cons,
                                    const double* guess, double* prim, int32_t* code, int32_t* iters, hipStream_t s);

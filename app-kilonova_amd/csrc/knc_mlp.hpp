@@ -150,15 +150,20 @@ __device__ inline float mlp_wave(const MlpDev& M, const float x[4])
     return M.b3 + (g == 0 ? part[0] : part[1]);
 }
 
-// Pressure guess of one zone from the wave inference: p = D 2^(y/tscale + tshift).
-__device__ inline double mlp_guess_wave(const MlpDev& M, double D, double S2, double tau, double gamma)
+// log2(p/D) guessed for one zone by the wave inference: y/tscale + tshift.
+__device__ inline float mlp_log2_guess_wave(const MlpDev& M, double D, double S2, double tau, double gamma)
 {
     float f[4], x[4];
     mlp_raw_features(D, S2, tau, gamma, f);
 #pragma unroll
     for (int m = 0; m < 4; ++m) x[m] = (f[m] - M.norm.shift[m]) * M.norm.scale[m];
     const float y = mlp_wave(M, x);
-    const float l2 = y / M.norm.tscale + M.norm.tshift;
+    return y / M.norm.tscale + M.norm.tshift;
+}
+
+// Pressure guess from its log2 form: p = D 2^l2.
+__device__ inline double mlp_guess_pressure(double D, float l2)
+{
     return D * (double)__builtin_amdgcn_exp2f(l2);
 }
 

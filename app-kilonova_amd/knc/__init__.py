@@ -16,11 +16,16 @@ def hydro_from_spec(spec):
                  spec.gamma_law_index, spec.plm_theta, spec.cfl_number, spec.light_speed)
 
 
-def context_from_setup(setup, **options):
-    """A Context holding every block and boundary of a setups.Setup."""
+def context_from_setup(setup, excision=False, **options):
+    """A Context holding every block and boundary of a setups.Setup; with
+    excision=True also the setup's moving excision surfaces and block source."""
     ctx = Context(hydro_from_spec(setup.hydro), **options)
     for b in setup.blocks:
         ctx.insert_block(b.index, b.rv, b.qv, b.conserved, b.scalar_mass)
     for side, bd in ((INNER, setup.inner), (OUTER, setup.outer)):
         ctx.set_boundary(side, bd.mode, bd.primitive, bd.scalar)
+    if excision and setup.excision is not None:
+        e = setup.excision
+        ctx.set_excision(e.inner_radius, e.outer_radius, e.inner_speed, e.outer_speed, e.delay, e.check_every,
+                         setup.block_source)
     return ctx

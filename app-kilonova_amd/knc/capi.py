@@ -45,13 +45,35 @@ class Options(ctypes.Structure):
     _fields_ = [("launch_mode", ctypes.c_int32), ("c2p_guess", ctypes.c_int32),
                 ("collect_stats", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("stage_threads", ctypes.c_int32)]
+                ("stage_threads", ctypes.c_int32), ("stage_passes", ctypes.c_int32)]
 
 
 class Error(ctypes.Structure):
     _fields_ = [("code", ctypes.c_int32), ("block", ctypes.c_int32), ("i", ctypes.c_int32),
                 ("j", ctypes.c_int32), ("r", ctypes.c_double), ("theta", ctypes.c_double),
                 ("value", ctypes.c_double), ("conserved", ctypes.c_double * 4)]
+
+
+class BlockData(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int32), ("pad", ctypes.c_int32), ("radial_vertices", ctypes.c_void_p),
+                ("conserved", ctypes.c_void_p), ("scalar_mass", ctypes.c_void_p)]
+
+
+BLOCK_SOURCE_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                   ctypes.POINTER(BlockData))
+
+
+class Excision(ctypes.Structure):
+    _fields_ = [("inner_radius", ctypes.c_double), ("outer_radius", ctypes.c_double),
+                ("inner_speed", ctypes.c_double), ("outer_speed", ctypes.c_double), ("delay", ctypes.c_double),
+                ("check_every", ctypes.c_int32), ("pad", ctypes.c_int32), ("source", BLOCK_SOURCE_FN),
+                ("user", ctypes.c_void_p)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("stage_threads", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
+                ("stage_passes", ctypes.c_int32), ("candidates", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("ms_per_stage", ctypes.c_double), ("default_ms_per_stage", ctypes.c_double)]
 
 
 class C2PStats(ctypes.Structure):
@@ -118,6 +140,9 @@ def lib():
     L.knc_synchronize.argtypes = [P]
     L.knc_set_timing.argtypes = [P, I32]
     L.knc_last_stage_time.argtypes = [P, P, P]
+    L.knc_set_excision.argtypes = [P, ctypes.POINTER(Excision)]
+    L.knc_excision_counts.argtypes = [P, P, P]
+    L.knc_autotune.argtypes = [P, I32, I32, ctypes.POINTER(TuneResult)]
     L.knc_hydro_try_to_primitive.argtypes = [ctypes.POINTER(Hydro), I64, P, P, P, P, P]
     L.knc_hydro_to_conserved.argtypes = [ctypes.POINTER(Hydro), I64, P, P]
     L.knc_hydro_intercell_flux.argtypes = [ctypes.POINTER(Hydro), I64, P, P, P, P, I32, P, P]
@@ -146,10 +171,10 @@ class Context:
     """A KNC ``State`` resident on one GPU plus the scheme that advances it."""
 
     def __init__(self, hydro, launch_mode=PER_BLOCK, c2p_guess=GUESS_PLAIN, collect_stats=False,
-                 rows_per_item=0, device=0, use_graph=True, stage_threads=0):
+                 rows_per_item=0, device=0, use_graph=True, stage_threads=0, stage_passes=0):
         self.hydro = hydro
         self.options = Options(launch_mode, c2p_guess, int(collect_stats), rows_per_item, device, int(use_graph),
-                               stage_threads)
+                               stage_threads, stage_passes)
         h = ctypes.c_void_p()
         _check(lib().knc_context_create(ctypes.byref(hydro), ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h
@@ -242,9 +267,76 @@ class Context:
         dt = ctypes.c_double()
         err = Error()
         rc = lib().knc_advance(self._h, int(fold), ctypes.byref(dt), ctypes.byref(err))
+        self._sync_block_set()
+        self._birth_keep = None
+        birth_error = self.__dict__.pop("_birth_error", None)
+        if birth_error is not None:
+            raise birth_error
         if rc:
             self._raise(rc, err)
         return dt.value
+
+    # -- excision (add_remove_blocks, src/scheme.rs:179-206) -----------------
+    def set_excision(self, inner_radius, outer_radius, inner_speed, outer_speed, delay=0.0, check_every=1,
+                     source=None):
+        """Moving excision surfaces.  ``source(index, time)`` returns
+        ``(radial_vertices, conserved (nr, nq, 4), scalar_mass (nr, nq))`` of a
+        new outer block (BlockState::from_model), or None for no births."""
+        self._source_py = source
+        self._birth_keep = None
+        cb = BLOCK_SOURCE_FN(self._birth) if source is not None else BLOCK_SOURCE_FN()
+        self._source_cb = cb
+        ex = Excision(float(inner_radius), float(outer_radius), float(inner_speed), float(outer_speed),
+                      float(delay), int(check_every), 0, cb, None)
+        _check(lib().knc_set_excision(self._h, ctypes.byref(ex)))
+
+    def clear_excision(self):
+        _check(lib().knc_set_excision(self._h, None))
+        self._source_py = None
+        self._source_cb = None
+
+    def _birth(self, user, index, time, out):
+        try:
+            rv, cons, sm = self._source_py(int(index), float(time))
+            rv = _f64(rv)
+            nr = rv.size - 1
+            nq = next(iter(self.shapes.values()))[1]
+            cons = _f64(cons).reshape(nr, nq, 4)
+            sm = _f64(sm).reshape(nr, nq)
+            self._birth_keep = (rv, cons, sm)
+            out[0].nr = nr
+            out[0].radial_vertices = rv.ctypes.data
+            out[0].conserved = cons.ctypes.data
+            out[0].scalar_mass = sm.ctypes.data
+            self.shapes[int(index)] = (nr, nq)
+            self.vertices[int(index)] = (rv.copy(), self.vertices[min(self.shapes)][1].copy())
+            return 0
+        except Exception as e:  # reported through the advance call's error
+            self._birth_error = e
+            return 1
+
+    def _sync_block_set(self):
+        n, lo, hi = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().knc_block_count(self._h, ctypes.byref(n), ctypes.byref(lo), ctypes.byref(hi)))
+        for k in [k for k in self.shapes if n.value == 0 or k < lo.value or k > hi.value]:
+            self.shapes.pop(k)
+            self.vertices.pop(k, None)
+
+    def excision_counts(self):
+        r, a = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().knc_excision_counts(self._h, ctypes.byref(r), ctypes.byref(a)))
+        return r.value, a.value
+
+    # -- autotuner ----------------------------------------------------------------
+    def autotune(self, trials=3, force=False):
+        """Times every supported stage shape on the current mesh and keeps the
+        fastest (knc_autotune); returns the chosen shape and timings."""
+        r = TuneResult()
+        _check(lib().knc_autotune(self._h, int(trials), int(force), ctypes.byref(r)))
+        return dict(stage_threads=r.stage_threads, rows_per_item=r.rows_per_item, stage_passes=r.stage_passes,
+                    candidates=r.candidates,
+                    from_cache=bool(r.from_cache), ms_per_stage=r.ms_per_stage,
+                    default_ms_per_stage=r.default_ms_per_stage)
 
     def synchronize(self):
         _check(lib().knc_synchronize(self._h))

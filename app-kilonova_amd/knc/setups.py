@@ -63,6 +63,23 @@ class Boundary:
 
 
 @dataclass
+class Excision:
+    """Moving excision surfaces (Mesh, src/mesh.rs:312-336): IES(t) =
+    inner_radius + max(t - delay, 0) inner_speed, OES(t) likewise; checked
+    before every step whose iteration is a multiple of check_every."""
+    inner_radius: float
+    outer_radius: float
+    inner_speed: float = 0.0
+    outer_speed: float = 0.0
+    delay: float = 0.0
+    check_every: int = 1
+
+    @property
+    def moving(self):
+        return self.inner_speed > 0.0 or self.outer_speed > 0.0
+
+
+@dataclass
 class Setup:
     name: str
     hydro: HydroSpec
@@ -71,9 +88,8 @@ class Setup:
     outer: Boundary
     fold: int = 10
     steps: int = 100
-    inner_excision_speed: float = 0.0
-    outer_excision_speed: float = 0.0
-    inner_radius: float = 0.0
+    excision: Excision = None
+    block_source: object = None    # (index, time) -> (rv, conserved, scalar_mass) of a new outer block
     model_extra: dict = field(default_factory=dict)
 
     @property
@@ -227,7 +243,7 @@ def jet_in_cloud(nr=1024, nq=256, nblocks=8, seed=2, steps=1000, riemann=0, corr
     blocks = _blocks_from_prim(hydro, rv, qv, sizes, prim, np.zeros_like(rho))
     inner = _jet_ghost(log_vertices(1.0, 100.0, nr, -2, 0), qv, 10.0, 1e-3, 1e-2, 0.1, 1e-4)
     return Setup("jet_in_cloud", hydro, blocks, inner, Boundary(1), fold=10, steps=steps,
-                 inner_excision_speed=0.5 * LIGHT_SPEED, inner_radius=1.0)
+                 excision=Excision(inner_radius=1.0, outer_radius=100.0, inner_speed=0.5 * LIGHT_SPEED))
 
 
 def uneven_sizes(total, nblocks, choices, rng):
@@ -285,28 +301,69 @@ def bomb_jet(nr=4096, nq=1024, nblocks=32, seed=5, steps=2000, nshell=64, rieman
     return Setup("bomb_jet", hydro, blocks, inner, Boundary(1), fold=10, steps=steps)
 
 
-def wind_shock(nq=256, nblocks=64, seed=3, steps=200, choices=(64, 128, 256, 512), riemann=0):
+class UnevenChain:
+    """Radial blocks of seeded sizes from `choices`, block k's size drawn
+    from its own generator (seed, k), so the chain extends outward without
+    changing the blocks already made.  Vertices are global: r0 10^(dlog n)
+    for global radial vertex n."""
+
+    def __init__(self, seed, choices, r0, dlog):
+        self.seed, self.choices, self.r0, self.dlog = seed, list(choices), r0, dlog
+        self._sizes = []
+
+    def size(self, k):
+        while len(self._sizes) <= k:
+            j = len(self._sizes)
+            self._sizes.append(int(np.random.default_rng([self.seed, j]).choice(self.choices)))
+        return self._sizes[k]
+
+    def offset(self, k):
+        return sum(self.size(j) for j in range(k))
+
+    def vertices(self, k):
+        n0 = self.offset(k)
+        return self.r0 * 10.0 ** (self.dlog * np.arange(n0, n0 + self.size(k) + 1, dtype=np.float64))
+
+
+def wind_shock(nq=256, nblocks=64, seed=3, steps=200, choices=(64, 128, 256, 512), riemann=0, check_every=50):
     """SRHD wind shock: `nblocks` radial blocks of seeded uneven sizes from
-    `choices` x nq, r from 1 over 4 decades log; slow wind (u=1) with
-    rho ~ r^-2 and p = 1e-3 rho; the inner boundary injects a fast wind with
-    Gamma=30; HLLE, RK2; excision surfaces move outward (IES at 0.1c, OES at c)."""
+    `choices` x nq, r from 1 over 4 decades log (the log spacing continues
+    past r = 1e4 for blocks born later); slow wind (u=1) with rho ~ r^-2
+    (1 + 0.1 cos 4 theta) and p = 1e-3 rho; the inner boundary injects a
+    fast wind with Gamma=30; HLLE, RK2, adaptive dt.  Excision every
+    `check_every` steps: both surfaces move far faster than the mesh, so
+    every check removes the innermost block and adds block outer+1 from the
+    wind model (the reference's rule of src/scheme.rs:179-206 at a fixed
+    cadence)."""
     hydro = HydroSpec(1, riemann, 2, True, 4.0 / 3.0, 1.5, 0.4)
-    rng = np.random.default_rng(seed)
-    sizes = [int(x) for x in rng.choice(list(choices), nblocks)]
-    nr = sum(sizes)
+    gamma = hydro.gamma_law_index
+    probe = UnevenChain(seed, choices, 1.0, 1.0)
+    nr = sum(probe.size(k) for k in range(nblocks))
+    chain = UnevenChain(seed, choices, 1.0, 4.0 / nr)
     qv = polar_vertices(nq)
-    rv = log_vertices(1.0, 1e4, nr)
-    rc, qc = cell_centers(rv, qv)
-    rho = rc ** -2.0 * (1.0 + 0.1 * np.cos(4.0 * qc))
-    prim = np.stack([rho, np.ones_like(rho), np.zeros_like(rho), 1e-3 * rho], -1)
-    blocks = _blocks_from_prim(hydro, rv, qv, sizes, prim, np.zeros_like(rho))
-    rg = log_vertices(1.0, 1e4, nr, -2, 0)
+
+    def model_block(k):
+        rv = chain.vertices(k)
+        rc, qc = cell_centers(rv, qv)
+        rho = rc ** -2.0 * (1.0 + 0.1 * np.cos(4.0 * qc))
+        prim = np.stack([rho, np.ones_like(rho), np.zeros_like(rho), 1e-3 * rho], -1)
+        cons = to_conserved(1, gamma, prim) * cell_volumes(rv, qv)[..., None]
+        return rv, cons, np.zeros(rho.shape)
+
+    blocks = []
+    for k in range(nblocks):
+        rv, cons, sm = model_block(k)
+        blocks.append(Block(k, rv, qv.copy(), cons, sm))
+    rg = chain.r0 * 10.0 ** (chain.dlog * np.arange(-2, 1, dtype=np.float64))
     rcg = np.sqrt(rg[:-1] * rg[1:])[:, None] * np.ones((1, nq))
     u = np.sqrt(30.0 ** 2 - 1.0)
     gp = np.stack([rcg ** -2.0, u + 0 * rcg, 0 * rcg, 1e-3 * rcg ** -2.0], -1)
-    return Setup("wind_shock", hydro, blocks, _ghost(gp, np.ones((2, nq))), Boundary(1), fold=50, steps=steps,
-                 inner_excision_speed=0.1 * LIGHT_SPEED, outer_excision_speed=LIGHT_SPEED, inner_radius=1.0,
-                 model_extra=dict(sizes=sizes))
+    fast = 1e20 * LIGHT_SPEED
+    exc = Excision(inner_radius=1.0, outer_radius=float(blocks[-1].rv[-1]), inner_speed=fast, outer_speed=fast,
+                   check_every=check_every)
+    return Setup("wind_shock", hydro, blocks, _ghost(gp, np.ones((2, nq))), Boundary(1), fold=check_every,
+                 steps=steps, excision=exc, block_source=lambda index, time: model_block(index),
+                 model_extra=dict(sizes=[b.nr for b in blocks]))
 
 
 WORKLOADS = {"euler_blast": euler_blast, "jet_in_cloud": jet_in_cloud, "bomb_jet": bomb_jet,

@@ -25,7 +25,11 @@ launch on average; achieved = 100 B x zones / average stage duration,
 measured with HIP events on the context's stream.
 
 CPU baseline: the C++ oracle (oracle/) on the host cores, a bounded sample
-of the same mesh (a few RK2 steps), reported as zone-updates/s.
+of the same mesh (100 RK2 steps after one warm-up step by default, about
+10-20 s on 16 threads), reported as zone-updates/s.
+
+The batched paths run the library's autotuner (knc_autotune) on the mesh
+before timing; the chosen shapes are reported.
 """
 import argparse
 import json
@@ -48,12 +52,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(setup, launch_mode, steps, warmup, fold, net=None, **shape):
+def run_gpu(setup, launch_mode, steps, warmup, fold, net=None, tune=False, **shape):
     import knc
     guess = knc.GUESS_LEARNED if net is not None else knc.GUESS_PLAIN
     ctx = knc.context_from_setup(setup, launch_mode=launch_mode, use_graph=True, c2p_guess=guess, **shape)
     if net is not None:
         net.attach(ctx)
+    ctx.tuned = ctx.autotune() if tune else None
+    if ctx.tuned:
+        log(f"bench: autotuned shape {ctx.tuned['stage_threads']} columns x {ctx.tuned['rows_per_item']} rows "
+            f"({ctx.tuned['ms_per_stage']:.4f} ms vs default {ctx.tuned['default_ms_per_stage']:.4f} ms)")
     ctx.set_timing(True)
     done = 0
     while done < warmup:
@@ -112,9 +120,9 @@ def newton_stats(setup, fold, net, steps=20):
     return out
 
 
-def read_traffic():
-    """HBM bytes per stage launch from the committed rocprofv3 PMC summary,
-    or None."""
+def read_traffic(learned):
+    """HBM bytes per stage launch of the plain- or learned-guess stage kernel
+    from the committed rocprofv3 PMC summary, or None."""
     path = os.path.join(ROOT, "profiles", "k_stage_bomb_jet_pmc.txt")
     if not os.path.exists(path):
         return None
@@ -124,9 +132,26 @@ def read_traffic():
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_stage_launch"])
+        return float(vals["hbm_bytes_per_stage_launch" + ("_learned" if learned else "")])
     except (KeyError, ValueError):
         return None
+
+
+# FP64 operations of one Newton iteration of the c2p as restated (add, sub,
+# mul, div, sqrt each 1): Q = E + p, den = Q^2 - S^2 (2), sqrt, p Q,
+# F = (gamma-1)(den - D sq - pQ) - pQ (5), dp = F den / (gamma p S^2 - den Q) (6),
+# p - dp, the convergence test 1e-12 (E + p) (2).
+NEWTON_FLOPS = 19
+FP64_VECTOR_PEAK = 78.6e12
+
+
+def c2p_roofline(zones, stage_ms, mean_iterations):
+    """Time the c2p Newton solves of one stage would take at the FP64 vector
+    peak, as a fraction of the measured stage time."""
+    flops = zones * mean_iterations * NEWTON_FLOPS
+    t = flops / FP64_VECTOR_PEAK
+    return {"flops_per_iteration": NEWTON_FLOPS, "mean_iterations": mean_iterations, "gflop_per_stage": flops / 1e9,
+            "ms_at_fp64_peak": 1e3 * t, "frac_of_stage": 1e3 * t / stage_ms, "peak_tflops": FP64_VECTOR_PEAK / 1e12}
 
 
 def main():
@@ -135,7 +160,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fold", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--nr", type=int, default=4096)
@@ -144,6 +169,7 @@ def main():
     ap.add_argument("--rows", type=int, default=0, help="radial rows per work item (0: library choice)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-stats", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; multi-GPU sharding of the "
@@ -157,7 +183,8 @@ def main():
         f"steps={args.steps} warmup={args.warmup} fold={args.fold}")
 
     shape = dict(stage_threads=args.threads, rows_per_item=args.rows)
-    ctx_b, wall_b, stage_ms_b = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, **shape)
+    tune = not args.no_tune and not (args.threads or args.rows)
+    ctx_b, wall_b, stage_ms_b = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, tune=tune, **shape)
     rate_b = zones * args.steps / wall_b
     log(f"bench: batched {rate_b / 1e9:.3f} Gzu/s, stage {stage_ms_b:.4f} ms")
 
@@ -171,7 +198,8 @@ def main():
     learned = None
     ctx_l = None
     if net is not None:
-        ctx_l, wall_l, stage_ms_l = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, net=net, **shape)
+        ctx_l, wall_l, stage_ms_l = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, net=net,
+                                            tune=tune, **shape)
         rate_l = zones * args.steps / wall_l
         log(f"bench: learned+batched {rate_l / 1e9:.3f} Gzu/s, stage {stage_ms_l:.4f} ms")
         learned = {"value": rate_l, "ms_per_step": 1e3 * wall_l / args.steps, "stage_ms": stage_ms_l}
@@ -221,7 +249,10 @@ def main():
                             if learned else "batched persistent stage launches, hipGraph per fold, plain c2p guess"),
                    "zones": zones},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(learned is not None)},
+        "c2p_fp64_roofline": (c2p_roofline(zones, head_stage, newton["learned" if learned else "plain"]
+                                           ["mean_iterations"]) if newton else None),
+        "autotuned": {"batched": ctx_b.tuned, "learned_batched": ctx_l.tuned if ctx_l is not None else None},
         "cpu_baseline": cpu,
         "paths": {"plain_per_block": plain,
                   "batched": {"value": rate_b, "ms_per_step": 1e3 * wall_b / args.steps, "stage_ms": stage_ms_b},

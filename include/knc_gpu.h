@@ -85,6 +85,10 @@ typedef struct knc_options {
     int32_t use_graph;       /* batched mode: 1 = capture fold steps in a hipGraph */
     int32_t stage_threads;   /* stage workgroup size = polar columns per work item
                                 (64, 128, 256, 512); 0 = chosen from nq */
+    int32_t stage_passes;    /* 1: fused stage (c2p inside the stage kernel);
+                                2: a c2p pass writing primitive planes, then the
+                                fused reconstruction -> flux -> update pass;
+                                0 = 1 (knc_autotune may choose either) */
 } knc_options;
 
 /* HydroError (src/physics/mod.rs:109-117): code, the failing zone and its
@@ -144,12 +148,76 @@ int knc_get_time(knc_context* ctx, double* time, int64_t* iteration);
 int knc_time_step(knc_context* ctx, double* dt, knc_error* err);
 
 /* nsteps RK steps (RungeKuttaOrder::try_advance_async with the stage of
- * try_advance_rk, src/scheme.rs:16-173) with a fixed dt. */
+ * try_advance_rk, src/scheme.rs:16-173) with a fixed dt, without excision. */
 int knc_advance_steps(knc_context* ctx, double dt, int32_t nsteps, knc_error* err);
 
-/* scheme::advance without excision (src/scheme.rs:212-240): dt from
- * time_step once, then `fold` RK steps.  dt_used may be NULL. */
+/* scheme::advance (src/scheme.rs:212-240): dt from time_step once, then
+ * `fold` RK steps, each preceded by the excision check of
+ * add_remove_blocks when excision is set (knc_set_excision).  dt_used may
+ * be NULL.  The time and iteration advance with every completed step. */
 int knc_advance(knc_context* ctx, int32_t fold, double* dt_used, knc_error* err);
+
+/* ---- moving excision surfaces ------------------------------------------ */
+/* A new outer block, written by a knc_block_source_fn: the buffers must stay
+ * valid until the callback is called again or the advance call returns. */
+typedef struct knc_block_data {
+    int32_t nr;
+    int32_t pad;
+    const double* radial_vertices;   /* nr+1 */
+    const double* conserved;         /* (nr, nq, 4), volume-integrated */
+    const double* scalar_mass;       /* (nr, nq) */
+} knc_block_data;
+
+/* Block birth (BlockState::from_model on mesh.subgrid(index) at `time`,
+ * src/scheme.rs:197-205).  Returns 0, or nonzero to abort the advance call
+ * with KNC_ERR_INVALID_ARGUMENT. */
+typedef int32_t (*knc_block_source_fn)(void* user, int32_t index, double time, knc_block_data* out);
+
+/* Excision surfaces of Mesh (src/mesh.rs:312-336): IES(t) = inner_radius +
+ * max(t - delay, 0) * inner_speed, OES(t) likewise from outer_radius.  With
+ * either speed positive (moving_excision_surfaces), before each RK step whose
+ * iteration is a multiple of check_every (1: every step, as
+ * src/scheme.rs:228-232) the innermost block is removed if its outer radius
+ * is below IES(t), and block outer+1 is added from `source` if the outermost
+ * block's outer radius is below OES(t) (src/scheme.rs:179-206).  The
+ * innermost block is never removed when it is the only one.  source may be
+ * NULL (no births). */
+typedef struct knc_excision {
+    double inner_radius;
+    double outer_radius;
+    double inner_speed;
+    double outer_speed;
+    double delay;
+    int32_t check_every;
+    int32_t pad;
+    knc_block_source_fn source;
+    void* user;
+} knc_excision;
+
+/* Sets (or with NULL clears) the excision surfaces of a context. */
+int knc_set_excision(knc_context* ctx, const knc_excision* excision);
+/* Blocks removed and added by excision since the context was created. */
+int knc_excision_counts(knc_context* ctx, int64_t* removed, int64_t* added);
+
+/* ---- autotuner (no counterpart in the reference) ------------------------ */
+/* Times the stage kernels of the context's current mesh for each supported
+ * workgroup shape (polar columns per work item), radial rows per work item
+ * and stage form (fused or two-pass), keeps the fastest for later advance
+ * calls, and reports it.  The
+ * state, time and statistics are unchanged.  `trials` timed RK steps per
+ * candidate (0: 3).  Results are cached per mesh shape and configuration
+ * within the process; force = 1 re-measures. */
+typedef struct knc_tune_result {
+    int32_t stage_threads;
+    int32_t rows_per_item;
+    int32_t stage_passes;
+    int32_t candidates;     /* shapes timed (0 when taken from the cache) */
+    int32_t from_cache;
+    int32_t pad;
+    double ms_per_stage;    /* of the chosen shape */
+    double default_ms_per_stage;   /* of the library's default choice */
+} knc_tune_result;
+int knc_autotune(knc_context* ctx, int32_t trials, int32_t force, knc_tune_result* out);
 
 int knc_get_c2p_stats(knc_context* ctx, knc_c2p_stats* out);
 int knc_reset_c2p_stats(knc_context* ctx);

@@ -199,6 +199,8 @@ class Sim:
         self.h = h
         self._p = ctypes.c_void_p(lib().ko_sim_create(ctypes.byref(h), int(nthreads)))
         self.nq = None
+        self.qv = None
+        self.radial_vertices = []
 
     def __del__(self):
         if getattr(self, "_p", None):
@@ -211,10 +213,51 @@ class Sim:
         cons = _f64(cons).reshape(nr, nq, 4)
         sm = _f64(scalar_mass).reshape(nr, nq)
         self.nq = nq
+        self.qv = qv.copy()
         lib().ko_sim_push_block(self._p, nr, nq, _p(rv), _p(qv), _p(cons), _p(sm))
+        self.radial_vertices.append(rv.copy())
 
     def pop_inner_block(self):
         lib().ko_sim_pop_inner_block(self._p)
+        self.radial_vertices.pop(0)
+
+    def blocks(self):
+        """(cons, scalar_mass) of every block, inner to outer."""
+        return [self.get_block(b, rv.size - 1, self.nq) for b, rv in enumerate(self.radial_vertices)]
+
+    def excise(self, ex, source, first_index):
+        """add_remove_blocks (src/scheme.rs:179-206) at the current time:
+        remove the inner block when its outer radius is inside the IES
+        (src/scheme.rs:191-194), add block outer+1 from `source` when the
+        outer block's outer radius is inside the OES (src/scheme.rs:196-205);
+        surfaces r0 + max(t - delay, 0) v (src/mesh.rs:318-336).  The only
+        block is never removed.  Returns the index of the inner block."""
+        t = self.time
+        ies = ex.inner_radius + max(t - ex.delay, 0.0) * ex.inner_speed
+        oes = ex.outer_radius + max(t - ex.delay, 0.0) * ex.outer_speed
+        outer_index = first_index + len(self.radial_vertices) - 1
+        remove = len(self.radial_vertices) > 1 and self.radial_vertices[0][-1] < ies
+        add = source is not None and self.radial_vertices[-1][-1] < oes
+        if remove:
+            self.pop_inner_block()
+            first_index += 1
+        if add:
+            rv, cons, sm = source(outer_index + 1, t)
+            self.push_block(rv, self.qv, cons, sm)
+        return first_index
+
+    def scheme_advance(self, fold, ex=None, source=None, first_index=0):
+        """scheme::advance (src/scheme.rs:212-240): dt from time_step once,
+        then `fold` RK steps, each preceded by the excision check when the
+        surfaces move (src/scheme.rs:228-232; every ex.check_every
+        iterations).  Returns (dt, index of the inner block)."""
+        dt = self.time_step()
+        moving = ex is not None and (ex.inner_speed > 0.0 or ex.outer_speed > 0.0)
+        for _ in range(fold):
+            if moving and self.iteration % ex.check_every == 0:
+                first_index = self.excise(ex, source, first_index)
+            self.advance(dt, 1)
+        return dt, first_index
 
     def num_blocks(self):
         return lib().ko_sim_num_blocks(self._p)

@@ -113,3 +113,23 @@ def test_learned_mode_requires_a_network():
     ctx = knc.context_from_setup(s, c2p_guess=knc.GUESS_LEARNED)
     with pytest.raises(RuntimeError, match="no network"):
         ctx.advance(1)
+
+
+def test_learned_two_pass_stage_equals_fused(trained):
+    """The learned guess feeds the c2p pass of the two-pass stage exactly as
+    it feeds the fused stage: same bits, and no fallbacks."""
+    net, _ = trained
+    s = knc.setups.bomb_jet(nr=512, nq=128, nblocks=8, nshell=8)
+    fused = knc.context_from_setup(s, launch_mode=knc.BATCHED, c2p_guess=knc.GUESS_LEARNED)
+    two = knc.context_from_setup(s, launch_mode=knc.BATCHED, c2p_guess=knc.GUESS_LEARNED, collect_stats=True,
+                                 stage_passes=2)
+    net.attach(fused)
+    net.attach(two)
+    for _ in range(3):
+        assert fused.advance(5) == two.advance(5)
+    for (x, m), (y, n) in zip(gpu_state(fused), gpu_state(two)):
+        assert np.array_equal(x, y) and np.array_equal(m, n)
+    st = two.c2p_stats()
+    # the c2p pass recovers each zone once per stage: 3 x 5 steps x 2 stages
+    assert int(st["hist"].sum()) == 30 * s.total_zones
+    assert st["fallbacks"] <= 1e-3 * st["zones"]

@@ -94,10 +94,10 @@ def test_euler_blast_one_step():
     assert max(err.values()) <= ONE_STEP, err
 
 
-@pytest.mark.parametrize("mode", [knc.PER_BLOCK, knc.BATCHED])
-def test_euler_blast_1000_steps(mode):
+@pytest.mark.parametrize("mode,passes", [(knc.PER_BLOCK, 1), (knc.BATCHED, 1), (knc.BATCHED, 2)])
+def test_euler_blast_1000_steps(mode, passes):
     s = knc.setups.euler_blast()
-    ctx, sim = _run_pair(s, 1000, 10, launch_mode=mode)
+    ctx, sim = _run_pair(s, 1000, 10, launch_mode=mode, stage_passes=passes)
     err = rel_l1(gpu_state(ctx), oracle_state(sim, s.blocks))
     assert max(err.values()) <= MANY_STEPS, err
 
@@ -109,25 +109,30 @@ def test_jet_in_cloud_one_step_full_size():
     assert max(err.values()) <= ONE_STEP, err
 
 
-@pytest.mark.parametrize("solver", [knc.HLLE, knc.HLLC])
-def test_jet_in_cloud_1000_steps_reduced(solver):
+@pytest.mark.parametrize("solver,passes", [(knc.HLLE, 1), (knc.HLLC, 1), (knc.HLLC, 2)])
+def test_jet_in_cloud_1000_steps_reduced(solver, passes):
     s = knc.setups.jet_in_cloud(nr=256, nq=64, nblocks=4, riemann=solver, corr=4.0)
-    ctx, sim = _run_pair(s, 1000, 10, launch_mode=knc.BATCHED)
+    ctx, sim = _run_pair(s, 1000, 10, launch_mode=knc.BATCHED, stage_passes=passes)
     err = rel_l1(gpu_state(ctx), oracle_state(sim, s.blocks))
     assert max(err.values()) <= MANY_STEPS, err
 
 
 def test_bomb_jet_uneven_blocks_per_block_equals_batched():
+    """Per-block, batched (graph / no graph) and two-pass stages give the
+    same bits."""
     s = knc.setups.bomb_jet(nr=512, nq=128, nblocks=8, nshell=8)
-    a = knc.context_from_setup(s, launch_mode=knc.PER_BLOCK)
-    b = knc.context_from_setup(s, launch_mode=knc.BATCHED, use_graph=True)
-    c = knc.context_from_setup(s, launch_mode=knc.BATCHED, use_graph=False, rows_per_item=5)
+    ctxs = [knc.context_from_setup(s, launch_mode=knc.PER_BLOCK),
+            knc.context_from_setup(s, launch_mode=knc.BATCHED, use_graph=True),
+            knc.context_from_setup(s, launch_mode=knc.BATCHED, use_graph=False, rows_per_item=5),
+            knc.context_from_setup(s, launch_mode=knc.BATCHED, stage_passes=2),
+            knc.context_from_setup(s, launch_mode=knc.PER_BLOCK, stage_passes=2, rows_per_item=7)]
     for _ in range(3):
-        assert a.advance(10) == b.advance(10) == c.advance(10)
-    sa, sb, sc = gpu_state(a), gpu_state(b), gpu_state(c)
-    for (x, m), (y, n), (z, o) in zip(sa, sb, sc):
-        assert np.array_equal(x, y) and np.array_equal(m, n)
-        assert np.array_equal(x, z) and np.array_equal(m, o)
+        dts = [c.advance(10) for c in ctxs]
+        assert all(d == dts[0] for d in dts)
+    ref = gpu_state(ctxs[0])
+    for c in ctxs[1:]:
+        for (x, m), (y, n) in zip(gpu_state(c), ref):
+            assert np.array_equal(x, y) and np.array_equal(m, n)
 
 
 def test_bomb_jet_one_step_vs_oracle():
@@ -152,12 +157,14 @@ def test_negative_density_is_reported_with_position():
     assert e.value.r == eo.value.r and e.value.theta == eo.value.theta
 
 
-@pytest.mark.parametrize("nq,threads", [(256, 64), (100, 64), (65, 64), (130, 128)])
-def test_polar_tiles_match_untiled_and_oracle(nq, threads):
+@pytest.mark.parametrize("nq,threads,passes", [(256, 64, 1), (100, 64, 1), (65, 64, 1), (130, 128, 1),
+                                                (100, 64, 2), (130, 128, 2)])
+def test_polar_tiles_match_untiled_and_oracle(nq, threads, passes):
     """Work items narrower than the mesh (halo columns recovered per item)
     give the same bits as one item spanning every column, and as the oracle."""
     s = knc.setups.bomb_jet(nr=96, nq=nq, nblocks=3, nshell=4)
-    tiled = knc.context_from_setup(s, launch_mode=knc.BATCHED, stage_threads=threads, rows_per_item=7)
+    tiled = knc.context_from_setup(s, launch_mode=knc.BATCHED, stage_threads=threads, rows_per_item=7,
+                                   stage_passes=passes)
     whole = knc.context_from_setup(s, launch_mode=knc.PER_BLOCK, stage_threads=512)
     sim = oracle_from_setup(s)
     for _ in range(2):

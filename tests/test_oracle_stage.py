@@ -180,3 +180,36 @@ def test_excision_removes_inner_block():
     sim.pop_inner_block()
     assert sim.num_blocks() == 3
     sim.advance(sim.time_step(), 2)
+
+
+def test_inner_excision_surface_removes_block_when_it_passes():
+    """IES(t) = r0 + t v (src/mesh.rs:318-321) passes the inner block's outer
+    radius between the 4th and 5th step: the block goes before step 5
+    (the check precedes each step, src/scheme.rs:228-232)."""
+    s = S.jet_in_cloud(nr=64, nq=16, nblocks=4, corr=2.0)
+    sim = oracle_from_setup(s, 1)
+    dt = sim.time_step()
+    r_out = s.blocks[0].rv[-1]
+    ex = S.Excision(inner_radius=1.0, outer_radius=100.0, inner_speed=(r_out - 1.0) / (3.5 * dt))
+    dt1, first = sim.scheme_advance(4, ex)
+    assert dt1 == dt and first == 0 and sim.num_blocks() == 4
+    _, first = sim.scheme_advance(1, ex, first_index=first)
+    assert first == 1 and sim.num_blocks() == 3
+
+
+def test_outer_excision_surface_adds_model_blocks():
+    """Each check removes the inner block and adds block outer+1 from the
+    model (src/scheme.rs:196-205); the new block continues the log spacing."""
+    s = S.wind_shock(nq=16, nblocks=6, choices=(8, 16), check_every=3)
+    sim = oracle_from_setup(s, 2)
+    first = 0
+    for _ in range(3):
+        _, first = sim.scheme_advance(3, s.excision, s.block_source, first)
+    # checks at iterations 3 and 6 (iteration 0 is at t = 0: nothing to do)
+    assert first == 2 and sim.num_blocks() == 6
+    rvs = sim.radial_vertices
+    for a, b in zip(rvs[:-1], rvs[1:]):
+        assert a[-1] == b[0]
+    ratio = rvs[-1][1:] / rvs[-1][:-1]
+    assert np.allclose(ratio, ratio[0], rtol=1e-12)
+    assert np.isclose(ratio[0], s.blocks[0].rv[1] / s.blocks[0].rv[0], rtol=1e-12)
