@@ -184,7 +184,11 @@ struct knc_context {
 
 namespace {
 
-bool learned(const knc_context* c) { return c->o.c2p_guess == KNC_GUESS_LEARNED && c->h.system == KNC_SRHD; }
+bool learned(const knc_context* c)
+{
+    return (c->o.c2p_guess == KNC_GUESS_LEARNED || c->o.c2p_guess == KNC_GUESS_LEARNED_STEP) &&
+           c->h.system == KNC_SRHD;
+}
 bool two_pass(const knc_context* c) { return c->o.stage_passes == 2; }
 
 void drop_graphs(knc_context* c)
@@ -388,14 +392,17 @@ int row_pass_grid(const knc_context* c, int nitems)
 // one.  The passes before the stage kernels (the learned-guess pass, the c2p
 // pass of the two-pass stage) cover every block first, since a block's stage
 // also reads its neighbours' edge rows.
-int launch_one_stage(knc_context* c, StageArgs a)
+int launch_one_stage(knc_context* c, StageArgs a, bool first_stage)
 {
     const bool stats = c->o.collect_stats != 0;
     const bool prims = two_pass(c);
     const bool lg = learned(c);
+    // KNC_GUESS_LEARNED_STEP evaluates the network on the first stage's input
+    // only; the later stages of the step start Newton from the same guesses.
+    const bool guess_pass = lg && (first_stage || c->o.c2p_guess == KNC_GUESS_LEARNED);
     auto pre_pass = [&](const StageArgs& b, int nitems) -> int {
         const int grid = row_pass_grid(c, nitems);
-        if (lg) {
+        if (guess_pass) {
             HIP_TRY(launch_guess(b, grid, c->threads, c->stream));
             c->last_launches += 1;
         }
@@ -442,23 +449,23 @@ int enqueue_rk_step(knc_context* c, int step)
     switch (c->h.runge_kutta_order) {
     case 1:
         a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
-        if ((rc = launch_one_stage(c, a))) return rc;
+        if ((rc = launch_one_stage(c, a, true))) return rc;
         c->cur = f1;
         break;
     case 2:
         a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
-        if ((rc = launch_one_stage(c, a))) return rc;
+        if ((rc = launch_one_stage(c, a, true))) return rc;
         a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.5;
-        if ((rc = launch_one_stage(c, a))) return rc;
+        if ((rc = launch_one_stage(c, a, false))) return rc;
         c->cur = f2;
         break;
     default:
         a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
-        if ((rc = launch_one_stage(c, a))) return rc;
+        if ((rc = launch_one_stage(c, a, true))) return rc;
         a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.75;
-        if ((rc = launch_one_stage(c, a))) return rc;
+        if ((rc = launch_one_stage(c, a, false))) return rc;
         a.in_slot = f2; a.out_slot = f1; a.avg_slot = c->cur; a.bweight = 1.0 / 3.0;
-        if ((rc = launch_one_stage(c, a))) return rc;
+        if ((rc = launch_one_stage(c, a, false))) return rc;
         c->cur = f1;
         break;
     }
@@ -717,7 +724,7 @@ std::map<std::vector<int64_t>, std::vector<int>> g_tune_cache;
 std::vector<int64_t> tune_key(const knc_context* c)
 {
     std::vector<int64_t> k = {c->h.system, c->h.riemann_solver, c->h.runge_kutta_order, c->o.collect_stats,
-                              learned(c), c->o.launch_mode, c->o.device, c->num_cu, c->nq};
+                              learned(c) ? c->o.c2p_guess : 0, c->o.launch_mode, c->o.device, c->num_cu, c->nq};
     for (auto& kv : c->blocks) k.push_back(kv.second.nr);
     return k;
 }
@@ -865,7 +872,8 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
         c->o.launch_mode = KNC_LAUNCH_PER_BLOCK;
         c->o.use_graph = 0;
     }
-    if (c->o.c2p_guess != KNC_GUESS_PLAIN && c->o.c2p_guess != KNC_GUESS_LEARNED) {
+    if (c->o.c2p_guess != KNC_GUESS_PLAIN && c->o.c2p_guess != KNC_GUESS_LEARNED &&
+        c->o.c2p_guess != KNC_GUESS_LEARNED_STEP) {
         delete c;
         return fail(KNC_ERR_INVALID_ARGUMENT, "unknown c2p guess mode");
     }

@@ -102,7 +102,7 @@ constexpr int ERR_LOG_CAP = 256;
 struct ErrEntry {
     unsigned long long key;
     int32_t code;
-    int32_t block_index;
+    int32_t pad;
     double value;
     double cons[4];
 };

@@ -25,22 +25,43 @@
 
 namespace knc {
 
-struct C2PLocal {
-    int iters;
-};
-
-__device__ inline void record_error(ErrLog* log, int step, int order, int row, int col, int code,
-                                    int block_index, double value, const Cons& u)
+// Loads and stores through the global address space.  Pointers taken from
+// the block table in device memory are generic; accessing them as global
+// lets the compiler emit global_load/global_store, where flat accesses would
+// also count against the LDS wait counter and serialise the row prefetch
+// with the LDS exchanges.
+__device__ inline double gld(const double* p, int64_t k)
 {
-    const unsigned long long key = ((unsigned long long)step << 48) | ((unsigned long long)order << 32) |
-                                   ((unsigned long long)row << 16) | (unsigned long long)col;
+    return ((const __attribute__((address_space(1))) double*)p)[k];
+}
+
+__device__ inline float gldf(const float* p, int64_t k)
+{
+    return ((const __attribute__((address_space(1))) float*)p)[k];
+}
+
+__device__ inline void gst(double* p, int64_t k, double v)
+{
+    ((__attribute__((address_space(1))) double*)p)[k] = v;
+}
+
+__device__ inline void gstf(float* p, int64_t k, float v)
+{
+    ((__attribute__((address_space(1))) float*)p)[k] = v;
+}
+
+// tag = (block order << 16) | row; the key orders failures by (step, block,
+// row, column) and the host maps the block order back to the block index.
+__device__ inline void record_error(ErrLog* log, int step, int tag, int col, int code, double value, const Cons& u)
+{
+    const unsigned long long key = ((unsigned long long)step << 48) | ((unsigned long long)(unsigned)tag << 16) |
+                                   (unsigned long long)col;
     atomicMin(&log->min_key, key);
     const unsigned int slot = atomicAdd(&log->count, 1u);
     if (slot < (unsigned int)ERR_LOG_CAP) {
         ErrEntry& e = log->entry[slot];
         e.key = key;
         e.code = code;
-        e.block_index = block_index;
         e.value = value;
         e.cons[0] = u.d; e.cons[1] = u.sr; e.cons[2] = u.sq; e.cons[3] = u.tau;
     }
@@ -51,8 +72,8 @@ __device__ inline void record_error(ErrLog* log, int step, int order, int row, i
 // volume-integrated state; rdv the reciprocal volume factor.
 template <int SYS, bool STATS, bool GUESS>
 __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, double usq, double utau,
-                                    double um, double rdv, int order, int row, int col, int block_index,
-                                    unsigned int* lds_hist, double guess)
+                                    double um, double rdv, int tag, int col, unsigned int* lds_hist,
+                                    double guess)
 {
     const Cons u{ud * rdv, usr * rdv, usq * rdv, utau * rdv};
     PS out;
@@ -100,7 +121,7 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
         if (STATS) atomicAdd(&lds_hist[it < HIST_BINS ? it : HIST_BINS - 1], 1u);
     }
     if (code != E_OK) {
-        record_error(a.err, a.step, order, row, col, code, block_index, value, u);
+        record_error(a.err, a.step, tag, col, code, value, u);
         out.P = Prim{0.0, 0.0, 0.0, 0.0};
         out.s = 0.0;
         return out;
@@ -116,23 +137,19 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
 struct RowSrc {
     const double* base;   // first plane at (row, 0): U planes, or boundary prim planes
     const float* guess;   // learned guess row (block rows of learned-guess stages)
-    int64_t np;           // plane stride
     double rvolA;         // 1/volA of the row (conserved rows)
-    int32_t fixed;        // 1: boundary primitive rows (no c2p)
-    int32_t order, row, index;
+    int32_t np;           // plane stride (nr nq < 2^26, or 2 nq for boundary rows)
+    int32_t tag;          // (block order << 16) | row, or -1 for primitive rows (no c2p)
 };
 
 __device__ inline RowSrc fixed_rows(const double* rows, int k2, int nq)
 {
     RowSrc r;
-    r.fixed = 1;
     r.base = rows + (int64_t)k2 * nq;
     r.guess = nullptr;
-    r.np = 2 * (int64_t)nq;
     r.rvolA = 1.0;
-    r.order = 0;
-    r.row = 0;
-    r.index = 0;
+    r.np = 2 * nq;
+    r.tag = -1;
     return r;
 }
 
@@ -142,14 +159,11 @@ template <bool PRIMS>
 __device__ inline RowSrc block_row(const StageArgs& a, const BlockDesc& S, int k)
 {
     RowSrc r;
-    r.fixed = PRIMS ? 1 : 0;
     r.base = (PRIMS ? S.prim : slot_of(S, a.in_slot)) + (int64_t)k * S.nq;
     r.guess = S.guess + (int64_t)k * S.nq;
-    r.np = (int64_t)S.nr * S.nq;
-    r.rvolA = rg_rvolA(S)[k];
-    r.order = S.order;
-    r.row = k;
-    r.index = S.index;
+    r.rvolA = PRIMS ? 1.0 : gld(rg_rvolA(S), k);
+    r.np = S.nr * S.nq;
+    r.tag = PRIMS ? -1 : (S.order << 16) | k;
     return r;
 }
 
@@ -188,8 +202,8 @@ __device__ inline Raw5 load_raw(const RowSrc& r, int j)
     const double* p = r.base + j;
     Raw5 x;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) x.v[f] = p[f * r.np];
-    x.g = GUESS && !r.fixed ? r.guess[j] : 0.0f;
+    for (int f = 0; f < 5; ++f) x.v[f] = gld(p, f * r.np);
+    x.g = GUESS && r.tag >= 0 ? gldf(r.guess, j) : 0.0f;
     return x;
 }
 
@@ -197,16 +211,16 @@ __device__ inline Raw5 load_raw(const RowSrc& r, int j)
 template <int SYS, bool STATS, bool GUESS>
 __device__ inline PS raw_primitive(const StageArgs& a, const RowSrc& r, const Raw5& x, int j, unsigned int* lds_hist)
 {
-    if (r.fixed) {
+    if (r.tag < 0) {
         PS q;
         q.P = Prim{x.v[0], x.v[1], x.v[2], x.v[3]};
         q.s = x.v[4];
         return q;
     }
-    const double rdv = r.rvolA * a.th.rdcost[j];
+    const double rdv = r.rvolA * gld(a.th.rdcost, j);
     const double guess = GUESS ? mlp_guess_pressure(x.v[0] * rdv, x.g) : 0.0;
-    return zone_primitive<SYS, STATS, GUESS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], rdv, r.order, r.row, j,
-                                             r.index, lds_hist, guess);
+    return zone_primitive<SYS, STATS, GUESS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], rdv, r.tag, j, lds_hist,
+                                             guess);
 }
 
 __device__ inline void lds_put(double* L, int W, int j, const PS& q)
@@ -325,7 +339,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     Cons G;
                     double Gs;
                     riemann<SYS, RS, 1>(add_half(q[1], gl), sub_half(q[2], gr), a.zp, &G, &Gs);
-                    const Flux5 f = scaled(G, Gs, c, rg_rb(B)[it.i0 + rr] * a.th.cq[j1]);
+                    const Flux5 f = scaled(G, Gs, c, gld(rg_rb(B), it.i0 + rr) * gld(a.th.cq, j1));
                     double* d = ldsR + rr * 5;
                     d[0] = f.d; d[1] = f.sr; d[2] = f.sq; d[3] = f.tau; d[4] = f.s;
                 }
@@ -337,6 +351,9 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
         Flux5 Fm;
         RowSrc src = resolve_row<PRIMS>(a, B, it.i0 - 2);
         Raw5 cur = load_raw<GUESS && !PRIMS>(src, j), nxt;
+        // The fused stage keeps the conserved rows k-1 and k-2 it loaded for
+        // the c2p: row k-2 is also the update operand U of this step.
+        Raw5 rm1, rm2;
 
         for (int k = it.i0 - 2; k < it.i1 + 2; ++k) {
             // Prefetch the next row and this step's update operands.
@@ -349,11 +366,15 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
             Raw5 ui, u0;
             if (update) {
                 const int64_t o = (int64_t)(k - 2) * nq + j;
+                if (PRIMS) {
 #pragma unroll
-                for (int f = 0; f < 5; ++f) ui.v[f] = Uin[o + f * np];
+                    for (int f = 0; f < 5; ++f) ui.v[f] = gld(Uin, o + f * np);
+                } else {
+                    ui = rm2;
+                }
                 if (avg) {
 #pragma unroll
-                    for (int f = 0; f < 5; ++f) u0.v[f] = U0[o + f * np];
+                    for (int f = 0; f < 5; ++f) u0.v[f] = gld(U0, o + f * np);
                 }
             }
             const PS Pk = raw_primitive<SYS, STATS, GUESS>(a, src, cur, j, lds_hist);
@@ -365,7 +386,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     Cons F;
                     double Fs;
                     riemann<SYS, RS, 0>(add_half(Pm2, gm2), sub_half(Pm1, g), a.zp, &F, &Fs);
-                    Fk = scaled(F, Fs, c, rr2[k - 1] * a.th.aq[j]);
+                    Fk = scaled(F, Fs, c, gld(rr2, k - 1) * gld(a.th.aq, j));
                 }
                 gm2 = g;
             }
@@ -395,7 +416,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     Cons G;
                     double Gs;
                     riemann<SYS, RS, 1>(add_half(Pl, gl), sub_half(P, gy), a.zp, &G, &Gs);
-                    Fy = scaled(G, Gs, c, rg_rb(B)[r] * a.th.cq[j]);
+                    Fy = scaled(G, Gs, c, gld(rg_rb(B), r) * gld(a.th.cq, j));
                 }
                 ldsF[tid] = Fy.d;
                 ldsF[NT + tid] = Fy.sr;
@@ -415,9 +436,9 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                 }
                 if (act) {
                     // Sources x dV and the update (scheme.rs:130-158).
-                    const double dv = rg_volA(B)[r] * a.th.dcost[j];
+                    const double dv = gld(rg_volA(B), r) * gld(a.th.dcost, j);
                     double ssr, ssq;
-                    sources<SYS>(P.P, rg_rinv(B)[r], a.th.cot[j], a.zp, &ssr, &ssq);
+                    sources<SYS>(P.P, gld(rg_rinv(B), r), gld(a.th.cot, j), a.zp, &ssr, &ssq);
                     const double sc0 = 0.0 * c * dv;
                     const double sc1 = ssr * c * dv;
                     const double sc2 = ssq * c * dv;
@@ -440,16 +461,20 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                         v4 = v4 * w1 + u0.v[4] * bw;
                     }
                     double* o = Uout + (int64_t)r * nq + j;
-                    o[0] = v0;
-                    o[np] = v1;
-                    o[2 * np] = v2;
-                    o[3 * np] = v3;
-                    o[4 * np] = v4;
+                    gst(o, 0, v0);
+                    gst(o, np, v1);
+                    gst(o, 2 * np, v2);
+                    gst(o, 3 * np, v3);
+                    gst(o, 4 * np, v4);
                 }
             }
             Fm = Fk;
             Pm2 = Pm1;
             Pm1 = Pk;
+            if (!PRIMS) {
+                rm2 = rm1;
+                rm1 = cur;
+            }
             cur = nxt;
             src = nsrc;
         }
@@ -487,19 +512,20 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
         const int64_t np = (int64_t)B.nr * nq;
         const double* U = slot_of(B, a.in_slot) + (int64_t)i * nq;
         double* P = B.prim + (int64_t)i * nq;
-        const double rvolA = rg_rvolA(B)[i];
+        const double rvolA = gld(rg_rvolA(B), i);
         for (int j = it.j0 + tid; j < it.j1; j += NT) {
-            const double rdv = rvolA * a.th.rdcost[j];
-            const double u0 = U[j], u1 = U[np + j], u2 = U[2 * np + j], u3 = U[3 * np + j], u4 = U[4 * np + j];
-            const double guess = GUESS ? mlp_guess_pressure(u0 * rdv, B.guess[(int64_t)i * nq + j]) : 0.0;
+            const double rdv = rvolA * gld(a.th.rdcost, j);
+            const double u0 = gld(U, j), u1 = gld(U, np + j), u2 = gld(U, 2 * np + j), u3 = gld(U, 3 * np + j);
+            const double u4 = gld(U, 4 * np + j);
+            const double guess = GUESS ? mlp_guess_pressure(u0 * rdv, gldf(B.guess, (int64_t)i * nq + j)) : 0.0;
             {
-                const PS q = zone_primitive<SYS, STATS, GUESS>(a, u0, u1, u2, u3, u4, rdv, B.order, i, j, B.index,
+                const PS q = zone_primitive<SYS, STATS, GUESS>(a, u0, u1, u2, u3, u4, rdv, (B.order << 16) | i, j,
                                                                lds_hist, guess);
-                P[j] = q.P.rho;
-                P[np + j] = q.P.ur;
-                P[2 * np + j] = q.P.uq;
-                P[3 * np + j] = q.P.p;
-                P[4 * np + j] = q.s;
+                gst(P, j, q.P.rho);
+                gst(P, np + j, q.P.ur);
+                gst(P, 2 * np + j, q.P.uq);
+                gst(P, 3 * np + j, q.P.p);
+                gst(P, 4 * np + j, q.s);
             }
         }
     }
@@ -529,16 +555,16 @@ __global__ void __launch_bounds__(NT) k_guess(StageArgs a)
         const int nq = B.nq;
         const int64_t np = (int64_t)B.nr * nq;
         const double* U = slot_of(B, a.in_slot) + (int64_t)i * nq;
-        const double rvolA = rg_rvolA(B)[i];
+        const double rvolA = gld(rg_rvolA(B), i);
         for (int c0 = it.j0; c0 < it.j1; c0 += NT) {
             const int j = c0 + (int)threadIdx.x;
             const bool act = j < it.j1;
             const int jj = act ? j : it.j1 - 1;
-            const double rdv = rvolA * a.th.rdcost[jj];
-            const double D = U[jj] * rdv, sr = U[np + jj] * rdv, sq = U[2 * np + jj] * rdv;
-            const double tau = U[3 * np + jj] * rdv;
+            const double rdv = rvolA * gld(a.th.rdcost, jj);
+            const double D = gld(U, jj) * rdv, sr = gld(U, np + jj) * rdv, sq = gld(U, 2 * np + jj) * rdv;
+            const double tau = gld(U, 3 * np + jj) * rdv;
             const float l2 = mlp_log2_guess_wave(a.mlp, D, sr * sr + sq * sq, tau, a.zp.gamma);
-            if (act) B.guess[(int64_t)i * nq + j] = l2;
+            if (act) gstf(B.guess, (int64_t)i * nq + j, l2);
         }
     }
 }
@@ -568,15 +594,15 @@ __global__ void __launch_bounds__(256) k_dt(StageArgs a, double* partial)
         const BlockDesc B = a.blocks[it.block];
         const int nq = B.nq;
         const int64_t np = (int64_t)B.nr * nq;
-        const double rvolA = rg_rvolA(B)[i];
-        const double dr = rg_dr(B)[i];
-        const double r0 = rg_rv(B)[i];
+        const double rvolA = gld(rg_rvolA(B), i);
+        const double dr = gld(rg_dr(B), i);
+        const double r0 = gld(rg_rv(B), i);
         for (int j = it.j0 + (int)threadIdx.x; j < it.j1; j += blockDim.x) {
             const double* s = slot_of(B, a.in_slot) + (int64_t)i * nq + j;
-            const double rdv = rvolA * a.th.rdcost[j];
-            const PS q = zone_primitive<SYS, false, false>(a, s[0], s[np], s[2 * np], s[3 * np], s[4 * np], rdv, B.order,
-                                                           i, j, B.index, nullptr, 0.0);
-            const double dl = dmin(dr, a.th.dq[j] * r0);
+            const double rdv = rvolA * gld(a.th.rdcost, j);
+            const PS q = zone_primitive<SYS, false, false>(a, gld(s, 0), gld(s, np), gld(s, 2 * np), gld(s, 3 * np),
+                                                           gld(s, 4 * np), rdv, (B.order << 16) | i, j, nullptr, 0.0);
+            const double dl = dmin(dr, gld(a.th.dq, j) * r0);
             m = dmin(m, dl / (max_signal_speed<SYS>(q.P, a.zp) * a.zp.c));
         }
     }

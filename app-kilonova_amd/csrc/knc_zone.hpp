@@ -108,9 +108,9 @@ __device__ inline int c2p_srhd_finish(const Cons& u, double S2, double p, Prim* 
     const double Q = (u.tau + u.d) + p;
     const double den = Q * Q - S2;
     const double sq = sqrt(den);
-    const double rQ = 1.0 / Q;
-    const double rsq = 1.0 / sq;
-    *rW = sq * rQ;
+    const double r = 1.0 / (Q * sq);
+    const double rsq = Q * r;
+    *rW = den * r;
     out->rho = u.d * *rW;
     out->ur = u.sr * rsq;
     out->uq = u.sq * rsq;
@@ -161,44 +161,52 @@ struct FluxState {
     double us, fs, vn, lm, lp, p;
 };
 
-template <int SYS, int AXIS>
+template <int SYS, int RS, int AXIS>
 __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams& zp)
 {
     FluxState st;
     st.u = p2c<SYS>(P, zp);
     const double un = AXIS == 0 ? P.ur : P.uq;
+    st.p = P.p;
     if (SYS == SYS_EULER) {
         st.vn = un;
         st.us = s;
         const double cs = sqrt(zp.gamma * P.p / P.rho);
         st.lm = st.vn - cs;
         st.lp = st.vn + cs;
-    } else {
-        const double uu = P.ur * P.ur + P.uq * P.uq;
-        const double W = sqrt(1.0 + uu);
-        const double rhoh = P.rho + P.p * zp.gfac;
-        const double rW = 1.0 / W;
-        const double rA = 1.0 / rhoh;
-        st.vn = un * rW;
-        st.us = s * W;
-        // Outer characteristic speeds, Marti & Mueller (2003) eq. (72), with
-        // 1 - v^2 = 1/W^2.
-        const double a2 = zp.gamma * P.p * rA;
-        const double vv = uu * (rW * rW);
-        const double v2n = st.vn * st.vn;
-        const double k0 = sqrt(a2 * (rW * rW) * (1.0 - vv * a2 - v2n * (1.0 - a2)));
-        const double rd = 1.0 / (1.0 - vv * a2);
-        const double b = st.vn * (1.0 - a2);
-        st.lm = (b - k0) * rd;
-        st.lp = (b + k0) * rd;
+        st.f.d = st.u.d * st.vn;
+        st.f.sr = st.u.sr * st.vn;
+        st.f.sq = st.u.sq * st.vn;
+        if (AXIS == 0) st.f.sr = st.f.sr + P.p; else st.f.sq = st.f.sq + P.p;
+        st.f.tau = (st.u.tau + P.p) * st.vn;
+        st.fs = st.us * st.vn;
+        return st;
     }
-    st.p = P.p;
-    st.f.d = st.u.d * st.vn;
-    st.f.sr = st.u.sr * st.vn;
-    st.f.sq = st.u.sq * st.vn;
+    const double uu = P.ur * P.ur + P.uq * P.uq;
+    const double W = sqrt(1.0 + uu);
+    const double rhoh = P.rho + P.p * zp.gfac;
+    const double ut = AXIS == 0 ? P.uq : P.ur;
+    // Outer characteristic speeds of Marti & Mueller (2003) eq. (72) with
+    // numerator and denominator multiplied by W^2 rho h (gp = gamma p,
+    // e = rho h - gp): lambda = (u_n W e -/+ sqrt(gp (rho h + u_t^2 e))) /
+    // (rho h + u^2 e), one division for both speeds.
+    const double gp = zp.gamma * P.p;
+    const double e = rhoh - gp;
+    const double rY = 1.0 / (rhoh + uu * e);
+    const double k0 = sqrt(gp * (rhoh + (ut * ut) * e));
+    const double b = (un * W) * e;
+    st.lm = (b - k0) * rY;
+    st.lp = (b + k0) * rY;
+    st.vn = RS == RS_HLLC ? un / W : 0.0;
+    st.us = s * W;
+    // Fluxes with v_n = u_n/W cancelled against the W of U:
+    // F_D = rho u_n, F_S = rho h u u_n + p n, F_tau = (rho h W - rho) u_n, F_s = s u_n.
+    st.f.d = P.rho * un;
+    st.f.sr = (rhoh * P.ur) * un;
+    st.f.sq = (rhoh * P.uq) * un;
     if (AXIS == 0) st.f.sr = st.f.sr + P.p; else st.f.sq = st.f.sq + P.p;
-    st.f.tau = (st.u.tau + P.p) * st.vn;
-    st.fs = st.us * st.vn;
+    st.f.tau = (rhoh * W - P.rho) * un;
+    st.fs = s * un;
     return st;
 }
 
@@ -212,8 +220,8 @@ template <int AXIS> __device__ inline double ctang(const Cons& u) { return AXIS 
 template <int SYS, int RS, int AXIS>
 __device__ inline void riemann(const PS& Lq, const PS& Rq, const ZoneParams& zp, Cons* F, double* Fs)
 {
-    const FluxState L = flux_state<SYS, AXIS>(Lq.P, Lq.s, zp);
-    const FluxState R = flux_state<SYS, AXIS>(Rq.P, Rq.s, zp);
+    const FluxState L = flux_state<SYS, RS, AXIS>(Lq.P, Lq.s, zp);
+    const FluxState R = flux_state<SYS, RS, AXIS>(Rq.P, Rq.s, zp);
     const double am = dmin(L.lm, R.lm);
     const double ap = dmax(L.lp, R.lp);
 
@@ -314,15 +322,17 @@ __device__ inline double max_signal_speed(const Prim& P, const ZoneParams& zp)
     return (v + cs) / (1.0 + v * cs);
 }
 
-// Generalised minmod (godunov-core piecewise_linear::plm_gradient).
+// Generalised minmod (godunov-core piecewise_linear::plm_gradient): all
+// three candidate slopes positive <=> their minimum is positive, all
+// negative <=> their maximum is negative, otherwise zero (v_min/v_max_f64).
 __device__ inline double plm(double th, double a, double b, double c)
 {
     const double x = (b - a) * th;
     const double y = (c - a) * 0.5;
     const double z = (c - b) * th;
-    if (x > 0.0 && y > 0.0 && z > 0.0) return dmin(x, dmin(y, z));
-    if (x < 0.0 && y < 0.0 && z < 0.0) return dmax(x, dmax(y, z));
-    return 0.0;
+    const double lo = fmin(x, fmin(y, z));
+    const double hi = fmax(x, fmax(y, z));
+    return lo > 0.0 ? lo : (hi < 0.0 ? hi : 0.0);
 }
 
 __device__ inline PS plm5(double th, const PS& a, const PS& b, const PS& c)

@@ -42,7 +42,10 @@ enum { KNC_DIRECTION_RADIAL = 0, KNC_DIRECTION_POLAR = 1 };
 enum { KNC_BOUNDARY_FIXED = 0, KNC_BOUNDARY_OUTFLOW = 1 };
 enum { KNC_SIDE_INNER = 0, KNC_SIDE_OUTER = 1 };
 enum { KNC_LAUNCH_PER_BLOCK = 0, KNC_LAUNCH_BATCHED = 1 };
-enum { KNC_GUESS_PLAIN = 0, KNC_GUESS_LEARNED = 1 };
+/* c2p initial pressure: PLAIN p = 0; LEARNED the network's guess for every
+ * stage's input; LEARNED_STEP the network evaluated on the first stage's
+ * input of each RK step, its guesses reused by the step's later stages. */
+enum { KNC_GUESS_PLAIN = 0, KNC_GUESS_LEARNED = 1, KNC_GUESS_LEARNED_STEP = 2 };
 
 /* return codes: hydro errors mirror HydroErrorType (src/physics/mod.rs:82-95) */
 enum {
@@ -78,7 +81,7 @@ typedef struct knc_options {
     int32_t launch_mode;     /* KNC_LAUNCH_PER_BLOCK: one stage launch per block;
                                 KNC_LAUNCH_BATCHED: one persistent launch per stage
                                 over every block, fold steps replayed as a hipGraph */
-    int32_t c2p_guess;       /* KNC_GUESS_PLAIN | KNC_GUESS_LEARNED */
+    int32_t c2p_guess;       /* KNC_GUESS_PLAIN | KNC_GUESS_LEARNED | KNC_GUESS_LEARNED_STEP */
     int32_t collect_stats;   /* 1: Newton iteration histogram and fallback counts */
     int32_t rows_per_item;   /* radial rows per work item (<= 64); 0 = chosen from the mesh */
     int32_t device;          /* HIP device ordinal */

@@ -127,7 +127,8 @@ int c2p_srhd_newton(double D, double S2, double tau, double gamma, double p,
 }
 
 // Primitive from the converged pressure: rho = D sqrt(den)/Q, u = S/sqrt(den)
-// (u = Gamma v with Gamma = Q/sqrt(den)); *rW receives 1/Gamma = sqrt(den)/Q.
+// (u = Gamma v with Gamma = Q/sqrt(den)); *rW receives 1/Gamma = sqrt(den)/Q,
+// formed as den/(Q sqrt(den)).
 int c2p_srhd(const Cons& u, double gamma, double guess, Prim* out, int* iters, double* rW)
 {
     *iters = 0;
@@ -140,9 +141,10 @@ int c2p_srhd(const Cons& u, double gamma, double guess, Prim* out, int* iters, d
     const double Q = (u.tau + u.d) + p;
     const double den = Q * Q - S2;
     const double sq = std::sqrt(den);
-    const double rQ = 1.0 / Q;
-    const double rsq = 1.0 / sq;
-    *rW = sq * rQ;
+    // one division: r = 1/(Q sqrt(den)), 1/Gamma = sqrt(den)/Q = den r, 1/sqrt(den) = Q r
+    const double r = 1.0 / (Q * sq);
+    const double rsq = Q * r;
+    *rW = den * r;
     out->rho = u.d * *rW;
     out->ur = u.sr * rsq;
     out->uq = u.sq * rsq;
@@ -202,39 +204,52 @@ FluxState flux_state(const Hydro& h, const Prim& P, double s, int axis)
     FluxState st;
     st.u = p2c(h, P);
     const double un = axis == 0 ? P.ur : P.uq;
+    st.p = P.p;
+    st.ut = 0.0;
     if (h.system == SYS_EULER) {
         st.vn = un;
         st.us = s;
         const double cs = std::sqrt(g * P.p / P.rho);
         st.lm = st.vn - cs;
         st.lp = st.vn + cs;
-    } else {
-        const double uu = P.ur * P.ur + P.uq * P.uq;
-        const double W = std::sqrt(1.0 + uu);
-        const double rhoh = P.rho + P.p * (g / (g - 1.0));
-        const double rW = 1.0 / W;
-        const double rA = 1.0 / rhoh;
-        st.vn = un * rW;
-        st.us = s * W;
-        // Outer characteristic speeds, Marti & Mueller (2003) eq. (72), with
-        // 1 - v^2 = 1/W^2.
-        const double a2 = g * P.p * rA;
-        const double vv = uu * (rW * rW);
-        const double v2n = st.vn * st.vn;
-        const double k0 = std::sqrt(a2 * (rW * rW) * (1.0 - vv * a2 - v2n * (1.0 - a2)));
-        const double rd = 1.0 / (1.0 - vv * a2);
-        const double b = st.vn * (1.0 - a2);
-        st.lm = (b - k0) * rd;
-        st.lp = (b + k0) * rd;
+        st.f.d = st.u.d * st.vn;
+        st.f.sr = st.u.sr * st.vn;
+        st.f.sq = st.u.sq * st.vn;
+        if (axis == 0) st.f.sr = st.f.sr + P.p; else st.f.sq = st.f.sq + P.p;
+        st.f.tau = (st.u.tau + P.p) * st.vn;
+        st.fs = st.us * st.vn;
+        return st;
     }
-    st.p = P.p;
-    st.f.d = st.u.d * st.vn;
-    st.f.sr = st.u.sr * st.vn;
-    st.f.sq = st.u.sq * st.vn;
+    const double uu = P.ur * P.ur + P.uq * P.uq;
+    const double W = std::sqrt(1.0 + uu);
+    const double rhoh = P.rho + P.p * (g / (g - 1.0));
+    const double ut = axis == 0 ? P.uq : P.ur;
+    // Outer characteristic speeds of Marti & Mueller (2003) eq. (72),
+    // numerator and denominator multiplied by W^2 rho h.  With gp = gamma p
+    // and e = rho h - gp:
+    //   denominator  W^2 rho h (1 - v^2 c_s^2) = rho h + u^2 e,
+    //   root         W^2 rho h c_s sqrt((1-v^2)(1 - v^2 c_s^2 - v_n^2 (1 - c_s^2)))
+    //                  = sqrt(gp (rho h + u_t^2 e)),
+    //   first term   W^2 rho h v_n (1 - c_s^2) = u_n W e,
+    // so one division serves both speeds.
+    const double gp = g * P.p;
+    const double e = rhoh - gp;
+    const double rY = 1.0 / (rhoh + uu * e);
+    const double k0 = std::sqrt(gp * (rhoh + (ut * ut) * e));
+    const double b = (un * W) * e;
+    st.lm = (b - k0) * rY;
+    st.lp = (b + k0) * rY;
+    st.vn = un / W;   // HLLC only
+    st.us = s * W;
+    // Physical fluxes with v_n = u_n/W cancelled against the W of U:
+    //   F_D = rho u_n,  F_S = rho h u u_n + p n,  F_tau = (tau + p) v_n = (rho h W - rho) u_n,
+    //   F_s = s W v_n = s u_n.
+    st.f.d = P.rho * un;
+    st.f.sr = (rhoh * P.ur) * un;
+    st.f.sq = (rhoh * P.uq) * un;
     if (axis == 0) st.f.sr = st.f.sr + P.p; else st.f.sq = st.f.sq + P.p;
-    st.f.tau = (st.u.tau + P.p) * st.vn;
-    st.fs = st.us * st.vn;
-    st.ut = 0.0;
+    st.f.tau = (rhoh * W - P.rho) * un;
+    st.fs = s * un;
     return st;
 }
 
@@ -358,14 +373,16 @@ double max_signal_speed(const Hydro& h, const Prim& P)
 
 // Generalised-minmod PLM difference (godunov-core plm_gradient):
 // minmod((b-a) theta, (c-a)/2, (c-b) theta).
+// All three candidate slopes positive <=> their minimum is positive; all
+// negative <=> their maximum is negative; otherwise the slope is zero.
 inline double plm(double theta, double a, double b, double c)
 {
     const double x = (b - a) * theta;
     const double y = (c - a) * 0.5;
     const double z = (c - b) * theta;
-    if (x > 0.0 && y > 0.0 && z > 0.0) return std::min(x, std::min(y, z));
-    if (x < 0.0 && y < 0.0 && z < 0.0) return std::max(x, std::max(y, z));
-    return 0.0;
+    const double lo = std::fmin(x, std::fmin(y, z));
+    const double hi = std::fmax(x, std::fmax(y, z));
+    return lo > 0.0 ? lo : (hi < 0.0 ? hi : 0.0);
 }
 
 // Primitive + scalar concentration, the element of the extended arrays

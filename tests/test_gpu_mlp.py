@@ -79,11 +79,12 @@ def test_trained_network_reduces_newton_iterations(trained):
     assert learned_it.mean() < plain_it.mean() - 0.3
 
 
-def test_learned_batched_path_matches_plain(trained):
+@pytest.mark.parametrize("mode", [knc.GUESS_LEARNED, knc.GUESS_LEARNED_STEP])
+def test_learned_batched_path_matches_plain(trained, mode):
     net, _ = trained
     s = knc.setups.jet_in_cloud(nr=256, nq=64, nblocks=4, corr=4.0)
     plain = knc.context_from_setup(s, launch_mode=knc.PER_BLOCK)
-    learned = knc.context_from_setup(s, launch_mode=knc.BATCHED, c2p_guess=knc.GUESS_LEARNED, collect_stats=True)
+    learned = knc.context_from_setup(s, launch_mode=knc.BATCHED, c2p_guess=mode, collect_stats=True)
     net.attach(learned)
     for _ in range(20):
         assert plain.advance(10) == learned.advance(10)
