@@ -126,9 +126,11 @@ def newton_stats(setup, fold, net, steps=20):
     return out
 
 
-def read_traffic(learned):
-    """HBM bytes per stage launch of the plain- or learned-guess stage kernel
-    from the committed rocprofv3 PMC summary, or None."""
+def read_traffic(head):
+    """HBM bytes per stage of the plain-guess path (head None) or of a
+    learned-guess path ("learned", "learned_step": the stage kernel plus its
+    share of the guess pass) from the committed rocprofv3 PMC summary, or
+    None."""
     path = os.path.join(ROOT, "profiles", "k_stage_bomb_jet_pmc.txt")
     if not os.path.exists(path):
         return None
@@ -138,9 +140,31 @@ def read_traffic(learned):
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_stage_launch" + ("_learned" if learned else "")])
+        return float(vals["hbm_bytes_per_stage_launch" + (f"_{head}" if head else "")])
     except (KeyError, ValueError):
         return None
+
+
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4   # CUs x SIMDs x clock / cycles per wave64 FP64 instruction
+
+
+def valu_roofline(zones, stage_ms):
+    """VALU issue rate of the plain-guess stage kernel: instructions per zone
+    from the committed SQ_INSTS_VALU summary x zones / measured stage time,
+    against the MI355X issue peak, or None."""
+    path = os.path.join(ROOT, "profiles", "k_stage_bomb_jet_sq.txt")
+    if not os.path.exists(path):
+        return None
+    per_zone = None
+    for line in open(path):
+        if line.startswith("valu_insts_per_zone"):
+            per_zone = float(line.split("=", 1)[1])
+    if per_zone is None:
+        return None
+    achieved = per_zone * zones / 64 / (stage_ms * 1e-3)
+    return {"kernel": "k_stage, plain guess", "bound": "fp64-valu", "valu_insts_per_zone": per_zone,
+            "achieved": achieved / 1e9, "peak": VALU_PEAK_WAVE_INSTS / 1e9, "unit": "G wave-instructions/s",
+            "frac": achieved / VALU_PEAK_WAVE_INSTS}
 
 
 # FP64 operations of one Newton iteration of the c2p as restated (add, sub,
@@ -160,6 +184,18 @@ def c2p_roofline(zones, stage_ms, mean_iterations):
             "ms_at_fp64_peak": 1e3 * t, "frac_of_stage": 1e3 * t / stage_ms, "peak_tflops": FP64_VECTOR_PEAK / 1e12}
 
 
+def copy_probe(mib=512, repeats=5):
+    """Device-to-device copies of a known size, so that FETCH_SIZE and
+    WRITE_SIZE per dispatch can be scaled to bytes."""
+    import torch
+    src = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"bench: copy probe, {repeats} x {mib} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,7 +212,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-stats", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run five 512 MiB device-to-device copies (calibration of the HBM traffic counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; multi-GPU sharding of the "
               "radial block chain is described in DESIGN.md but not built", file=sys.stderr)
@@ -261,7 +302,8 @@ def main():
                             if head else "batched persistent stage launches, hipGraph per fold, plain c2p guess"),
                    "zones": zones},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(head is not None)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(head)},
+        "valu_roofline": valu_roofline(zones, stage_ms_b),
         "c2p_fp64_roofline": (c2p_roofline(zones, head_stage, newton[head or "plain"]["mean_iterations"])
                               if newton else None),
         "autotuned": {"batched": ctx_b.tuned, **{k: c.tuned for k, c in ctx_l.items()}},
