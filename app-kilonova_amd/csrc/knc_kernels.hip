@@ -718,7 +718,7 @@ __global__ void k_plm(double th, int64_t n, const double* a, const double* b, co
         out[k] = plm(th, a[k], b[k], c[k]);
 }
 
-// Exact FP64 arithmetic probe: a/b, sqrt(a), a*b+c evaluated as two roundings.
+// Exact FP64 arithmetic probe: a/b, sqrt(|a|), a*b+c evaluated as two roundings.
 __global__ void k_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div,
                              double* sq, double* mad)
 {

@@ -76,28 +76,37 @@ __device__ inline double c2p_srhd_polish(double D, double S2, double E, const Zo
     return p - F * den / (zp.gamma * p * S2 - den * Q);
 }
 
+// The loop only iterates; lanes leave it as they converge or fail, and the
+// polish step runs once per wavefront after the loop rather than once per
+// distinct convergence iteration.
 __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const ZoneParams& zp,
                                       double p, double* p_out, int* iters)
 {
     const double E = tau + D;
-    for (int n = 1; n <= C2P_MAX_ITER; ++n) {
+    int code = E_ROOT_FINDER_FAILED;
+    int n = 1;
+    for (; n <= C2P_MAX_ITER; ++n) {
         const double Q = E + p;
         const double den = Q * Q - S2;
-        if (!(den > 0.0)) { *iters = n; return E_ROOT_FINDER_FAILED; }
+        if (!(den > 0.0)) break;
         const double sq = sqrt(den);
         const double pQ = p * Q;
         const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
         const double dp = F * den / (zp.gamma * p * S2 - den * Q);
         p = p - dp;
-        if (!(fabs(p) < 1e300)) { *iters = n; return E_ROOT_FINDER_FAILED; }
+        if (!(fabs(p) < 1e300)) break;
         if (fabs(dp) <= C2P_TOL * (E + p)) {
-            *iters = n + 1;
-            *p_out = c2p_srhd_polish(D, S2, E, zp, p);
-            return E_OK;
+            code = E_OK;
+            break;
         }
     }
-    *iters = C2P_MAX_ITER;
-    return E_ROOT_FINDER_FAILED;
+    if (code != E_OK) {
+        *iters = n > C2P_MAX_ITER ? C2P_MAX_ITER : n;
+        return code;
+    }
+    *iters = n + 1;
+    *p_out = c2p_srhd_polish(D, S2, E, zp, p);
+    return E_OK;
 }
 
 // Primitive from a converged pressure: rho = D sqrt(den)/Q, u = S/sqrt(den);

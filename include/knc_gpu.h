@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define KNC_ABI_VERSION 1
+#define KNC_ABI_VERSION 2
 
 /* hydrodynamics system and options */
 enum { KNC_EULER = 0, KNC_SRHD = 1 };

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_function():
 
 
 def test_abi_version():
-    assert knc.lib().knc_abi_version() == 1
+    assert knc.lib().knc_abi_version() == 2
 
 
 def test_invalid_hydro_is_rejected_before_any_device_call():
@@ -49,3 +49,21 @@ def test_invalid_hydro_is_rejected_before_any_device_call():
 def test_context_errors_are_reported_not_raised_in_c():
     with pytest.raises(RuntimeError):
         knc.Context(knc.relativistic_hydro(runge_kutta_order=4))
+
+
+def test_invalid_options_are_rejected_before_any_device_call():
+    h = knc.relativistic_hydro()
+    out = ctypes.c_void_p()
+    for bad in (capi.Options(1, 3, 0, 0, 0, 1, 0, 0),       # unknown c2p guess mode
+                capi.Options(1, 0, 0, 0, 0, 1, 0, 3)):      # stage_passes must be 0, 1 or 2
+        rc = knc.lib().knc_context_create(ctypes.byref(h), ctypes.byref(bad), ctypes.byref(out))
+        assert rc == 10 and out.value is None
+
+
+def test_null_context_arguments_are_errors():
+    L = knc.lib()
+    assert L.knc_set_excision(None, None) == 10
+    assert L.knc_excision_counts(None, None, None) == 10
+    assert L.knc_autotune(None, 1, 0, None) == 10
+    assert L.knc_advance(None, 1, None, None) == 10
+    assert L.knc_block_remove(None, 0) == 10

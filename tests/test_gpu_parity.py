@@ -17,15 +17,24 @@ MANY_STEPS = 1e-9
 
 
 def test_fp64_arithmetic_rounds_like_host():
+    """Device division, square root and unfused multiply-add round like the
+    host's, over the whole exponent range and at the edges of the device
+    square root's range scaling (2^-767, subnormals, 0, inf)."""
     rng = np.random.default_rng(0)
     n = 1 << 18
     a = rng.standard_normal(n) * np.exp(rng.uniform(-300, 300, n) / 10)
+    a[: n // 4] = np.ldexp(rng.uniform(1.0, 2.0, n // 4), rng.integers(-1074, 1024, n // 4))
+    edges = np.array([0.0, -0.0, 5e-324, 2.0 ** -1022, 2.0 ** -767, np.nextafter(2.0 ** -767, 0),
+                      np.nextafter(2.0 ** -767, 1), 1.0, np.nextafter(1.0, 2), 1.7976931348623157e308, np.inf])
+    a[n // 4: n // 4 + edges.size] = edges
     b = rng.standard_normal(n) * np.exp(rng.uniform(-300, 300, n) / 10)
     c = rng.standard_normal(n)
-    div, sq, mad = knc.fp64_probe(a, b, c)
-    assert np.array_equal(div, a / b)
-    assert np.array_equal(sq, np.sqrt(np.abs(a)))
-    assert np.array_equal(mad, a * b + c)
+    with np.errstate(all="ignore"):
+        div, sq, mad = knc.fp64_probe(a, b, c)
+        assert np.array_equal(div, a / b, equal_nan=True)
+        assert np.array_equal(sq, np.sqrt(np.abs(a)), equal_nan=True)
+        m = np.isfinite(a * b + c)
+        assert np.array_equal(mad[m], (a * b + c)[m])
 
 
 def _random_prims(system, n, rng):
