@@ -546,6 +546,8 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
 template <int NT>
 __global__ void __launch_bounds__(NT) k_guess(StageArgs a)
 {
+    __shared__ float2 tanh_tab[TANH_N];
+    tanh_table_init(tanh_tab);
     const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const WorkItem it = a.items[u / a.rows];
@@ -563,7 +565,7 @@ __global__ void __launch_bounds__(NT) k_guess(StageArgs a)
             const double rdv = rvolA * gld(a.th.rdcost, jj);
             const double D = gld(U, jj) * rdv, sr = gld(U, np + jj) * rdv, sq = gld(U, 2 * np + jj) * rdv;
             const double tau = gld(U, 3 * np + jj) * rdv;
-            const float l2 = mlp_log2_guess_wave(a.mlp, D, sr * sr + sq * sq, tau, a.zp.gamma);
+            const float l2 = mlp_log2_guess_wave(a.mlp, tanh_tab, D, sr * sr + sq * sq, tau, a.zp.gamma);
             if (act) gstf(B.guess, (int64_t)i * nq + j, l2);
         }
     }

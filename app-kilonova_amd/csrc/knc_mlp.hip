@@ -178,6 +178,8 @@ __global__ void k_mlp_features(int64_t n, const double* cons, const double* gamm
 __global__ void __launch_bounds__(256) k_mlp_predict(MlpDev M, int64_t n, const double* cons, const double* gamma,
                                                      float* y, double* guess)
 {
+    __shared__ float2 tanh_tab[TANH_N];
+    tanh_table_init(tanh_tab);
     const int64_t base = (int64_t)blockIdx.x * blockDim.x;
     for (int64_t b0 = base; b0 < n; b0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t k = b0 + threadIdx.x;
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(256) k_mlp_predict(MlpDev M, int64_t n, const 
         float f[4], x[4];
         mlp_raw_features(D, S2, cons[4 * kk + 3], gamma[kk], f);
         for (int m = 0; m < 4; ++m) x[m] = (f[m] - M.norm.shift[m]) * M.norm.scale[m];
-        const float out = mlp_wave(M, x);
+        const float out = mlp_wave(M, tanh_tab, x);
         if (valid) {
             if (y) y[k] = out;
             if (guess) guess[k] = D * (double)__builtin_amdgcn_exp2f(out / M.norm.tscale + M.norm.tshift);

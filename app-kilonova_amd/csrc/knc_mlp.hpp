@@ -66,6 +66,34 @@ __device__ inline float fast_tanh(float x)
     return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
+// Inference tanh from a table in LDS: TANH_N intervals over [-8, 8], each
+// entry (tanh(x_i), tanh(x_{i+1}) - tanh(x_i)), linear interpolation.  The
+// interpolation error is at most h^2/8 max|tanh''| = 2.4e-5 (h = 1/64), far
+// below the BF16 resolution of the next layer's operands, and it costs no
+// transcendental instruction (exp2 and rcp issue at a quarter of the rate).
+constexpr int TANH_N = 1024;
+constexpr float TANH_X0 = -8.0f;
+constexpr float TANH_RH = TANH_N / 16.0f;   // intervals per unit of x
+
+// Fills the table; every thread of the workgroup must call it, and it ends
+// with a barrier.
+__device__ inline void tanh_table_init(float2* tab)
+{
+    for (int i = threadIdx.x; i < TANH_N; i += blockDim.x) {
+        const float a = tanhf(TANH_X0 + i / TANH_RH);
+        const float b = tanhf(TANH_X0 + (i + 1) / TANH_RH);
+        tab[i] = make_float2(a, b - a);
+    }
+    __syncthreads();
+}
+
+__device__ inline float table_tanh(const float2* tab, float x)
+{
+    const float t = fminf(fmaxf(fmaf(x, TANH_RH, -TANH_X0 * TANH_RH), 0.0f), TANH_N - 0.5f);
+    const float2 e = tab[(int)t];
+    return fmaf(__builtin_amdgcn_fractf(t), e.y, e.x);
+}
+
 __device__ inline float safe_log2(float x)
 {
     return __builtin_amdgcn_logf(x > 1e-30f ? x : 1e-30f);
@@ -89,8 +117,9 @@ __host__ __device__ inline int mfma_row(int g, int v) { return 8 * (v / 4) + 4 *
 __host__ __device__ inline int mlp_perm(int c, int g, int i) { return 16 * c + 8 * (i / 4) + 4 * g + (i % 4); }
 
 // Wave-level inference: every lane passes its zone's normalised features
-// and receives y = MLP(x) for its own zone.  Must be called by all 64 lanes.
-__device__ inline float mlp_wave(const MlpDev& M, const float x[4])
+// and receives y = MLP(x) for its own zone.  Must be called by all 64 lanes;
+// tab is the workgroup's tanh table (tanh_table_init).
+__device__ inline float mlp_wave(const MlpDev& M, const float2* tab, const float x[4])
 {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 5;
@@ -120,7 +149,7 @@ __device__ inline float mlp_wave(const MlpDev& M, const float x[4])
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) b2[c][nt][i] = (__bf16)fast_tanh(h[c >> 1][nt][8 * (c & 1) + i]);
+            for (int i = 0; i < 8; ++i) b2[c][nt][i] = (__bf16)table_tanh(tab, h[c >> 1][nt][8 * (c & 1) + i]);
     floatx16 o[2][2];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
@@ -141,8 +170,8 @@ __device__ inline float mlp_wave(const MlpDev& M, const float x[4])
         const floatx16 w = M.w3[mt * 2 + g];
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            part[0] += w[v] * fast_tanh(o[mt][0][v]);
-            part[1] += w[v] * fast_tanh(o[mt][1][v]);
+            part[0] = fmaf(w[v], table_tanh(tab, o[mt][0][v]), part[0]);
+            part[1] = fmaf(w[v], table_tanh(tab, o[mt][1][v]), part[1]);
         }
     }
     part[0] += __shfl_xor(part[0], 32, 64);
@@ -151,13 +180,14 @@ __device__ inline float mlp_wave(const MlpDev& M, const float x[4])
 }
 
 // log2(p/D) guessed for one zone by the wave inference: y/tscale + tshift.
-__device__ inline float mlp_log2_guess_wave(const MlpDev& M, double D, double S2, double tau, double gamma)
+__device__ inline float mlp_log2_guess_wave(const MlpDev& M, const float2* tab, double D, double S2, double tau,
+                                           double gamma)
 {
     float f[4], x[4];
     mlp_raw_features(D, S2, tau, gamma, f);
 #pragma unroll
     for (int m = 0; m < 4; ++m) x[m] = (f[m] - M.norm.shift[m]) * M.norm.scale[m];
-    const float y = mlp_wave(M, x);
+    const float y = mlp_wave(M, tab, x);
     return y / M.norm.tscale + M.norm.tshift;
 }
 
