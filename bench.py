@@ -9,7 +9,7 @@ reduction run once per fold of 10 steps.  A step is one full RK2 step
 
 Paths timed on the same initial data:
   plain    one stage launch per block per stage (KNC_LAUNCH_PER_BLOCK),
-           plain c2p guess
+           plain c2p guess, shape autotuned like the batched paths
   batched  one persistent launch per stage over every block, the fold's
            steps replayed as a hipGraph (KNC_LAUNCH_BATCHED), plain guess
   learned  batched + the learned c2p pressure guess (BF16-MFMA network,
@@ -256,7 +256,10 @@ def main():
     plain = None
     parity = {}
     if not args.no_plain:
-        ctx_p, wall_p, stage_ms_p = run_gpu(setup, knc.PER_BLOCK, args.steps, args.warmup, args.fold)
+        # the per-block path is tuned too, so that each block's launch fills
+        # the device as well as its size allows
+        ctx_p, wall_p, stage_ms_p = run_gpu(setup, knc.PER_BLOCK, args.steps, args.warmup, args.fold, tune=tune,
+                                            **shape)
         rate_p = zones * args.steps / wall_p
         log(f"bench: plain {rate_p / 1e9:.3f} Gzu/s, stage {stage_ms_p:.4f} ms")
         from util import gpu_state, rel_l1  # noqa: E402
@@ -306,7 +309,8 @@ def main():
         "valu_roofline": valu_roofline(zones, stage_ms_b),
         "c2p_fp64_roofline": (c2p_roofline(zones, head_stage, newton[head or "plain"]["mean_iterations"])
                               if newton else None),
-        "autotuned": {"batched": ctx_b.tuned, **{k: c.tuned for k, c in ctx_l.items()}},
+        "autotuned": {"batched": ctx_b.tuned, **{k: c.tuned for k, c in ctx_l.items()},
+                      "plain_per_block": ctx_p.tuned if plain else None},
         "cpu_baseline": cpu,
         "paths": {"plain_per_block": plain,
                   "batched": {"value": rate_b, "ms_per_step": 1e3 * wall_b / args.steps, "stage_ms": stage_ms_b},
