@@ -185,3 +185,35 @@ def test_polar_tiles_match_untiled_and_oracle(nq, threads, passes):
     for (x, m), (y, n) in zip(st, sw):
         assert np.array_equal(x, y) and np.array_equal(m, n)
     assert max_rel_l1(st, oracle_state(sim, s.blocks)) <= ONE_STEP
+
+
+@pytest.mark.parametrize("maker", ["euler", "srhd"])
+def test_one_dimensional_mesh_matches_oracle(maker):
+    """nq = 1: the 1-D update of src/scheme.rs:99-105 (no polar fluxes)."""
+    if maker == "euler":
+        s = knc.setups.euler_blast(nr=256, nq=1)
+    else:
+        s = knc.setups.jet_in_cloud(nr=256, nq=1, nblocks=4, corr=2.0)
+    ctx, sim = _run_pair(s, 1, 1, launch_mode=knc.BATCHED)
+    assert max_rel_l1(gpu_state(ctx), oracle_state(sim, s.blocks)) <= ONE_STEP
+    ctx, sim = _run_pair(s, 200, 10, launch_mode=knc.BATCHED)
+    assert max_rel_l1(gpu_state(ctx), oracle_state(sim, s.blocks)) <= MANY_STEPS
+
+
+@pytest.mark.parametrize("order", [1, 3])
+def test_runge_kutta_orders_match_oracle(order):
+    """RK1 and RK3 (SSP weights 3/4 and 1/3 fused into the last stage)."""
+    s = knc.setups.jet_in_cloud(nr=256, nq=64, nblocks=4, riemann=knc.HLLC, corr=4.0)
+    s.hydro.runge_kutta_order = order
+    ctx, sim = _run_pair(s, 100, 10, launch_mode=knc.BATCHED)
+    assert max_rel_l1(gpu_state(ctx), oracle_state(sim, s.blocks)) <= MANY_STEPS
+
+
+def test_euler_hlle_and_outflow_both_sides_match_oracle():
+    """The reference's Euler solver (HLLE) with zero-gradient rows at both
+    ends of the mesh."""
+    s = knc.setups.euler_blast()
+    s.hydro.riemann_solver = knc.HLLE
+    s.inner = knc.setups.Boundary(knc.OUTFLOW)
+    ctx, sim = _run_pair(s, 300, 10, launch_mode=knc.BATCHED)
+    assert max_rel_l1(gpu_state(ctx), oracle_state(sim, s.blocks)) <= MANY_STEPS
