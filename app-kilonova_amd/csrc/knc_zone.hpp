@@ -58,14 +58,15 @@ __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b 
 // Converged when |dp| <= 1e-12 (tau + D + p), then c2p_srhd_polish.  Error
 // mapping follows relativistic_hydro.rs:68-88.
 // ---------------------------------------------------------------------------
-// Canonical final step: round the converged iterate to the power-of-two grid
-// 2^(ilogb(tau + D) - 33), which depends on the state only, and take one more
-// Newton step, so the recovered pressure does not depend on the starting
-// guess (the iterate is only defined to ~eps (tau + D + p)).
-__device__ inline double c2p_srhd_polish(double D, double S2, double E, const ZoneParams& zp, double p)
+// Canonical final steps: round the converged iterate to the power-of-two
+// grid 2^(ilogb(tau + D) - 33), which depends on the state only, and take a
+// Newton step; then the same on the grid 2^(ilogb(tau + D) - 40).  The
+// recovered pressure then does not depend on the starting guess except in
+// about one zone in 10^9 (the iterate is only defined to ~eps (tau + D + p)).
+__device__ inline double c2p_srhd_newton_step(double D, double S2, double E, const ZoneParams& zp, double p,
+                                              int shift)
 {
-    if (!(E > 0.0)) return p;
-    const int e = ilogb(E) - 33;
+    const int e = ilogb(E) - shift;
     p = ldexp(rint(ldexp(p, -e)), e);
     const double Q = E + p;
     const double den = Q * Q - S2;
@@ -74,6 +75,13 @@ __device__ inline double c2p_srhd_polish(double D, double S2, double E, const Zo
     const double pQ = p * Q;
     const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
     return p - F * den / (zp.gamma * p * S2 - den * Q);
+}
+
+__device__ inline double c2p_srhd_polish(double D, double S2, double E, const ZoneParams& zp, double p)
+{
+    if (!(E > 0.0)) return p;
+    p = c2p_srhd_newton_step(D, S2, E, zp, p, 33);
+    return c2p_srhd_newton_step(D, S2, E, zp, p, 40);
 }
 
 // The loop only iterates; lanes leave it as they converge or fail, and the
@@ -104,7 +112,7 @@ __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const Zon
         *iters = n > C2P_MAX_ITER ? C2P_MAX_ITER : n;
         return code;
     }
-    *iters = n + 1;
+    *iters = n + 2;
     *p_out = c2p_srhd_polish(D, S2, E, zp, p);
     return E_OK;
 }

@@ -27,6 +27,8 @@ written on stage 2 (fused RK average), i.e. 100 B per zone per stage
 launch on average; achieved = 100 B x zones / average stage duration,
 measured with HIP events on the context's stream.
 
+The default run is 2000 timed steps per GPU path (the benchmark's length).
+
 CPU baseline: the C++ oracle (oracle/) on the host cores, a bounded sample
 of the same mesh (100 RK2 steps after one warm-up step by default, about
 10-20 s on 16 threads), reported as zone-updates/s.
@@ -199,7 +201,7 @@ def copy_probe(mib=512, repeats=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fold", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=100)

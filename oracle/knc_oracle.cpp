@@ -81,16 +81,18 @@ struct Cons { double d, sr, sq, tau; };   // SRHD (D,S_r,S_q,tau); Euler (rho,m_
 // Error mapping follows relativistic_hydro.rs:68-88: D<0 and tau<0 are
 // errors, a negative recovered pressure is floored to 1e-3 rho.
 // ---------------------------------------------------------------------------
-// Canonical final step.  A converged iterate is only defined to within the
+// Canonical final steps.  A converged iterate is only defined to within the
 // rounding noise of f, ~eps (tau + D + p); different starting guesses land at
 // different points of that band.  Rounding the iterate to the power-of-two
 // grid 2^(ilogb(tau + D) - 33) (which depends on the state only) and taking
-// one more Newton step makes the result a function of the state alone,
-// except when two iterates straddle a grid boundary.
-double c2p_srhd_polish(double D, double S2, double E, double gamma, double p)
+// one Newton step makes the result a function of the state alone, except
+// when two iterates straddle a grid midpoint (probability ~ eps 2^33 ~ 1e-6).
+// A second rounding, to the finer grid 2^(ilogb(tau + D) - 40), and Newton
+// step maps those rare pairs together again except with probability ~5e-4,
+// so two starting guesses disagree in about one zone in 10^9.
+double c2p_srhd_newton_step(double D, double S2, double E, double gamma, double p, int shift)
 {
-    if (!(E > 0.0)) return p;
-    const int e = std::ilogb(E) - 33;
+    const int e = std::ilogb(E) - shift;
     p = std::ldexp(std::rint(std::ldexp(p, -e)), e);
     const double Q = E + p;
     const double den = Q * Q - S2;
@@ -99,6 +101,13 @@ double c2p_srhd_polish(double D, double S2, double E, double gamma, double p)
     const double pQ = p * Q;
     const double F = (gamma - 1.0) * (den - D * sq - pQ) - pQ;
     return p - F * den / (gamma * p * S2 - den * Q);
+}
+
+double c2p_srhd_polish(double D, double S2, double E, double gamma, double p)
+{
+    if (!(E > 0.0)) return p;
+    p = c2p_srhd_newton_step(D, S2, E, gamma, p, 33);
+    return c2p_srhd_newton_step(D, S2, E, gamma, p, 40);
 }
 
 int c2p_srhd_newton(double D, double S2, double tau, double gamma, double p,
@@ -117,7 +126,7 @@ int c2p_srhd_newton(double D, double S2, double tau, double gamma, double p,
         p = p - dp;
         if (!(std::fabs(p) < 1e300)) { *iters = n; return ROOT_FINDER_FAILED; }
         if (std::fabs(dp) <= C2P_TOL * (E + p)) {
-            *iters = n + 1;
+            *iters = n + 2;
             *p_out = c2p_srhd_polish(D, S2, E, gamma, p);
             return OK;
         }

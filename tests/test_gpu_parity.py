@@ -217,3 +217,25 @@ def test_euler_hlle_and_outflow_both_sides_match_oracle():
     s.inner = knc.setups.Boundary(knc.OUTFLOW)
     ctx, sim = _run_pair(s, 300, 10, launch_mode=knc.BATCHED)
     assert max_rel_l1(gpu_state(ctx), oracle_state(sim, s.blocks)) <= MANY_STEPS
+
+
+def test_c2p_error_codes_and_floor_match_oracle():
+    """The error mapping of RelativisticHydro::try_to_primitive
+    (src/physics/relativistic_hydro.rs:68-88) on crafted states: D < 0,
+    tau < 0, no root (|S| > tau + D), and a recovered pressure below zero
+    (floored to 1e-3 rho, reported as NegativePressure)."""
+    h = knc.relativistic_hydro()
+    ho = oracle.hydro(system=oracle.SRHD, gamma=4.0 / 3.0)
+    cold = knc.to_conserved(h, np.array([[1.0, 10.0, 0.0, 1e-12], [1e-3, 3.0, 4.0, 1e-14]]))
+    cold[:, 3] *= 1.0 - 1e-9                      # root moves below p = 0
+    cons = np.vstack([[[-1.0, 0.0, 0.0, 1.0]],      # NegativeDensity
+                      [[1.0, 0.0, 0.0, -1.0]],      # NegativeEnergyDensity
+                      [[1.0, 10.0, 0.0, 0.1]],      # no root: S^2 > (tau + D)^2
+                      cold])
+    p1, c1, i1 = knc.try_to_primitive(h, cons)
+    p2, c2, i2 = oracle.try_to_primitive(ho, cons)
+    assert list(c1) == [1, 3, 4, 2, 2]
+    assert np.array_equal(c1, c2) and np.array_equal(i1, i2)
+    floored = c1 == 2
+    assert np.array_equal(p1[floored], p2[floored])
+    assert np.array_equal(p1[floored, 3], 1e-3 * p1[floored, 0])
