@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -162,13 +163,22 @@ struct knc_context {
     double* d_dt_partial = nullptr;       // per-workgroup minima of the CFL pass
     double* d_dt = nullptr;
 
-    // graphs keyed by (starting slot, fold)
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    // graphs keyed by (starting slot, steps, first fold step for model boundaries)
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 
     // learned c2p guess: device copy of the network's operand fragments
     void* d_mlp = nullptr;
     MlpDev mlp{};
     bool has_mlp = false;
+
+    // model boundaries: ghost rows per stage of a fold, side-major within a
+    // stage (5 planes x 2nq each), filled on the host and uploaded per segment
+    knc_boundary_fn bsrc[2] = {nullptr, nullptr};
+    void* buser[2] = {nullptr, nullptr};
+    double* h_bseq = nullptr;   // pinned
+    double* d_bseq = nullptr;
+    size_t cap_bseq = 0;        // doubles
+    bool trial = false;         // autotuner trial steps: fixed rows only
 
     // moving excision surfaces
     bool exc_on = false;
@@ -440,7 +450,24 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage)
 
 // One RK step of RungeKuttaOrder (godunov-core) with the weighted averages of
 // src/state.rs:179-212 fused into the last stage.  Rotates c->cur.
-int enqueue_rk_step(knc_context* c, int step)
+// Ghost-row sets of stage `stage` of fold step `fold_step` for the model
+// boundaries (the stage kernel sees them as fixed rows).
+void stage_boundaries(knc_context* c, StageArgs& a, int fold_step, int stage)
+{
+    if (c->trial) return;
+    const size_t set = (size_t)10 * c->nq;
+    const size_t base = ((size_t)fold_step * c->h.runge_kutta_order + stage) * 2 * set;
+    if (c->bsrc[0]) {
+        a.bnd.inner_mode = BND_FIXED;
+        a.bnd.inner_rows = c->d_bseq + base;
+    }
+    if (c->bsrc[1]) {
+        a.bnd.outer_mode = BND_FIXED;
+        a.bnd.outer_rows = c->d_bseq + base + set;
+    }
+}
+
+int enqueue_rk_step(knc_context* c, int step, int fold_step)
 {
     const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
     StageArgs a = base_args(c);
@@ -449,22 +476,28 @@ int enqueue_rk_step(knc_context* c, int step)
     switch (c->h.runge_kutta_order) {
     case 1:
         a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        stage_boundaries(c, a, fold_step, 0);
         if ((rc = launch_one_stage(c, a, true))) return rc;
         c->cur = f1;
         break;
     case 2:
         a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        stage_boundaries(c, a, fold_step, 0);
         if ((rc = launch_one_stage(c, a, true))) return rc;
         a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.5;
+        stage_boundaries(c, a, fold_step, 1);
         if ((rc = launch_one_stage(c, a, false))) return rc;
         c->cur = f2;
         break;
     default:
         a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        stage_boundaries(c, a, fold_step, 0);
         if ((rc = launch_one_stage(c, a, true))) return rc;
         a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.75;
+        stage_boundaries(c, a, fold_step, 1);
         if ((rc = launch_one_stage(c, a, false))) return rc;
         a.in_slot = f2; a.out_slot = f1; a.avg_slot = c->cur; a.bweight = 1.0 / 3.0;
+        stage_boundaries(c, a, fold_step, 2);
         if ((rc = launch_one_stage(c, a, false))) return rc;
         c->cur = f1;
         break;
@@ -558,7 +591,7 @@ int require_blocks(knc_context* c)
 
 // Enqueues n RK steps from slot c->cur: replayed from a hipGraph captured
 // per (starting slot, n) in batched mode, launched one by one otherwise.
-int enqueue_steps(knc_context* c, int n)
+int enqueue_steps(knc_context* c, int n, int first)
 {
     if (n <= 0) return KNC_OK;
     const int start_slot = c->cur;
@@ -566,17 +599,19 @@ int enqueue_steps(knc_context* c, int n)
     if (!graph) {
         int rc;
         for (int s = 0; s < n; ++s)
-            if ((rc = enqueue_rk_step(c, s + 1))) return rc;
+            if ((rc = enqueue_rk_step(c, s + 1, first + s))) return rc;
         return KNC_OK;
     }
-    auto key = std::make_pair(start_slot, n);
+    // the ghost-row pointers of model boundaries depend on the fold step
+    const bool model = c->bsrc[0] || c->bsrc[1];
+    auto key = std::make_tuple(start_slot, n, model ? first : 0);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         hipGraph_t g;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int crc = KNC_OK;
         const int launches = c->last_launches;
-        for (int s = 0; s < n && crc == KNC_OK; ++s) crc = enqueue_rk_step(c, s + 1);
+        for (int s = 0; s < n && crc == KNC_OK; ++s) crc = enqueue_rk_step(c, s + 1, first + s);
         hipError_t ce = hipStreamEndCapture(c->stream, &g);
         c->cur = start_slot;
         c->last_launches = launches;
@@ -642,16 +677,77 @@ int excise(knc_context* c, double t, bool apply, bool* change)
     return KNC_OK;
 }
 
+// Room for the ghost-row sets of every stage of a fold.
+int ensure_boundary_sets(knc_context* c, int fold)
+{
+    const size_t need = (size_t)fold * c->h.runge_kutta_order * 2 * 10 * c->nq;
+    if (need <= c->cap_bseq) return KNC_OK;
+    drop_graphs(c);
+    if (c->d_bseq) HIP_TRY(hipFree(c->d_bseq));
+    if (c->h_bseq) HIP_TRY(hipHostFree(c->h_bseq));
+    c->d_bseq = nullptr;
+    c->h_bseq = nullptr;
+    c->cap_bseq = 0;
+    HIP_TRY(hipMalloc(&c->d_bseq, need * sizeof(double)));
+    HIP_TRY(hipHostMalloc(&c->h_bseq, need * sizeof(double), hipHostMallocDefault));
+    c->cap_bseq = need;
+    return KNC_OK;
+}
+
+// Calls the boundary sources for every stage of fold step s, whose input
+// state is at time t0: stage times t0, t0 + dt and, for RK3, the averaged
+// time of the second stage's output (src/state.rs:189-208).
+int fill_boundary_sets(knc_context* c, int s, double t0, double dt)
+{
+    const int order = c->h.runge_kutta_order;
+    const double t1 = t0 + dt;
+    const double times[3] = {t0, t1, (t1 + dt) * (-0.75 + 1.0) + t0 * 0.75};
+    const int nq = c->nq;
+    const int32_t inner = c->blocks.begin()->first, outer = c->blocks.rbegin()->first;
+    std::vector<double> prim((size_t)8 * nq), scal((size_t)2 * nq);
+    for (int st = 0; st < order; ++st)
+        for (int side = 0; side < 2; ++side) {
+            if (!c->bsrc[side]) continue;
+            const int32_t r = c->bsrc[side](c->buser[side], side, times[st], inner, outer, nq, prim.data(),
+                                            scal.data());
+            if (r) return fail(KNC_ERR_INVALID_ARGUMENT, "boundary source returned %d (side %d, t = %.6e)", r, side,
+                               times[st]);
+            double* d = c->h_bseq + (((size_t)s * order + st) * 2 + side) * 10 * nq;
+            const int n2 = 2 * nq;
+            for (int k = 0; k < n2; ++k) {
+                for (int q = 0; q < 4; ++q) d[(size_t)q * n2 + k] = prim[4 * k + q];
+                d[(size_t)4 * n2 + k] = scal[k];
+            }
+        }
+    return KNC_OK;
+}
+
+// Enqueues fold steps [first, first + n), uploading their ghost-row sets
+// first when a side has a model boundary.
+int flush_steps(knc_context* c, int first, int n)
+{
+    if (n <= 0) return KNC_OK;
+    if (c->bsrc[0] || c->bsrc[1]) {
+        const size_t per_step = (size_t)c->h.runge_kutta_order * 2 * 10 * c->nq;
+        HIP_TRY(hipMemcpyAsync(c->d_bseq + first * per_step, c->h_bseq + first * per_step,
+                               n * per_step * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    return enqueue_steps(c, n, first);
+}
+
 // Runs `fold` RK steps after an optional dt pass (scheme::advance,
 // src/scheme.rs:212-240); returns after the stream drained and the error log
 // was read.  With moving excision surfaces the steps run in segments between
-// block-set changes: dt is read back first, so the state time of every step
-// is known on the host.
+// block-set changes; with moving surfaces or model boundaries dt is read back
+// first, so the state time of every step is known on the host.
 int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* dt_used, knc_error* err)
 {
     int rc;
     if ((rc = require_blocks(c))) return rc;
     c->last_launches = 0;
+    const bool moving = with_dt && excision_moving(c);
+    const bool model = c->bsrc[0] || c->bsrc[1];
+    if (model && (rc = ensure_boundary_sets(c, fold))) return rc;
     hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
     HIP_TRY(hipGetLastError());
     if (with_dt) {
@@ -660,15 +756,15 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
         hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->d_dt, dt_fixed);
         HIP_TRY(hipGetLastError());
     }
-    const bool moving = with_dt && excision_moving(c);
+    const bool host_dt = with_dt && (moving || model);
     double dt = dt_fixed;
-    if (moving) {
+    if (host_dt) {
         HIP_TRY(hipStreamSynchronize(c->stream));
         HIP_TRY(hipMemcpy(&dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost));
         if ((rc = collect_error(c, err))) return rc;
     }
     if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    int pending = 0;
+    int pending = 0, first = 0;
     double t = c->time;
     int64_t iter = c->iteration;
     const int every = std::max(1, c->exc.check_every);
@@ -677,7 +773,8 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
             bool change = false;
             if ((rc = excise(c, t, false, &change))) return rc;
             if (change) {
-                if ((rc = enqueue_steps(c, pending))) return rc;
+                if ((rc = flush_steps(c, first, pending))) return rc;
+                first = s;
                 pending = 0;
                 HIP_TRY(hipStreamSynchronize(c->stream));
                 if ((rc = collect_error(c, err))) return rc;
@@ -687,14 +784,15 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
                 if ((rc = require_blocks(c))) return rc;
             }
         }
+        if (model && (rc = fill_boundary_sets(c, s, t, dt))) return rc;
         ++pending;
         t = advance_time(c->h.runge_kutta_order, t, dt);
         ++iter;
     }
-    if ((rc = enqueue_steps(c, pending))) return rc;
+    if ((rc = flush_steps(c, first, pending))) return rc;
     if (c->timing) HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (!moving) HIP_TRY(hipMemcpy(&dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost));
+    if (!host_dt) HIP_TRY(hipMemcpy(&dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost));
     if (c->timing) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -703,7 +801,7 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
     }
     if (dt_used) *dt_used = dt;
     if ((rc = collect_error(c, err))) return rc;
-    if (moving) {
+    if (host_dt || model) {
         c->time = t;
         c->iteration = iter;
     } else {
@@ -742,11 +840,17 @@ int time_shape(knc_context* c, int threads, int rows, int passes, int trials, do
     c->dirty = true;
     if ((rc = rebuild_tables(c))) return rc;
     const int saved = c->cur;
+    c->trial = true;
     for (int w = 0; w < trials + 1; ++w) {
         if (w == 1) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-        if ((rc = enqueue_rk_step(c, 1))) return rc;
+        rc = enqueue_rk_step(c, 1, 0);
         c->cur = saved;
+        if (rc) {
+            c->trial = false;
+            return rc;
+        }
     }
+    c->trial = false;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
     float t = 0.f;
@@ -926,7 +1030,8 @@ void knc_context_destroy(knc_context* c)
         (void)hipFree(kv.second.prim);
     }
     void* ptrs[] = {c->d_theta, c->d_bnd, c->d_blocks, c->d_items, c->d_nitems, c->d_err, c->d_stats,
-                    c->d_dt_partial, c->d_dt, c->d_mlp};
+                    c->d_dt_partial, c->d_dt, c->d_mlp, c->d_bseq};
+    if (c->h_bseq) (void)hipHostFree(c->h_bseq);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1107,6 +1212,17 @@ int knc_advance(knc_context* c, int32_t fold, double* dt_used, knc_error* err)
 {
     if (fold < 1) return fail(KNC_ERR_INVALID_ARGUMENT, "fold must be >= 1");
     return run_steps(c, true, 0.0, fold, dt_used, err);
+}
+
+int knc_set_boundary_source(knc_context* c, int32_t side, knc_boundary_fn fn, void* user)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (side != KNC_SIDE_INNER && side != KNC_SIDE_OUTER) return fail(KNC_ERR_INVALID_ARGUMENT, "bad side");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->bsrc[side] = fn;
+    c->buser[side] = fn ? user : nullptr;
+    drop_graphs(c);
+    return KNC_OK;
 }
 
 int knc_set_excision(knc_context* c, const knc_excision* ex)

@@ -59,6 +59,10 @@ class BlockData(ctypes.Structure):
                 ("conserved", ctypes.c_void_p), ("scalar_mass", ctypes.c_void_p)]
 
 
+BOUNDARY_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double))
+
 BLOCK_SOURCE_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
                                    ctypes.POINTER(BlockData))
 
@@ -141,6 +145,7 @@ def lib():
     L.knc_set_timing.argtypes = [P, I32]
     L.knc_last_stage_time.argtypes = [P, P, P]
     L.knc_set_excision.argtypes = [P, ctypes.POINTER(Excision)]
+    L.knc_set_boundary_source.argtypes = [P, I32, BOUNDARY_FN, P]
     L.knc_excision_counts.argtypes = [P, P, P]
     L.knc_autotune.argtypes = [P, I32, I32, ctypes.POINTER(TuneResult)]
     L.knc_hydro_try_to_primitive.argtypes = [ctypes.POINTER(Hydro), I64, P, P, P, P, P]
@@ -225,6 +230,32 @@ class Context:
             primitive = _f64(primitive)
             scalar = _f64(scalar)
         _check(lib().knc_set_boundary(self._h, side, mode, _p(primitive), _p(scalar)))
+
+    def set_boundary_source(self, side, fn):
+        """Model boundary: ``fn(side, time, inner_index, outer_index)`` returns
+        the ghost rows ``(primitive (2, nq, 4), scalar (2, nq))`` at a stage's
+        input time (BlockState::from_model on the boundary block, then c2p;
+        src/scheme.rs:46-52); None returns the side to its fixed rows."""
+        if not hasattr(self, "_bsrc"):
+            self._bsrc = [None, None]
+        if fn is None:
+            self._bsrc[side] = None
+            _check(lib().knc_set_boundary_source(self._h, side, BOUNDARY_FN(), None))
+            return
+
+        def call(user, sd, t, inner, outer, nq, prim, scalar):
+            try:
+                P, S = fn(int(sd), float(t), int(inner), int(outer))
+                np.ctypeslib.as_array(prim, shape=(8 * nq,))[:] = np.asarray(P, np.float64).reshape(-1)
+                np.ctypeslib.as_array(scalar, shape=(2 * nq,))[:] = np.asarray(S, np.float64).reshape(-1)
+                return 0
+            except Exception as e:  # reported through the advance call's error
+                self._birth_error = e
+                return 1
+
+        cb = BOUNDARY_FN(call)
+        self._bsrc[side] = cb
+        _check(lib().knc_set_boundary_source(self._h, side, cb, None))
 
     # -- time ----------------------------------------------------------------
     def set_time(self, time, iteration=0):

@@ -144,6 +144,20 @@ int knc_block_count(knc_context* ctx, int32_t* count, int32_t* inner_index, int3
  * model-built ghost blocks of src/scheme.rs:46-52. */
 int knc_set_boundary(knc_context* ctx, int32_t side, int32_t mode, const double* primitive, const double* scalar);
 
+/* Ghost rows from the initial model at every stage's input time, as
+ * try_advance_rk builds them (BlockState::from_model on the boundary block
+ * mesh.subgrid(inner_index - 1) for side 0 or mesh.subgrid(outer_index + 1)
+ * for side 1 at state.time, then c2p; src/scheme.rs:46-52).  The callback
+ * fills the two rows next to the mesh edge, in radial order: primitive
+ * (2*nq*4) and scalar concentration rho*X (2*nq).  It is called for every
+ * stage of an advance call before the steps are launched (the stage times
+ * follow from the fold's dt); it returns 0, or nonzero to abort the call. */
+typedef int32_t (*knc_boundary_fn)(void* user, int32_t side, double time, int32_t inner_index,
+                                   int32_t outer_index, int32_t nq, double* primitive, double* scalar);
+/* Makes `side` a model boundary fed by fn; fn = NULL returns it to the rows
+ * of knc_set_boundary. */
+int knc_set_boundary_source(knc_context* ctx, int32_t side, knc_boundary_fn fn, void* user);
+
 int knc_set_time(knc_context* ctx, double time, int64_t iteration);
 int knc_get_time(knc_context* ctx, double* time, int64_t* iteration);
 

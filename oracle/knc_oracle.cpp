@@ -528,10 +528,17 @@ struct C2PStats {
     std::atomic<long long> zones{0}, iterations{0}, floors{0};
 };
 
+// Ghost rows evaluated at each stage's input time, as BlockState::from_model
+// on the boundary blocks at state.time (src/scheme.rs:46-52): fills prim
+// (2*nq*4) and scalar (2*nq) for side 0 (inner) or 1 (outer).
+typedef int (*BoundarySource)(void* user, int side, double time, int nq, double* prim, double* scalar);
+
 struct Sim {
     Hydro h;
     std::vector<Block> blocks;   // radial order
     Boundary inner, outer;
+    BoundarySource source[2] = {nullptr, nullptr};
+    void* source_user[2] = {nullptr, nullptr};
     double time = 0.0;
     long long iteration = 0;
     int nthreads = 1;
@@ -777,6 +784,31 @@ ErrRec time_step(Sim& sim, double* dt)
 
 // RungeKuttaOrder::try_advance with the SSP weights of godunov-core:
 // RK1: U1; RK2: avg(L(L(U0)), U0; 1/2); RK3: b = 3/4 then 1/3.
+// Refreshes the ghost rows of the sides that have a boundary source, at
+// the time of the state the next stage starts from.
+int refresh_boundaries(Sim& sim, double t)
+{
+    for (int side = 0; side < 2; ++side) {
+        if (!sim.source[side]) continue;
+        Boundary& bd = side == 0 ? sim.inner : sim.outer;
+        const int nq = sim.blocks.empty() ? 0 : sim.blocks.front().nq;
+        std::vector<double> prim((size_t)8 * nq), scalar((size_t)2 * nq);
+        if (sim.source[side](sim.source_user[side], side, t, nq, prim.data(), scalar.data())) return 1;
+        bd.mode = BND_FIXED;
+        bd.rows.resize((size_t)2 * nq);
+        for (int k = 0; k < 2 * nq; ++k)
+            bd.rows[k] = PS{{prim[4 * k], prim[4 * k + 1], prim[4 * k + 2], prim[4 * k + 3]}, scalar[k]};
+    }
+    return 0;
+}
+
+ErrRec boundary_source_failed()
+{
+    ErrRec e;
+    e.code = 10;
+    return e;
+}
+
 ErrRec rk_step(Sim& sim, double dt)
 {
     const int nb = (int)sim.blocks.size();
@@ -787,6 +819,9 @@ ErrRec rk_step(Sim& sim, double dt)
     const double t0 = sim.time;
     double t;
     const int order = sim.h.runge_kutta_order;
+    // each stage's ghost rows come from the model at its input state's time:
+    // t0, t0 + dt, and (RK3) the averaged time of the second stage's output
+    if (refresh_boundaries(sim, t0)) return boundary_source_failed();
     if (order == 1) {
         e = stage(sim, U0, M0, nullptr, nullptr, 0.0, dt, U1, M1);
         if (e.code) return e;
@@ -794,6 +829,7 @@ ErrRec rk_step(Sim& sim, double dt)
     } else if (order == 2) {
         e = stage(sim, U0, M0, nullptr, nullptr, 0.0, dt, U1, M1);
         if (e.code) return e;
+        if (refresh_boundaries(sim, t0 + dt)) return boundary_source_failed();
         e = stage(sim, U1, M1, &U0, &M0, 0.5, dt, U2, M2);
         if (e.code) return e;
         U1.swap(U2); M1.swap(M2);
@@ -802,11 +838,13 @@ ErrRec rk_step(Sim& sim, double dt)
     } else {
         e = stage(sim, U0, M0, nullptr, nullptr, 0.0, dt, U1, M1);
         if (e.code) return e;
+        if (refresh_boundaries(sim, t0 + dt)) return boundary_source_failed();
         e = stage(sim, U1, M1, &U0, &M0, 0.75, dt, U2, M2);
         if (e.code) return e;
         const double t1 = t0 + dt;
         const double t2 = (t1 + dt) * (-0.75 + 1.0) + t0 * 0.75;
         const double b3 = 1.0 / 3.0;
+        if (refresh_boundaries(sim, t2)) return boundary_source_failed();
         e = stage(sim, U2, M2, &U0, &M0, b3, dt, U1, M1);
         if (e.code) return e;
         t = (t2 + dt) * (-b3 + 1.0) + t0 * b3;
@@ -986,6 +1024,16 @@ int ko_sim_set_boundary(void* p, int side, int mode, int nq, const double* prim,
         for (int k = 0; k < 2 * nq; ++k)
             bd.rows[k] = PS{{prim[4 * k], prim[4 * k + 1], prim[4 * k + 2], prim[4 * k + 3]}, scalar[k]};
     }
+    return 0;
+}
+
+// Ghost rows of `side` from a source evaluated at every stage's input time
+// (NULL: back to the rows set by ko_sim_set_boundary).
+int ko_sim_set_boundary_source(void* p, int side, BoundarySource fn, void* user)
+{
+    Sim* s = (Sim*)p;
+    s->source[side] = fn;
+    s->source_user[side] = user;
     return 0;
 }
 

@@ -67,6 +67,9 @@ def hydro(system=SRHD, gamma=4.0 / 3.0, plm_theta=1.5, cfl=0.4, rk=2, riemann=HL
 
 _lib = None
 
+BOUNDARY_SOURCE = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
 
 def lib():
     global _lib
@@ -88,6 +91,8 @@ def lib():
         L.ko_sim_iteration.argtypes = [P]
         L.ko_sim_advance.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(KoError)]
         L.ko_plm_gradient.argtypes = [ctypes.c_double, ctypes.c_int64, P, P, P, P]
+        L.ko_sim_set_boundary_source.restype = ctypes.c_int
+        L.ko_sim_set_boundary_source.argtypes = [P, ctypes.c_int, BOUNDARY_SOURCE, P]
         _lib = L
     return _lib
 
@@ -201,6 +206,7 @@ class Sim:
         self.nq = None
         self.qv = None
         self.radial_vertices = []
+        self._sources = [None, None]
 
     def __del__(self):
         if getattr(self, "_p", None):
@@ -261,6 +267,27 @@ class Sim:
 
     def num_blocks(self):
         return lib().ko_sim_num_blocks(self._p)
+
+    def set_boundary_source(self, side, fn):
+        """Ghost rows of `side` from fn(side, time) -> (prim (2, nq, 4),
+        scalar (2, nq)) at every stage's input time, or None to clear."""
+        if fn is None:
+            self._sources[side] = None
+            lib().ko_sim_set_boundary_source(self._p, side, None, None)
+            return
+
+        def call(user, sd, t, nq, prim, scalar):
+            try:
+                P, S = fn(int(sd), float(t))
+                np.ctypeslib.as_array(prim, shape=(8 * nq,))[:] = np.asarray(P, np.float64).reshape(-1)
+                np.ctypeslib.as_array(scalar, shape=(2 * nq,))[:] = np.asarray(S, np.float64).reshape(-1)
+                return 0
+            except Exception:
+                return 1
+
+        cb = BOUNDARY_SOURCE(call)
+        self._sources[side] = cb
+        lib().ko_sim_set_boundary_source(self._p, side, cb, None)
 
     def set_boundary(self, side, mode, prim=None, scalar=None):
         nq = self.nq
