@@ -67,3 +67,23 @@ def test_null_context_arguments_are_errors():
     assert L.knc_autotune(None, 1, 0, None) == 10
     assert L.knc_advance(None, 1, None, None) == 10
     assert L.knc_block_remove(None, 0) == 10
+
+
+DRIVER = os.path.join(os.path.dirname(capi.LIB_PATH), "knc_drive")
+
+
+def test_compiled_driver_is_built_and_checks_its_arguments():
+    """knc_drive is a plain C++ caller of the C ABI (no Python, no HIP
+    headers); bad arguments are rejected before any device call."""
+    import subprocess
+    assert os.path.exists(DRIVER)
+    r = subprocess.run([DRIVER, "100", "16", "3"], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+def test_compiled_driver_advances_a_mesh():
+    import subprocess
+    r = subprocess.run([DRIVER, "512", "128", "4", "40", "10", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "G zone-updates/s" in r.stdout and "iteration 50" in r.stdout
