@@ -224,8 +224,9 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X; multi-GPU sharding of the "
-              "radial block chain is described in DESIGN.md but not built", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs on a single MI355X; sharding the radial block "
+              "chain across GPUs (halo rows per cut, a min-reduction of dt per fold) is not implemented",
+              file=sys.stderr)
         sys.exit(2)
 
     import knc
