@@ -1263,14 +1263,8 @@ int knc_context_set_mlp(knc_context* c, const knc_mlp* mm)
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (!c->d_mlp) HIP_TRY(hipMalloc(&c->d_mlp, bytes));
     HIP_TRY(hipMemcpy(c->d_mlp, m->d_frag, bytes, hipMemcpyDeviceToDevice));
-    const char* src = (const char*)m->d_frag;
-    char* dst = (char*)c->d_mlp;
     c->mlp = m->dev;
-    c->mlp.a1 = (const bf16x8*)(dst + ((const char*)m->dev.a1 - src));
-    c->mlp.a2 = (const bf16x8*)(dst + ((const char*)m->dev.a2 - src));
-    c->mlp.c1 = (const floatx16*)(dst + ((const char*)m->dev.c1 - src));
-    c->mlp.c2 = (const floatx16*)(dst + ((const char*)m->dev.c2 - src));
-    c->mlp.w3 = (const floatx16*)(dst + ((const char*)m->dev.w3 - src));
+    c->mlp.image = (const uint4*)c->d_mlp;
     c->has_mlp = true;
     drop_graphs(c);
     return KNC_OK;

@@ -539,35 +539,95 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
 // ---------------------------------------------------------------------------
 // Learned c2p guess pass: before a learned-guess stage, the network's
 // log2(p/D) for every zone of the stage's input into the blocks' guess
-// planes.  One workgroup per (work item, row), NT columns per pass; the
-// inference is wavefront-wide (BF16 MFMA), so every lane runs it and lanes
-// past the item's last column evaluate that column without storing.
+// planes.  A workgroup walks its (work item row, NT-column pass) steps in
+// grid-stride order and issues the loads of the next step before the
+// inference of the current one, so the wavefront's HBM latency overlaps
+// the MFMA/LDS work.  The inference is wavefront-wide, so every lane runs
+// it and lanes past the item's last column evaluate that column without
+// storing.
 // ---------------------------------------------------------------------------
-template <int NT>
-__global__ void __launch_bounds__(NT) k_guess(StageArgs a)
+// One step of a workgroup: unit u = (item, row), pass columns c0.., and
+// the item's bounds.  All fields are uniform over the workgroup.
+struct GuessStep {
+    int u, c0, block, i, j1;
+};
+
+// The workgroup's next step after p (c0 < 0: the first step at or after
+// unit p.u); false when none is left.
+__device__ inline bool guess_advance(const StageArgs& a, int nunits, int nt, GuessStep& p)
 {
-    __shared__ float2 tanh_tab[TANH_N];
-    tanh_table_init(tanh_tab);
-    const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
-    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    int u = p.u, c0 = p.c0;
+    if (c0 >= 0) {
+        c0 += nt;
+        if (c0 < p.j1) {
+            p.c0 = c0;
+            return true;
+        }
+        u += gridDim.x;
+    }
+    for (; u < nunits; u += gridDim.x) {
         const WorkItem it = a.items[u / a.rows];
         const int i = it.i0 + u % a.rows;
-        if (i >= it.i1) continue;
-        const BlockDesc B = a.blocks[it.block];
-        const int nq = B.nq;
-        const int64_t np = (int64_t)B.nr * nq;
-        const double* U = slot_of(B, a.in_slot) + (int64_t)i * nq;
-        const double rvolA = gld(rg_rvolA(B), i);
-        for (int c0 = it.j0; c0 < it.j1; c0 += NT) {
-            const int j = c0 + (int)threadIdx.x;
-            const bool act = j < it.j1;
-            const int jj = act ? j : it.j1 - 1;
-            const double rdv = rvolA * gld(a.th.rdcost, jj);
-            const double D = gld(U, jj) * rdv, sr = gld(U, np + jj) * rdv, sq = gld(U, 2 * np + jj) * rdv;
-            const double tau = gld(U, 3 * np + jj) * rdv;
-            const float l2 = mlp_log2_guess_wave(a.mlp, tanh_tab, D, sr * sr + sq * sq, tau, a.zp.gamma);
-            if (act) gstf(B.guess, (int64_t)i * nq + j, l2);
+        if (i < it.i1) {
+            p = GuessStep{u, it.j0, it.block, i, it.j1};
+            return true;
         }
+    }
+    return false;
+}
+
+// The raw inputs of one lane for one step: four conserved planes and the
+// inverse zone volume.
+struct GuessIn {
+    double u[4];
+    double rdv;
+};
+
+__device__ inline GuessIn guess_load(const StageArgs& a, const GuessStep& p)
+{
+    const BlockDesc B = a.blocks[p.block];
+    const int j = p.c0 + (int)threadIdx.x;
+    const int jj = j < p.j1 ? j : p.j1 - 1;
+    const int64_t np = (int64_t)B.nr * B.nq;
+    const double* U = slot_of(B, a.in_slot) + (int64_t)p.i * B.nq + jj;
+    GuessIn x;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) x.u[f] = gld(U, f * np);
+    x.rdv = gld(rg_rvolA(B), p.i) * gld(a.th.rdcost, jj);
+    return x;
+}
+
+// Four wavefronts per SIMD (at most 128 VGPRs) from 128-thread workgroups
+// up: with all weight fragments held in registers the compiler settles at
+// three, and the extra wavefront hides more of the MFMA -> tanh -> MFMA
+// dependency chain (125 vs 146 us per pass on bomb_jet 4096x1024, no spills
+// in the inner loop).  A 64-thread workgroup is held to two per SIMD by its
+// 20 KiB of LDS anyway.
+template <int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 128 ? 4 : 1))) k_guess(StageArgs a)
+{
+    __shared__ MlpShared mlp;
+    mlp_shared_load(&mlp, a.mlp);
+    const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
+    GuessStep p{(int)blockIdx.x, -1, 0, 0, 0};
+    if (!guess_advance(a, nunits, NT, p)) return;
+    GuessIn cur = guess_load(a, p);
+    for (;;) {
+        GuessStep q = p;
+        const bool more = guess_advance(a, nunits, NT, q);
+        GuessIn nxt{};
+        if (more) nxt = guess_load(a, q);
+        const double D = cur.u[0] * cur.rdv, sr = cur.u[1] * cur.rdv, sq = cur.u[2] * cur.rdv;
+        const double tau = cur.u[3] * cur.rdv;
+        const float l2 = mlp_log2_guess_wave(a.mlp, &mlp, D, sr * sr + sq * sq, tau, a.zp.gamma);
+        const int j = p.c0 + (int)threadIdx.x;
+        if (j < p.j1) {
+            const BlockDesc& B = a.blocks[p.block];
+            gstf(B.guess, (int64_t)p.i * B.nq + j, l2);
+        }
+        if (!more) break;
+        p = q;
+        cur = nxt;
     }
 }
 

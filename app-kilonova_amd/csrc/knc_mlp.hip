@@ -178,8 +178,8 @@ __global__ void k_mlp_features(int64_t n, const double* cons, const double* gamm
 __global__ void __launch_bounds__(256) k_mlp_predict(MlpDev M, int64_t n, const double* cons, const double* gamma,
                                                      float* y, double* guess)
 {
-    __shared__ float2 tanh_tab[TANH_N];
-    tanh_table_init(tanh_tab);
+    __shared__ MlpShared sh;
+    mlp_shared_load(&sh, M);
     const int64_t base = (int64_t)blockIdx.x * blockDim.x;
     for (int64_t b0 = base; b0 < n; b0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t k = b0 + threadIdx.x;
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_mlp_predict(MlpDev M, int64_t n, const 
         float f[4], x[4];
         mlp_raw_features(D, S2, cons[4 * kk + 3], gamma[kk], f);
         for (int m = 0; m < 4; ++m) x[m] = (f[m] - M.norm.shift[m]) * M.norm.scale[m];
-        const float out = mlp_wave(M, tanh_tab, x);
+        const float out = mlp_wave(M, &sh, x);
         if (valid) {
             if (y) y[k] = out;
             if (guess) guess[k] = D * (double)__builtin_amdgcn_exp2f(out / M.norm.tscale + M.norm.tshift);
@@ -202,9 +202,12 @@ __global__ void __launch_bounds__(256) k_mlp_predict(MlpDev M, int64_t n, const 
 // ---------------------------------------------------------------------------
 int MlpHost::upload_fragments()
 {
-    // a1[mt][lane][i], a2[mt][c][lane][i] (bf16), c1/c2/w3[mt][g][v] (f32)
-    std::vector<__bf16> a1(2 * 64 * 8), a2(2 * 4 * 64 * 8);
-    std::vector<float> c1(2 * 2 * 16), c2(2 * 2 * 16), w3(2 * 2 * 16);
+    // The MlpShared layout: a1 | a2b | a2 | a3 (bf16x8 per lane) | tanh table.
+    std::vector<__bf16> img(MLP_IMAGE_FRAGS * 8, (__bf16)0.0f);
+    __bf16* a1 = img.data();
+    __bf16* a2b = a1 + 2 * 64 * 8;
+    __bf16* a2 = a2b + 2 * 64 * 8;
+    __bf16* a3 = a2 + 2 * 4 * 64 * 8;
     const float* W1 = params.data() + MLP_OFF_W1;
     const float* B1 = params.data() + MLP_OFF_B1;
     const float* W2 = params.data() + MLP_OFF_W2;
@@ -215,44 +218,29 @@ int MlpHost::upload_fragments()
             const int g = lane / 32, row = 32 * mt + lane % 32;
             for (int i = 0; i < 8; ++i) {
                 const int k = 8 * g + i;
-                a1[(mt * 64 + lane) * 8 + i] = (__bf16)(k < MLP_IN ? W1[row * MLP_IN + k] : 0.0f);
+                a1[(mt * 64 + lane) * 8 + i] =
+                    (__bf16)(k < MLP_IN ? W1[row * MLP_IN + k] : (k == MLP_IN ? B1[row] : 0.0f));
+                a2b[(mt * 64 + lane) * 8 + i] = (__bf16)(k == 0 ? B2[row] : 0.0f);
                 for (int c = 0; c < 4; ++c)
                     a2[((mt * 4 + c) * 64 + lane) * 8 + i] = (__bf16)W2[row * MLP_H + mlp_perm(c, g, i)];
             }
         }
-    for (int mt = 0; mt < 2; ++mt)
-        for (int g = 0; g < 2; ++g)
-            for (int v = 0; v < 16; ++v) {
-                const int row = 32 * mt + mfma_row(g, v);
-                c1[(mt * 2 + g) * 16 + v] = B1[row];
-                c2[(mt * 2 + g) * 16 + v] = B2[row];
-                w3[(mt * 2 + g) * 16 + v] = W3[row];
-            }
+    for (int c = 0; c < 4; ++c)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int i = 0; i < 8; ++i)
+                a3[(c * 64 + lane) * 8 + i] = (__bf16)(lane % 32 == 0 ? W3[mlp_perm(c, lane / 32, i)] : 0.0f);
+    std::vector<uint16_t> tab(TANH_N);
+    for (int m = 0; m < TANH_N; ++m) {
+        const __bf16 t = (__bf16)(float)std::tanh((m - TANH_MID) / (double)TANH_RH);
+        std::memcpy(&tab[m], &t, 2);
+    }
     if (!d_frag) {
         if (hipMalloc(&d_frag, frag_bytes()) != hipSuccess) return KNC_ERR_HIP;
     }
-    char* base = (char*)d_frag;
-    size_t off = 0;
-    auto put = [&](const void* src, size_t bytes) {
-        hipError_t e = hipMemcpy(base + off, src, bytes, hipMemcpyHostToDevice);
-        off += (bytes + 255) / 256 * 256;
-        return e;
-    };
-    if (put(a1.data(), a1.size() * 2) != hipSuccess || put(a2.data(), a2.size() * 2) != hipSuccess ||
-        put(c1.data(), c1.size() * 4) != hipSuccess || put(c2.data(), c2.size() * 4) != hipSuccess ||
-        put(w3.data(), w3.size() * 4) != hipSuccess)
+    if (hipMemcpy(d_frag, img.data(), img.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((char*)d_frag + img.size() * 2, tab.data(), tab.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
         return KNC_ERR_HIP;
-    off = 0;
-    auto at = [&](size_t bytes) {
-        void* p = base + off;
-        off += (bytes + 255) / 256 * 256;
-        return p;
-    };
-    dev.a1 = (const bf16x8*)at(a1.size() * 2);
-    dev.a2 = (const bf16x8*)at(a2.size() * 2);
-    dev.c1 = (const floatx16*)at(c1.size() * 4);
-    dev.c2 = (const floatx16*)at(c2.size() * 4);
-    dev.w3 = (const floatx16*)at(w3.size() * 4);
+    dev.image = (const uint4*)d_frag;
     dev.b3 = params[MLP_OFF_B3];
     std::memcpy(dev.norm.shift, norm.data(), 4 * sizeof(float));
     std::memcpy(dev.norm.scale, norm.data() + 4, 4 * sizeof(float));
@@ -261,7 +249,7 @@ int MlpHost::upload_fragments()
     return KNC_OK;
 }
 
-size_t MlpHost::frag_bytes() { return 256 * 2 + (2 * 64 * 8 * 2 + 2 * 4 * 64 * 8 * 2 + 3 * 2 * 2 * 16 * 4) + 5 * 256; }
+size_t MlpHost::frag_bytes() { return MLP_IMAGE_BYTES; }
 
 }  // namespace knc
 

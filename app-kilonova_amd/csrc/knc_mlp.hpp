@@ -41,20 +41,35 @@ struct MlpNorm {
     float tscale;
 };
 
-// Weights laid out as MFMA operand fragments (built on the host):
+// Inference image: the weights as MFMA operand fragments plus the tanh
+// table, built on the host and copied whole into each workgroup's LDS
+// (MlpShared).  Fragments (one bf16x8 per lane):
 //   a1[mt][lane]      layer-1 A operand, rows = hidden 32mt + lane%32,
-//                     k = 8(lane/32) + i (features 0..3, zero padded)
+//                     k = 8(lane/32) + i: W1 for k = 0..3, b1 for k = 4
+//                     (the B operand carries 1 there), zero beyond
+//   a2b[mt][lane]     layer-2 bias as an A operand: b2 of row 32mt + lane%32
+//                     at k = 0, zero elsewhere (B operand: 1 at k = 0)
 //   a2[mt][c][lane]   layer-2 A operand, rows = output 32mt + lane%32,
 //                     k-chunk c with the hidden units permuted so that the
 //                     layer-1 accumulator registers are the B operand as is
-//   c1/c2/w3[mt][g]   per-lane-group fragments (16 floats) of b1, b2, W3 in
-//                     the accumulator row order 8(v/4) + 4g + v%4
+//   a3[c][lane]       layer-3 A operand: row 0 (lanes 0 and 32) holds W3 in
+//                     the same permuted k order, rows 1..31 are zero
+//   tanh              the TANH_N BF16 tanh entries
+constexpr int MLP_IMAGE_FRAGS = 2 * 64 + 2 * 64 + 2 * 4 * 64 + 4 * 64;
+constexpr int TANH_N = 2048;
+constexpr int MLP_IMAGE_BYTES = MLP_IMAGE_FRAGS * 16 + TANH_N * 2;   // 20 KiB
+
+struct MlpShared {
+    bf16x8 a1[2 * 64];
+    bf16x8 a2b[2 * 64];
+    bf16x8 a2[2 * 4 * 64];
+    bf16x8 a3[4 * 64];
+    uint16_t tanh[TANH_N];
+};
+static_assert(sizeof(MlpShared) == MLP_IMAGE_BYTES, "inference image layout");
+
 struct MlpDev {
-    const bf16x8* a1;
-    const bf16x8* a2;
-    const floatx16* c1;
-    const floatx16* c2;
-    const floatx16* w3;
+    const uint4* image;   // MLP_IMAGE_BYTES, the MlpShared layout
     float b3;
     MlpNorm norm;
 };
@@ -66,32 +81,53 @@ __device__ inline float fast_tanh(float x)
     return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
-// Inference tanh from a table in LDS: TANH_N intervals over [-8, 8], each
-// entry (tanh(x_i), tanh(x_{i+1}) - tanh(x_i)), linear interpolation.  The
-// interpolation error is at most h^2/8 max|tanh''| = 2.4e-5 (h = 1/64), far
-// below the BF16 resolution of the next layer's operands, and it costs no
-// transcendental instruction (exp2 and rcp issue at a quarter of the rate).
-constexpr int TANH_N = 1024;
-constexpr float TANH_X0 = -8.0f;
-constexpr float TANH_RH = TANH_N / 16.0f;   // intervals per unit of x
+// Inference tanh: a table of TANH_N BF16 values in LDS, entry m holding
+// tanh((m - 1024)/256) rounded to BF16, read at the nearest entry.  The
+// index is formed two zones at a time in packed FP16: m 2^-24 =
+// x 2^-16 + 2^-14, clamped to [0, 2047 2^-24].  Below 2^-13 the FP16 spacing
+// is 2^-24, so the bit pattern of the rounded value is the integer m
+// (subnormals included, which the kernels keep: FP16 denormal mode is
+// IEEE).  For |x| < 4, |x - x_m| < 2^-8 (FP16 truncation of x, below 2^-9,
+// plus half an entry), so the entry is within 2^-8 of tanh(x) before its
+// BF16 rounding; beyond, tanh(x) rounds to +-1 in BF16 like the end
+// entries.  The entries are part of the inference image.
+constexpr int TANH_MID = 1024;       // entry of x = 0
+constexpr float TANH_RH = 256.0f;    // entries per unit of x
 
-// Fills the table; every thread of the workgroup must call it, and it ends
-// with a barrier.
-__device__ inline void tanh_table_init(float2* tab)
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef short short2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Copies the inference image into the workgroup's LDS; every thread of the
+// workgroup must call it, and it ends with a barrier.
+__device__ inline void mlp_shared_load(MlpShared* sh, const MlpDev& M)
 {
-    for (int i = threadIdx.x; i < TANH_N; i += blockDim.x) {
-        const float a = tanhf(TANH_X0 + i / TANH_RH);
-        const float b = tanhf(TANH_X0 + (i + 1) / TANH_RH);
-        tab[i] = make_float2(a, b - a);
-    }
+    uint4* dst = reinterpret_cast<uint4*>(sh);
+    for (int i = threadIdx.x; i < MLP_IMAGE_BYTES / 16; i += blockDim.x) dst[i] = M.image[i];
     __syncthreads();
 }
 
-__device__ inline float table_tanh(const float2* tab, float x)
+// tanh of two FP32 values as a packed BF16 pair (a in the low half).
+__device__ inline uint32_t tanh_pair(const uint16_t* tab, float a, float b)
 {
-    const float t = fminf(fmaxf(fmaf(x, TANH_RH, -TANH_X0 * TANH_RH), 0.0f), TANH_N - 0.5f);
-    const float2 e = tab[(int)t];
-    return fmaf(__builtin_amdgcn_fractf(t), e.y, e.x);
+    const half2_t x = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+    half2_t m = __builtin_elementwise_fma(x, (half2_t)(_Float16)0x1p-16f, (half2_t)(_Float16)0x1p-14f);
+    m = __builtin_elementwise_max(m, (half2_t)(_Float16)0.0f);
+    m = __builtin_elementwise_min(m, (half2_t)(_Float16)(2047 * 0x1p-24f));
+    const uint32_t bits = __builtin_bit_cast(uint32_t, m);
+    short2_t r;
+    r.x = (short)tab[bits & 0xffffu];
+    r.y = (short)tab[bits >> 16];
+    return __builtin_bit_cast(uint32_t, r);
+}
+
+// Eight accumulator values -> an MFMA operand of their tanh.
+__device__ inline bf16x8 tanh_operand(const uint16_t* tab, const float* v)
+{
+    u32x4 w;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = tanh_pair(tab, v[2 * q], v[2 * q + 1]);
+    return __builtin_bit_cast(bf16x8, w);
 }
 
 __device__ inline float safe_log2(float x)
@@ -118,76 +154,81 @@ __host__ __device__ inline int mlp_perm(int c, int g, int i) { return 16 * c + 8
 
 // Wave-level inference: every lane passes its zone's normalised features
 // and receives y = MLP(x) for its own zone.  Must be called by all 64 lanes;
-// tab is the workgroup's tanh table (tanh_table_init).
-__device__ inline float mlp_wave(const MlpDev& M, const float2* tab, const float x[4])
+// sh is the workgroup's inference image (mlp_shared_load).  The 64 zones go
+// through the network as two tiles of 32 (the MFMA N dimension), one after
+// the other, which halves the live accumulators and lets more wavefronts
+// share a SIMD.
+__device__ inline float mlp_wave(const MlpDev& M, const MlpShared* sh, const float x[4])
 {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 5;
-    // Layer 1 B operand: lanes 0..31 carry features (k = 0..3) of zone
-    // 32nt + lane; lanes 32..63 carry the zero padding k = 8..15.
-    float xp[4];
+    const floatx16 zero = {};
+    // B operand with 1 at k = 0 (layer-2 bias) for lane group 0
+    bf16x8 one;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) xp[m] = __shfl_xor(x[m], 32, 64);
-    bf16x8 bx0, bx1;
+    for (int i = 0; i < 8; ++i) one[i] = (__bf16)((g == 0 && i == 0) ? 1.0f : 0.0f);
+    float y0 = 0.0f, y1 = 0.0f;
+#pragma unroll 1
+    for (int nt = 0; nt < 2; ++nt) {
+        // Layer 1 B operand: lanes 0..31 carry the features (k = 0..3) of
+        // zone 32nt + lane and 1 at k = 4; lanes 32..63 carry k = 8..15 = 0.
+        float xs[4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        bx0[i] = (__bf16)((g == 0 && i < 4) ? x[i & 3] : 0.0f);
-        bx1[i] = (__bf16)((g == 0 && i < 4) ? xp[i & 3] : 0.0f);
-    }
-    floatx16 h[2][2];
+        for (int m = 0; m < 4; ++m) {
+            const float o = __shfl_xor(x[m], 32, 64);
+            xs[m] = nt == 0 ? x[m] : o;
+        }
+        bf16x8 bx;
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        const bf16x8 a = M.a1[mt * 64 + lane];
-        const floatx16 c = M.c1[mt * 2 + g];
-        h[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bx0, c, 0, 0, 0);
-        h[mt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bx1, c, 0, 0, 0);
-    }
-    // tanh, then the accumulators become layer 2's B operand (bf16).
-    bf16x8 b2[4][2];
+        for (int i = 0; i < 8; ++i) bx[i] = (__bf16)(g == 0 ? (i < 4 ? xs[i & 3] : (i == 4 ? 1.0f : 0.0f)) : 0.0f);
+        floatx16 h[2];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-            for (int i = 0; i < 8; ++i) b2[c][nt][i] = (__bf16)table_tanh(tab, h[c >> 1][nt][8 * (c & 1) + i]);
-    floatx16 o[2][2];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        const floatx16 cb = M.c2[mt * 2 + g];
-        o[mt][0] = cb;
-        o[mt][1] = cb;
+        for (int mt = 0; mt < 2; ++mt) h[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a1[mt * 64 + lane], bx, zero, 0, 0, 0);
+        // tanh; the accumulators become layer 2's B operand (bf16).
+        bf16x8 b2[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const bf16x8 a = M.a2[(mt * 4 + c) * 64 + lane];
-            o[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[c][0], o[mt][0], 0, 0, 0);
-            o[mt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[c][1], o[mt][1], 0, 0, 0);
-        }
-    }
-    // Layer 3: each lane sums its 32 hidden rows, the partner lane the rest.
-    float part[2] = {0.0f, 0.0f};
+            float v[8];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        const floatx16 w = M.w3[mt * 2 + g];
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            part[0] = fmaf(w[v], table_tanh(tab, o[mt][0][v]), part[0]);
-            part[1] = fmaf(w[v], table_tanh(tab, o[mt][1][v]), part[1]);
+            for (int i = 0; i < 8; ++i) v[i] = h[c >> 1][8 * (c & 1) + i];
+            b2[c] = tanh_operand(sh->tanh, v);
         }
+        floatx16 o[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            o[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a2b[mt * 64 + lane], one, zero, 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                o[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a2[(mt * 4 + c) * 64 + lane], b2[c], o[mt], 0, 0, 0);
+        }
+        // Layer 3 on the same MFMA, with W3 as row 0 of the A operand:
+        // element 0 of lanes 0..31 receives W3 . tanh(o) of zone 32nt + lane.
+        floatx16 acc = zero;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = o[c >> 1][8 * (c & 1) + i];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a3[c * 64 + lane], tanh_operand(sh->tanh, v), acc, 0, 0, 0);
+        }
+        if (nt == 0)
+            y0 = acc[0];
+        else
+            y1 = acc[0];
     }
-    part[0] += __shfl_xor(part[0], 32, 64);
-    part[1] += __shfl_xor(part[1], 32, 64);
-    return M.b3 + (g == 0 ? part[0] : part[1]);
+    const float hi = __shfl_xor(y1, 32, 64);
+    return (g == 0 ? y0 : hi) + M.b3;
 }
 
 // log2(p/D) guessed for one zone by the wave inference: y/tscale + tshift.
-__device__ inline float mlp_log2_guess_wave(const MlpDev& M, const float2* tab, double D, double S2, double tau,
+__device__ inline float mlp_log2_guess_wave(const MlpDev& M, const MlpShared* sh, double D, double S2, double tau,
                                            double gamma)
 {
     float f[4], x[4];
     mlp_raw_features(D, S2, tau, gamma, f);
 #pragma unroll
     for (int m = 0; m < 4; ++m) x[m] = (f[m] - M.norm.shift[m]) * M.norm.scale[m];
-    const float y = mlp_wave(M, tab, x);
+    const float y = mlp_wave(M, sh, x);
     return y / M.norm.tscale + M.norm.tshift;
 }
 
