@@ -170,6 +170,12 @@ __device__ inline float mlp_wave(const MlpDev& M, const MlpShared* sh, const flo
     float y0 = 0.0f, y1 = 0.0f;
 #pragma unroll 1
     for (int nt = 0; nt < 2; ++nt) {
+        // The weight fragments are read from LDS for every tile: an opaque
+        // copy of the lane index keeps the compiler from hoisting the reads
+        // out of the loop into 64 long-lived VGPRs, which would cost spills
+        // at four wavefronts per SIMD.
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
         // Layer 1 B operand: lanes 0..31 carry the features (k = 0..3) of
         // zone 32nt + lane and 1 at k = 4; lanes 32..63 carry k = 8..15 = 0.
         float xs[4];
@@ -183,7 +189,7 @@ __device__ inline float mlp_wave(const MlpDev& M, const MlpShared* sh, const flo
         for (int i = 0; i < 8; ++i) bx[i] = (__bf16)(g == 0 ? (i < 4 ? xs[i & 3] : (i == 4 ? 1.0f : 0.0f)) : 0.0f);
         floatx16 h[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) h[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a1[mt * 64 + lane], bx, zero, 0, 0, 0);
+        for (int mt = 0; mt < 2; ++mt) h[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a1[mt * 64 + ln], bx, zero, 0, 0, 0);
         // tanh; the accumulators become layer 2's B operand (bf16).
         bf16x8 b2[4];
 #pragma unroll
@@ -196,10 +202,10 @@ __device__ inline float mlp_wave(const MlpDev& M, const MlpShared* sh, const flo
         floatx16 o[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            o[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a2b[mt * 64 + lane], one, zero, 0, 0, 0);
+            o[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a2b[mt * 64 + ln], one, zero, 0, 0, 0);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                o[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a2[(mt * 4 + c) * 64 + lane], b2[c], o[mt], 0, 0, 0);
+                o[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a2[(mt * 4 + c) * 64 + ln], b2[c], o[mt], 0, 0, 0);
         }
         // Layer 3 on the same MFMA, with W3 as row 0 of the A operand:
         // element 0 of lanes 0..31 receives W3 . tanh(o) of zone 32nt + lane.
@@ -209,7 +215,7 @@ __device__ inline float mlp_wave(const MlpDev& M, const MlpShared* sh, const flo
             float v[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) v[i] = o[c >> 1][8 * (c & 1) + i];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a3[c * 64 + lane], tanh_operand(sh->tanh, v), acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sh->a3[c * 64 + ln], tanh_operand(sh->tanh, v), acc, 0, 0, 0);
         }
         if (nt == 0)
             y0 = acc[0];

@@ -68,8 +68,8 @@ def test_trained_network_reduces_newton_iterations(trained):
         P1, c1, i1 = knc.try_to_primitive(h, cons[m], guess[m])
         ok = (c0 == 0) & (c1 == 0)
         assert ok.mean() > 0.999
-        # both converge to the same root, to the Newton tolerance scale
-        # 1e-12 (tau + D + p) that bounds how well p is determined
+        # both converge to the same root: p is determined to the rounding
+        # noise of the Newton function, ~1e-16 (tau + D + p)
         Q = cons[m][ok, 3] + cons[m][ok, 0] + P0[ok, 3]
         assert np.max(np.abs(P1[ok, 3] - P0[ok, 3]) / Q) < 1e-11
         plain_it.append(i0[ok])
