@@ -110,6 +110,15 @@ std::vector<double> radial_table(int nr, const double* rv)
     return t;
 }
 
+// A device copy of an initial model (knc_model) and the mesh its ghost
+// blocks and births come from.
+struct DevModel {
+    bool on = false;
+    ModelDev dev{};
+    double* d_table = nullptr;
+    knc_mesh mesh{};
+};
+
 struct HostBlock {
     int32_t index = 0;
     int nr = 0, nq = 0;
@@ -180,6 +189,13 @@ struct knc_context {
     size_t cap_bseq = 0;        // doubles
     bool trial = false;         // autotuner trial steps: fixed rows only
 
+    // initial models evaluated on the device: per side for model boundaries,
+    // one for excision births; stage times of the fold's ghost-row sets
+    DevModel bmodel[2];
+    DevModel birth;
+    double* d_qc = nullptr;          // polar zone centres 0.5 (q0 + q1)
+    std::vector<double> stage_times; // per (fold step, stage)
+
     // moving excision surfaces
     bool exc_on = false;
     knc_excision exc{};
@@ -192,6 +208,10 @@ struct knc_context {
     int last_launches = 0;
 };
 
+int insert_block(knc_context* c, int32_t index, int32_t nr, int32_t nq, const double* rv, const double* qv,
+                 const double* conserved, const double* scalar_mass);
+int insert_block_model(knc_context* c, int32_t index, const DevModel& dm, double t);
+
 namespace {
 
 bool learned(const knc_context* c)
@@ -200,6 +220,108 @@ bool learned(const knc_context* c)
            c->h.system == KNC_SRHD;
 }
 bool two_pass(const knc_context* c) { return c->o.stage_passes == 2; }
+// A side whose ghost rows are refreshed at every stage time (host callback
+// or device model).
+bool model_side(const knc_context* c, int side) { return c->bsrc[side] || c->bmodel[side].on; }
+bool any_model_side(const knc_context* c) { return model_side(c, 0) || model_side(c, 1); }
+
+// Mesh::zone_dlogr and the vertices of block `index`
+// (Mesh::subgrid_extent, SphericalPolarGrid::vertex_coordinate_signed;
+// src/mesh.rs:341-375, 202-213).  f64::log(10.0) is ln(x)/ln(10).
+double mesh_zone_dlogr(const knc_mesh& m)
+{
+    return m.num_radial_zones > 0 ? 1.0 / (double)m.num_radial_zones : KNC_PI / (double)m.num_polar_zones;
+}
+
+void mesh_vertices(const knc_mesh& m, int32_t index, double* rv, double* qv)
+{
+    const double dlogr = mesh_zone_dlogr(m);
+    const double b = (double)m.block_size * dlogr;
+    const double r0 = m.reference_radius * std::pow(1.0 + b, (double)index);
+    const double r1 = m.reference_radius * std::pow(1.0 + b, (double)index + 1.0);
+    const double q0 = m.num_polar_zones == 1 ? KNC_PI * 0.5 - dlogr : 0.0;
+    const double q1 = m.num_polar_zones == 1 ? KNC_PI * 0.5 + dlogr : KNC_PI;
+    const double y0 = std::log(r0) / std::log(10.0), y1 = std::log(r1) / std::log(10.0);
+    const double dy = (y1 - y0) / (double)m.block_size;
+    const double dq = (q1 - q0) / (double)m.num_polar_zones;
+    if (rv)
+        for (int i = 0; i <= m.block_size; ++i) rv[i] = std::pow(10.0, y0 + dy * (double)i);
+    if (qv)
+        for (int j = 0; j <= m.num_polar_zones; ++j) qv[j] = q0 + dq * (double)j;
+}
+
+int check_mesh(const knc_mesh* m)
+{
+    if (!m) return fail(KNC_ERR_INVALID_ARGUMENT, "mesh is NULL");
+    if (!(m->reference_radius > 0.0)) return fail(KNC_ERR_INVALID_ARGUMENT, "reference_radius must be positive");
+    if (m->block_size < 2) return fail(KNC_ERR_INVALID_ARGUMENT, "must have at least 2 radial zones per block");
+    if (m->block_size >= 65536) return fail(KNC_ERR_INVALID_ARGUMENT, "block_size must be below 65536");
+    if (m->num_polar_zones < 1 || m->num_polar_zones > MAX_NQ)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "num_polar_zones must be in [1, %d]", MAX_NQ);
+    if (m->num_radial_zones < 0) return fail(KNC_ERR_INVALID_ARGUMENT, "num_radial_zones must be >= 0");
+    if (m->num_polar_zones == 1 && m->num_radial_zones == 0)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "num_radial_zones is not optional when num_polar_zones=1");
+    return KNC_OK;
+}
+
+// The two ghost rows next to the live mesh: rows nr-2, nr-1 of block
+// inner-1 (side 0) or rows 0, 1 of block outer+1 (side 1), with the
+// radial factors of the block tables.
+GhostRows ghost_rows(const knc_mesh& m, int side, int32_t inner, int32_t outer)
+{
+    std::vector<double> rv((size_t)m.block_size + 1);
+    mesh_vertices(m, side ? outer + 1 : inner - 1, rv.data(), nullptr);
+    const int i0 = side ? 0 : m.block_size - 2;
+    GhostRows g;
+    for (int i = 0; i < 2; ++i) {
+        const double r0 = rv[i0 + i], r1 = rv[i0 + i + 1];
+        g.rc[i] = std::sqrt(r0 * r1);
+        g.volA[i] = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
+        g.rvolA[i] = 1.0 / g.volA[i];
+    }
+    return g;
+}
+
+void free_model(DevModel& dm)
+{
+    if (dm.d_table) (void)hipFree(dm.d_table);
+    dm = DevModel{};
+}
+
+// Validates and uploads a model; the table rows must strictly increase in
+// their first column (LookupTable::from_rows, src/lookup_table_v2.rs:35-48).
+int load_model(DevModel& dm, const knc_mesh* mesh, const knc_model* m)
+{
+    if (!m) return fail(KNC_ERR_INVALID_ARGUMENT, "model is NULL");
+    if (m->kind < KNC_MODEL_KINETIC_BOMB || m->kind > KNC_MODEL_WIND_SHOCK)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "unknown model kind %d", m->kind);
+    int rows = m->kind == KNC_MODEL_WIND_SHOCK ? m->table_rows : 0;
+    if (rows < 0) return fail(KNC_ERR_INVALID_ARGUMENT, "table_rows must be >= 0");
+    if (rows > 0) {
+        if (!m->table) return fail(KNC_ERR_INVALID_ARGUMENT, "table is NULL");
+        if (rows < 2) return fail(KNC_ERR_INVALID_ARGUMENT, "the table must have at least two rows");
+        for (int k = 1; k < rows; ++k)
+            if (!(m->table[4 * k] > m->table[4 * (k - 1)]))
+                return fail(KNC_ERR_INVALID_ARGUMENT, "the left-most column of the table must increase monotonically");
+    }
+    DevModel nd;
+    nd.on = true;
+    nd.dev.kind = m->kind;
+    nd.dev.table_rows = rows;
+    for (int k = 0; k < MODEL_NPARAM; ++k) nd.dev.p[k] = m->params[k];
+    if (mesh) nd.mesh = *mesh;
+    if (rows > 0) {
+        HIP_TRY(hipMalloc(&nd.d_table, (size_t)4 * rows * sizeof(double)));
+        if (hipMemcpy(nd.d_table, m->table, (size_t)4 * rows * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(nd.d_table);
+            return fail(KNC_ERR_HIP, "hipMemcpy failed");
+        }
+    }
+    nd.dev.table = nd.d_table;
+    free_model(dm);
+    dm = nd;
+    return KNC_OK;
+}
 
 void drop_graphs(knc_context* c)
 {
@@ -255,6 +377,11 @@ int upload_theta(knc_context* c, int nq, const double* qv)
     c->th.cot = c->d_theta + 4 * m;
     c->th.dq = c->d_theta + 5 * m;
     c->th.nq = nq;
+    std::vector<double> qc(nq);
+    for (int j = 0; j < nq; ++j) qc[j] = 0.5 * (qv[j] + qv[j + 1]);
+    if (c->d_qc) HIP_TRY(hipFree(c->d_qc));
+    HIP_TRY(hipMalloc(&c->d_qc, nq * sizeof(double)));
+    HIP_TRY(hipMemcpy(c->d_qc, qc.data(), nq * sizeof(double), hipMemcpyHostToDevice));
     if (c->d_bnd) HIP_TRY(hipFree(c->d_bnd));
     HIP_TRY(hipMalloc(&c->d_bnd, (size_t)2 * 5 * 2 * nq * sizeof(double)));
     HIP_TRY(hipMemset(c->d_bnd, 0, (size_t)2 * 5 * 2 * nq * sizeof(double)));
@@ -457,11 +584,11 @@ void stage_boundaries(knc_context* c, StageArgs& a, int fold_step, int stage)
     if (c->trial) return;
     const size_t set = (size_t)10 * c->nq;
     const size_t base = ((size_t)fold_step * c->h.runge_kutta_order + stage) * 2 * set;
-    if (c->bsrc[0]) {
+    if (model_side(c, 0)) {
         a.bnd.inner_mode = BND_FIXED;
         a.bnd.inner_rows = c->d_bseq + base;
     }
-    if (c->bsrc[1]) {
+    if (model_side(c, 1)) {
         a.bnd.outer_mode = BND_FIXED;
         a.bnd.outer_rows = c->d_bseq + base + set;
     }
@@ -603,7 +730,7 @@ int enqueue_steps(knc_context* c, int n, int first)
         return KNC_OK;
     }
     // the ghost-row pointers of model boundaries depend on the fold step
-    const bool model = c->bsrc[0] || c->bsrc[1];
+    const bool model = any_model_side(c);
     auto key = std::make_tuple(start_slot, n, model ? first : 0);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
@@ -653,7 +780,7 @@ int excise(knc_context* c, double t, bool apply, bool* change)
     const int32_t inner_index = bi.index, outer_index = bo.index;
     const bool remove = c->blocks.size() > 1 &&
                         bi.rv[bi.nr] < excision_surface(c->exc.inner_radius, c->exc.inner_speed, c->exc.delay, t);
-    const bool add = c->exc.source != nullptr &&
+    const bool add = (c->exc.source != nullptr || c->birth.on) &&
                      bo.rv[bo.nr] < excision_surface(c->exc.outer_radius, c->exc.outer_speed, c->exc.delay, t);
     *change = remove || add;
     if (!apply) return KNC_OK;
@@ -662,7 +789,10 @@ int excise(knc_context* c, double t, bool apply, bool* change)
         if ((rc = knc_block_remove(c, inner_index))) return rc;
         c->exc_removed += 1;
     }
-    if (add) {
+    if (add && !c->exc.source) {
+        if ((rc = insert_block_model(c, outer_index + 1, c->birth, t))) return rc;
+        c->exc_added += 1;
+    } else if (add) {
         knc_block_data d{};
         const int32_t idx = outer_index + 1;
         const int32_t src_rc = c->exc.source(c->exc.user, idx, t, &d);
@@ -704,6 +834,7 @@ int fill_boundary_sets(knc_context* c, int s, double t0, double dt)
     const double times[3] = {t0, t1, (t1 + dt) * (-0.75 + 1.0) + t0 * 0.75};
     const int nq = c->nq;
     const int32_t inner = c->blocks.begin()->first, outer = c->blocks.rbegin()->first;
+    for (int st = 0; st < order; ++st) c->stage_times[(size_t)s * order + st] = times[st];
     std::vector<double> prim((size_t)8 * nq), scal((size_t)2 * nq);
     for (int st = 0; st < order; ++st)
         for (int side = 0; side < 2; ++side) {
@@ -732,6 +863,25 @@ int flush_steps(knc_context* c, int first, int n)
         HIP_TRY(hipMemcpyAsync(c->d_bseq + first * per_step, c->h_bseq + first * per_step,
                                n * per_step * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
+    if (c->bmodel[0].on || c->bmodel[1].on) {
+        // device-model sides, written after the host sets they share the
+        // buffer with; the block set is fixed within a segment
+        const int order = c->h.runge_kutta_order;
+        const int32_t inner = c->blocks.begin()->first, outer = c->blocks.rbegin()->first;
+        GhostRows g[2];
+        for (int side = 0; side < 2; ++side)
+            if (c->bmodel[side].on) g[side] = ghost_rows(c->bmodel[side].mesh, side, inner, outer);
+        for (int s = first; s < first + n; ++s)
+            for (int st = 0; st < order; ++st)
+                for (int side = 0; side < 2; ++side) {
+                    if (!c->bmodel[side].on) continue;
+                    double* out = c->d_bseq + (((size_t)s * order + st) * 2 + side) * 10 * c->nq;
+                    const int tag0 = (side ? GHOST_OUTER_ORDER : GHOST_INNER_ORDER) << 16;
+                    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g[side], c->d_qc,
+                                              c->th.dcost, c->th.rdcost, c->stage_times[(size_t)s * order + st],
+                                              s - first + 1, tag0, out, c->d_err, c->stream));
+                }
+    }
     return enqueue_steps(c, n, first);
 }
 
@@ -746,8 +896,9 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
     if ((rc = require_blocks(c))) return rc;
     c->last_launches = 0;
     const bool moving = with_dt && excision_moving(c);
-    const bool model = c->bsrc[0] || c->bsrc[1];
+    const bool model = any_model_side(c);
     if (model && (rc = ensure_boundary_sets(c, fold))) return rc;
+    if (model) c->stage_times.assign((size_t)fold * c->h.runge_kutta_order, 0.0);
     hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
     HIP_TRY(hipGetLastError());
     if (with_dt) {
@@ -1029,8 +1180,11 @@ void knc_context_destroy(knc_context* c)
         (void)hipFree(kv.second.guess);
         (void)hipFree(kv.second.prim);
     }
+    free_model(c->bmodel[0]);
+    free_model(c->bmodel[1]);
+    free_model(c->birth);
     void* ptrs[] = {c->d_theta, c->d_bnd, c->d_blocks, c->d_items, c->d_nitems, c->d_err, c->d_stats,
-                    c->d_dt_partial, c->d_dt, c->d_mlp, c->d_bseq};
+                    c->d_dt_partial, c->d_dt, c->d_mlp, c->d_bseq, c->d_qc};
     if (c->h_bseq) (void)hipHostFree(c->h_bseq);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1044,6 +1198,14 @@ int knc_block_insert(knc_context* c, int32_t index, int32_t nr, int32_t nq, cons
                      const double* conserved, const double* scalar_mass)
 {
     if (!c || !rv || !qv || !conserved || !scalar_mass) return fail(KNC_ERR_INVALID_ARGUMENT, "NULL argument");
+    return insert_block(c, index, nr, nq, rv, qv, conserved, scalar_mass);
+}
+
+// knc_block_insert; with conserved NULL the block starts at zero in every
+// slot (the caller fills slot c->cur on the device).
+extern "C++" int insert_block(knc_context* c, int32_t index, int32_t nr, int32_t nq, const double* rv, const double* qv,
+                 const double* conserved, const double* scalar_mass)
+{
     if (nr < 2) return fail(KNC_ERR_INVALID_ARGUMENT, "must have at least 2 radial zones per block");
     if (nr >= 65536) return fail(KNC_ERR_INVALID_ARGUMENT, "nr must be below 65536");
     if (nq < 1 || nq > MAX_NQ) return fail(KNC_ERR_INVALID_ARGUMENT, "nq must be in [1, %d]", MAX_NQ);
@@ -1083,17 +1245,20 @@ int knc_block_insert(knc_context* c, int32_t index, int32_t nr, int32_t nq, cons
         (void)hipFree(b.guess);
         return fail(KNC_ERR_HIP, "hipMalloc failed");
     }
-    std::vector<double> soa(5 * n);
-    for (size_t k = 0; k < n; ++k) {
-        soa[k] = conserved[4 * k];
-        soa[n + k] = conserved[4 * k + 1];
-        soa[2 * n + k] = conserved[4 * k + 2];
-        soa[3 * n + k] = conserved[4 * k + 3];
-        soa[4 * n + k] = scalar_mass[k];
-    }
     HIP_TRY(hipMemcpy(b.rgeo, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(b.mem, 0, (size_t)NSLOT * 5 * n * sizeof(double)));
-    HIP_TRY(hipMemcpy(b.mem + (size_t)c->cur * 5 * n, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (conserved) {
+        std::vector<double> soa(5 * n);
+        for (size_t k = 0; k < n; ++k) {
+            soa[k] = conserved[4 * k];
+            soa[n + k] = conserved[4 * k + 1];
+            soa[2 * n + k] = conserved[4 * k + 2];
+            soa[3 * n + k] = conserved[4 * k + 3];
+            soa[4 * n + k] = scalar_mass[k];
+        }
+        HIP_TRY(hipMemcpy(b.mem + (size_t)c->cur * 5 * n, soa.data(), soa.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+    }
     c->blocks.emplace(index, b);
     c->dirty = true;
     return KNC_OK;
@@ -1221,8 +1386,149 @@ int knc_set_boundary_source(knc_context* c, int32_t side, knc_boundary_fn fn, vo
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->bsrc[side] = fn;
     c->buser[side] = fn ? user : nullptr;
+    if (fn) free_model(c->bmodel[side]);
     drop_graphs(c);
     return KNC_OK;
+}
+
+int knc_mesh_block_vertices(const knc_mesh* mesh, int32_t index, double* rv, double* qv)
+{
+    int rc;
+    if ((rc = check_mesh(mesh))) return rc;
+    if (!rv || !qv) return fail(KNC_ERR_INVALID_ARGUMENT, "NULL argument");
+    mesh_vertices(*mesh, index, rv, qv);
+    return KNC_OK;
+}
+
+int knc_model_eval(knc_context* c, const knc_model* model, int64_t n, const double* r, const double* q, double t,
+                   double* prim, double* scal)
+{
+    if (!c || !model || (n > 0 && (!r || !q || !prim || !scal))) return fail(KNC_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n < 0) return fail(KNC_ERR_INVALID_ARGUMENT, "n must be >= 0");
+    if (n == 0) return KNC_OK;
+    DevModel dm;
+    int rc;
+    if ((rc = load_model(dm, nullptr, model))) return rc;
+    double* d = nullptr;
+    int* dfail = nullptr;
+    const size_t bytes = (size_t)n * 7 * sizeof(double);
+    if (hipMalloc(&d, bytes) != hipSuccess || hipMalloc(&dfail, 2 * sizeof(int)) != hipSuccess) {
+        if (d) (void)hipFree(d);
+        free_model(dm);
+        return fail(KNC_ERR_HIP, "hipMalloc failed");
+    }
+    int hfail[2] = {0, 0};
+    hipError_t e = hipMemcpy(d, r, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, q, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dfail, 0, 2 * sizeof(int));
+    if (e == hipSuccess) e = launch_model_eval(dm.dev, n, d, d + n, t, d + 2 * n, d + 6 * n, dfail, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(prim, d + 2 * n, (size_t)n * 4 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(scal, d + 6 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hfail, dfail, 2 * sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    (void)hipFree(dfail);
+    free_model(dm);
+    if (e != hipSuccess) return fail(KNC_ERR_HIP, "HIP error: %s", hipGetErrorString(e));
+    if (hfail[0]) return fail(KNC_ERR_MODEL, "the model cannot be evaluated at point %d (the reference panics)", hfail[1]);
+    return KNC_OK;
+}
+
+// BlockState::from_model of mesh block `index` at time t, inserted as a new
+// block (src/state.rs:55-69 on the vertices of src/mesh.rs:202-264).
+extern "C++" int insert_block_model(knc_context* c, int32_t index, const DevModel& dm, double t)
+{
+    const knc_mesh& m = dm.mesh;
+    const int nr = m.block_size, nq = m.num_polar_zones;
+    std::vector<double> rv((size_t)nr + 1), qv((size_t)nq + 1);
+    mesh_vertices(m, index, rv.data(), qv.data());
+    int rc;
+    if ((rc = insert_block(c, index, nr, nq, rv.data(), qv.data(), nullptr, nullptr))) return rc;
+    const HostBlock& b = c->blocks.at(index);
+    std::vector<double> tab((size_t)2 * nr);
+    for (int i = 0; i < nr; ++i) {
+        const double r0 = rv[i], r1 = rv[i + 1];
+        tab[i] = std::sqrt(r0 * r1);
+        tab[nr + i] = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
+    }
+    double* d = nullptr;
+    int* dfail = nullptr;
+    if (hipMalloc(&d, tab.size() * sizeof(double)) != hipSuccess || hipMalloc(&dfail, 2 * sizeof(int)) != hipSuccess) {
+        if (d) (void)hipFree(d);
+        (void)knc_block_remove(c, index);
+        return fail(KNC_ERR_HIP, "hipMalloc failed");
+    }
+    int hfail[2] = {0, 0};
+    const size_t n = (size_t)nr * nq;
+    hipError_t e = hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dfail, 0, 2 * sizeof(int));
+    if (e == hipSuccess)
+        e = launch_block_from_model(c->h.system, dm.dev, c->zp, nr, nq, d, d + nr, c->d_qc, c->th.dcost, t,
+                                    b.mem + (size_t)c->cur * 5 * n, dfail, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(hfail, dfail, 2 * sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    (void)hipFree(dfail);
+    if (e != hipSuccess) return fail(KNC_ERR_HIP, "HIP error: %s", hipGetErrorString(e));
+    if (hfail[0]) {
+        (void)knc_block_remove(c, index);
+        return fail(KNC_ERR_MODEL, "the model cannot be evaluated in block %d zone %d (the reference panics)", index,
+                    hfail[1]);
+    }
+    return KNC_OK;
+}
+
+int knc_block_insert_model(knc_context* c, int32_t index, const knc_mesh* mesh, const knc_model* model, double t)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    int rc;
+    if ((rc = check_mesh(mesh))) return rc;
+    DevModel dm;
+    if ((rc = load_model(dm, mesh, model))) return rc;
+    rc = insert_block_model(c, index, dm, t);
+    free_model(dm);
+    return rc;
+}
+
+// The mesh's polar grid must be the blocks' (every block shares it).
+static int check_mesh_polar(const knc_context* c, const knc_mesh& m)
+{
+    if (c->blocks.empty()) return KNC_OK;
+    std::vector<double> qv((size_t)m.num_polar_zones + 1);
+    mesh_vertices(m, c->blocks.begin()->first, nullptr, qv.data());
+    if (m.num_polar_zones != c->nq || std::memcmp(qv.data(), c->qv.data(), qv.size() * sizeof(double)) != 0)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "the mesh's polar grid differs from the blocks'");
+    return KNC_OK;
+}
+
+int knc_set_boundary_model(knc_context* c, int32_t side, const knc_mesh* mesh, const knc_model* model)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (side != KNC_SIDE_INNER && side != KNC_SIDE_OUTER) return fail(KNC_ERR_INVALID_ARGUMENT, "bad side");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    if (!model) {
+        free_model(c->bmodel[side]);
+        return KNC_OK;
+    }
+    int rc;
+    if ((rc = check_mesh(mesh)) || (rc = check_mesh_polar(c, *mesh))) return rc;
+    if ((rc = load_model(c->bmodel[side], mesh, model))) return rc;
+    c->bsrc[side] = nullptr;
+    c->buser[side] = nullptr;
+    return KNC_OK;
+}
+
+int knc_set_birth_model(knc_context* c, const knc_mesh* mesh, const knc_model* model)
+{
+    if (!c) return fail(KNC_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!model) {
+        free_model(c->birth);
+        return KNC_OK;
+    }
+    int rc;
+    if ((rc = check_mesh(mesh)) || (rc = check_mesh_polar(c, *mesh))) return rc;
+    return load_model(c->birth, mesh, model);
 }
 
 int knc_set_excision(knc_context* c, const knc_excision* ex)

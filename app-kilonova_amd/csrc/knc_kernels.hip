@@ -50,23 +50,6 @@ __device__ inline void gstf(float* p, int64_t k, float v)
     ((__attribute__((address_space(1))) float*)p)[k] = v;
 }
 
-// tag = (block order << 16) | row; the key orders failures by (step, block,
-// row, column) and the host maps the block order back to the block index.
-__device__ inline void record_error(ErrLog* log, int step, int tag, int col, int code, double value, const Cons& u)
-{
-    const unsigned long long key = ((unsigned long long)step << 48) | ((unsigned long long)(unsigned)tag << 16) |
-                                   (unsigned long long)col;
-    atomicMin(&log->min_key, key);
-    const unsigned int slot = atomicAdd(&log->count, 1u);
-    if (slot < (unsigned int)ERR_LOG_CAP) {
-        ErrEntry& e = log->entry[slot];
-        e.key = key;
-        e.code = code;
-        e.value = value;
-        e.cons[0] = u.d; e.cons[1] = u.sr; e.cons[2] = u.sq; e.cons[3] = u.tau;
-    }
-}
-
 // c2p of one zone (BlockState::try_to_primitive, src/state.rs:76-94, and the
 // scalar concentration s = M/dV/Gamma of src/scheme.rs:35).  u is the
 // volume-integrated state; rdv the reciprocal volume factor.

@@ -6,6 +6,7 @@
 
 #include "knc_device.hpp"
 #include "knc_mlp.hpp"
+#include "knc_model.hpp"
 #include "knc_zone.hpp"
 
 namespace knc {
@@ -31,6 +32,36 @@ struct StageArgs {
     int32_t step;                // error-key step number (0 = dt pass)
     int32_t rows;                // rows per work item (row-parallel passes: k_prim, k_guess, k_dt)
     MlpDev mlp;                  // learned c2p guess (learned-guess kernels only)
+};
+
+// One failed zone into the error log.  tag = (block order << 16) | row, or
+// (GHOST_INNER_ORDER / GHOST_OUTER_ORDER << 16) | row for the model ghost
+// rows; the key orders failures by (step, block, row, column) and the host
+// maps the block order back to the block index.
+__device__ inline void record_error(ErrLog* log, int step, int tag, int col, int code, double value, const Cons& u)
+{
+    const unsigned long long key = ((unsigned long long)step << 48) | ((unsigned long long)(unsigned)tag << 16) |
+                                   (unsigned long long)col;
+    atomicMin(&log->min_key, key);
+    const unsigned int slot = atomicAdd(&log->count, 1u);
+    if (slot < (unsigned int)ERR_LOG_CAP) {
+        ErrEntry& e = log->entry[slot];
+        e.key = key;
+        e.code = code;
+        e.value = value;
+        e.cons[0] = u.d; e.cons[1] = u.sr; e.cons[2] = u.sq; e.cons[3] = u.tau;
+    }
+}
+
+constexpr int GHOST_INNER_ORDER = 0x7ffe;
+constexpr int GHOST_OUTER_ORDER = 0x7fff;
+
+// Geometry of the two ghost rows of a model boundary: centre radius
+// sqrt(r0 r1), the radial volume factor 2 pi (r1^3 - r0^3)/3 and its inverse.
+struct GhostRows {
+    double rc[2];
+    double volA[2];
+    double rvolA[2];
 };
 
 // One stage: fused (c2p inside the stage kernel) or, with prims, the flux
@@ -68,6 +99,16 @@ hipError_t launch_max_signal_speed(int system, const ZoneParams& zp, int64_t n, 
                                    hipStream_t s);
 hipError_t launch_plm(double th, int64_t n, const double* a, const double* b, const double* c, double* out,
                       hipStream_t s);
+// Initial models (knc_models.hip).  fail[0..1] receives the first failure
+// (KNC_ERR_MODEL, zone index) of a point or block evaluation.
+hipError_t launch_model_eval(const ModelDev& m, int64_t n, const double* r, const double* q, double t, double* prim,
+                             double* scal, int* fail, hipStream_t s);
+hipError_t launch_block_from_model(int system, const ModelDev& m, const ZoneParams& zp, int nr, int nq,
+                                   const double* rc, const double* volA, const double* qc, const double* dcost,
+                                   double t, double* planes, int* fail, hipStream_t s);
+hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
+                             const double* qc, const double* dcost, const double* rdcost, double t, int step,
+                             int tag0, double* out, ErrLog* err, hipStream_t s);
 hipError_t launch_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
                              double* mad, hipStream_t s);
 

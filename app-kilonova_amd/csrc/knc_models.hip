@@ -1,0 +1,164 @@
+// knc_models.hip -- device kernels of the initial models: point evaluation,
+// BlockState::from_model for a whole block (src/state.rs:55-69) and the
+// model ghost rows of a stage (src/scheme.rs:30-52).
+#include <hip/hip_runtime.h>
+
+#include "../../include/knc_gpu.h"
+#include "knc_device.hpp"
+#include "knc_kernels.hpp"
+#include "knc_model.hpp"
+#include "knc_zone.hpp"
+
+namespace knc {
+
+namespace {
+
+// First failure of a model kernel: fail[0] = code, fail[1] = zone index.
+__device__ inline void note_failure(int* fail, int code, int zone)
+{
+    if (atomicCAS(&fail[0], 0, code) == 0) fail[1] = zone;
+}
+
+__global__ void k_model_eval(ModelDev m, int64_t n, const double* r, const double* q, double t, double* prim,
+                             double* scal, int* fail)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        Prim P{0.0, 0.0, 0.0, 0.0};
+        double X = 0.0;
+        if (!model_at(m, r[k], q[k], t, &P, &X)) note_failure(fail, KNC_ERR_MODEL, (int)k);
+        prim[4 * k] = P.rho;
+        prim[4 * k + 1] = P.ur;
+        prim[4 * k + 2] = P.uq;
+        prim[4 * k + 3] = P.p;
+        scal[k] = X;
+    }
+}
+
+// from_model: conserved = to_conserved(P) * dV, scalar_mass =
+// lab_frame_mass(conserved) * X, written as the five planes of a block slot.
+template <int SYS>
+__global__ void k_block_from_model(ModelDev m, ZoneParams zp, int nr, int nq, const double* rc, const double* volA,
+                                   const double* qc, const double* dcost, double t, double* planes, int* fail)
+{
+    const int64_t n = (int64_t)nr * nq;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(k / nq), j = (int)(k % nq);
+        Prim P{0.0, 0.0, 0.0, 0.0};
+        double X = 0.0;
+        if (!model_at(m, rc[i], qc[j], t, &P, &X)) note_failure(fail, KNC_ERR_MODEL, (int)k);
+        const Cons u = p2c<SYS>(P, zp);
+        const double dv = volA[i] * dcost[j];
+        const double d = u.d * dv;
+        planes[k] = d;
+        planes[n + k] = u.sr * dv;
+        planes[2 * n + k] = u.sq * dv;
+        planes[3 * n + k] = u.tau * dv;
+        planes[4 * n + k] = d * X;
+    }
+}
+
+// Ghost rows of one side at one stage time: from_model on the two rows next
+// to the mesh edge, then try_to_primitive of U/dV (as U * rvolA * rdcost,
+// the stage's own convention) and s = M/dV/Gamma, written as the boundary
+// row set (5 planes x 2nq, radial order).  A c2p failure is logged like a
+// stage failure, under the ghost block's tag.
+template <int SYS>
+__global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, const double* qc, const double* dcost,
+                             const double* rdcost, double t, int step, int tag0, double* out, ErrLog* err)
+{
+    const int n2 = 2 * nq;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
+        const int i = k / nq, j = k % nq;
+        Prim P{0.0, 0.0, 0.0, 0.0};
+        double X = 0.0;
+        const bool ok = model_at(m, g.rc[i], qc[j], t, &P, &X);
+        const Cons c = p2c<SYS>(P, zp);
+        const double dv = g.volA[i] * dcost[j];
+        const double ud = c.d * dv, usr = c.sr * dv, usq = c.sq * dv, utau = c.tau * dv;
+        const double um = ud * X;
+        const double rdv = g.rvolA[i] * rdcost[j];
+        const Cons u{ud * rdv, usr * rdv, usq * rdv, utau * rdv};
+        Prim Q{0.0, 0.0, 0.0, 0.0};
+        double rW = 1.0, value = u.d;
+        int code = ok ? E_OK : KNC_ERR_MODEL;
+        if (code == E_OK) {
+            if (SYS == SYS_EULER) {
+                code = c2p_euler(u, zp, &Q);
+            } else if (u.d < 0.0) {
+                code = E_NEGATIVE_DENSITY;
+            } else if (u.tau < 0.0) {
+                code = E_NEGATIVE_ENERGY_DENSITY;
+                value = u.tau;
+            } else {
+                const double S2 = u.sr * u.sr + u.sq * u.sq;
+                double p = 0.0;
+                int it = 0;
+                code = c2p_srhd_newton(u.d, S2, u.tau, zp, 0.0, &p, &it);
+                if (code == E_OK) {
+                    code = c2p_srhd_finish(u, S2, p, &Q, &rW);
+                    if (code == E_NEGATIVE_PRESSURE) code = E_OK;
+                } else {
+                    value = u.tau;
+                }
+            }
+        }
+        double s = 0.0;
+        if (code != E_OK) {
+            record_error(err, step, tag0 | i, j, code, value, u);
+            Q = Prim{0.0, 0.0, 0.0, 0.0};
+        } else {
+            s = (um * rdv) * rW;
+        }
+        out[k] = Q.rho;
+        out[n2 + k] = Q.ur;
+        out[2 * n2 + k] = Q.uq;
+        out[3 * n2 + k] = Q.p;
+        out[4 * n2 + k] = s;
+    }
+}
+
+int grid_of(int64_t n, int block)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g > 4096) g = 4096;
+    return g < 1 ? 1 : (int)g;
+}
+
+}  // namespace
+
+hipError_t launch_model_eval(const ModelDev& m, int64_t n, const double* r, const double* q, double t, double* prim,
+                             double* scal, int* fail, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_model_eval, dim3(grid_of(n, 256)), dim3(256), 0, s, m, n, r, q, t, prim, scal, fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_from_model(int system, const ModelDev& m, const ZoneParams& zp, int nr, int nq,
+                                   const double* rc, const double* volA, const double* qc, const double* dcost,
+                                   double t, double* planes, int* fail, hipStream_t s)
+{
+    const dim3 g(grid_of((int64_t)nr * nq, 256)), b(256);
+    if (system == SYS_SRHD)
+        hipLaunchKernelGGL(k_block_from_model<SYS_SRHD>, g, b, 0, s, m, zp, nr, nq, rc, volA, qc, dcost, t, planes,
+                           fail);
+    else
+        hipLaunchKernelGGL(k_block_from_model<SYS_EULER>, g, b, 0, s, m, zp, nr, nq, rc, volA, qc, dcost, t, planes,
+                           fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
+                             const double* qc, const double* dcost, const double* rdcost, double t, int step,
+                             int tag0, double* out, ErrLog* err, hipStream_t s)
+{
+    const dim3 g(grid_of(2 * (int64_t)nq, 256)), b(256);
+    if (system == SYS_SRHD)
+        hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, rdcost, t, step, tag0, out,
+                           err);
+    else
+        hipLaunchKernelGGL(k_model_rows<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, rdcost, t, step, tag0,
+                           out, err);
+    return hipGetLastError();
+}
+
+}  // namespace knc

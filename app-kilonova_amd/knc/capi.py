@@ -32,6 +32,11 @@ GUESS_PLAIN, GUESS_LEARNED, GUESS_LEARNED_STEP = 0, 1, 2
 
 ERROR_KINDS = {1: "NegativeDensity", 2: "NegativePressure", 3: "NegativeEnergyDensity",
                4: "RootFinderFailed"}
+ERR_MODEL = 14   # an initial model evaluated where the reference panics
+
+# initial models on the device (src/models/*.rs); params in struct field order
+KINETIC_BOMB, JET_IN_CLOUD, JET_IN_STAR, HALO_KILONOVA, WIND_SHOCK = 1, 2, 3, 4, 5
+MODEL_NPARAM = 12
 
 
 class Hydro(ctypes.Structure):
@@ -78,6 +83,44 @@ class TuneResult(ctypes.Structure):
     _fields_ = [("stage_threads", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
                 ("stage_passes", ctypes.c_int32), ("candidates", ctypes.c_int32), ("from_cache", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("ms_per_stage", ctypes.c_double), ("default_ms_per_stage", ctypes.c_double)]
+
+
+class Model(ctypes.Structure):
+    """knc_model: kind, wind_shock table rows, parameters, table pointer."""
+    _fields_ = [("kind", ctypes.c_int32), ("table_rows", ctypes.c_int32),
+                ("params", ctypes.c_double * MODEL_NPARAM), ("table", ctypes.c_void_p)]
+
+
+class Mesh(ctypes.Structure):
+    """knc_mesh: the reference's Mesh fields that fix block geometry
+    (src/mesh.rs:84-113); num_radial_zones = 0 stands for None."""
+    _fields_ = [("reference_radius", ctypes.c_double), ("num_radial_zones", ctypes.c_int32),
+                ("num_polar_zones", ctypes.c_int32), ("block_size", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+def make_model(kind, params, table=None):
+    """A Model and the array that keeps its table alive."""
+    m = Model()
+    m.kind = int(kind)
+    vals = list(params)
+    if len(vals) > MODEL_NPARAM:
+        raise ValueError("too many model parameters")
+    for k, v in enumerate(vals):
+        m.params[k] = float(v)
+    tab = None
+    if table is not None:
+        tab = _f64(table).reshape(-1, 4)
+        m.table_rows = tab.shape[0]
+        m.table = tab.ctypes.data
+    return m, tab
+
+
+def mesh_block_vertices(mesh, index):
+    """Radial and polar vertices of block `index` of a Mesh."""
+    rv = np.empty(mesh.block_size + 1)
+    qv = np.empty(mesh.num_polar_zones + 1)
+    _check(lib().knc_mesh_block_vertices(ctypes.byref(mesh), int(index), _p(rv), _p(qv)))
+    return rv, qv
 
 
 class C2PStats(ctypes.Structure):
@@ -155,6 +198,11 @@ def lib():
     L.knc_hydro_max_signal_speed.argtypes = [ctypes.POINTER(Hydro), I64, P, P]
     L.knc_hydro_plm_gradient.argtypes = [D, I64, P, P, P, P]
     L.knc_fp64_probe.argtypes = [I64, P, P, P, P, P, P]
+    L.knc_mesh_block_vertices.argtypes = [ctypes.POINTER(Mesh), I32, P, P]
+    L.knc_model_eval.argtypes = [P, ctypes.POINTER(Model), I64, P, P, D, P, P]
+    L.knc_block_insert_model.argtypes = [P, I32, ctypes.POINTER(Mesh), ctypes.POINTER(Model), D]
+    L.knc_set_boundary_model.argtypes = [P, I32, ctypes.POINTER(Mesh), ctypes.POINTER(Model)]
+    L.knc_set_birth_model.argtypes = [P, ctypes.POINTER(Mesh), ctypes.POINTER(Model)]
     _lib = L
     return L
 
@@ -209,6 +257,37 @@ class Context:
         _check(lib().knc_block_insert(self._h, int(index), nr, nq, _p(rv), _p(qv), _p(cons), _p(sm)))
         self.shapes[int(index)] = (nr, nq)
         self.vertices[int(index)] = (rv.copy(), qv.copy())
+
+    def insert_block_model(self, index, mesh, kind, params, time, table=None):
+        """Block `index` of `mesh` from the initial model on the device
+        (BlockState::from_model, src/state.rs:55-69)."""
+        m, _keep = make_model(kind, params, table)
+        _check(lib().knc_block_insert_model(self._h, int(index), ctypes.byref(mesh), ctypes.byref(m), float(time)))
+        rv, qv = mesh_block_vertices(mesh, index)
+        self.shapes[int(index)] = (rv.size - 1, qv.size - 1)
+        self.vertices[int(index)] = (rv, qv)
+
+    def set_boundary_model(self, side, mesh, kind=None, params=(), table=None):
+        """Ghost rows of `side` from the initial model on the device at every
+        stage time (src/scheme.rs:30-52); kind None returns the side to fixed
+        or outflow rows."""
+        if kind is None:
+            _check(lib().knc_set_boundary_model(self._h, int(side), None, None))
+            return
+        m, _keep = make_model(kind, params, table)
+        _check(lib().knc_set_boundary_model(self._h, int(side), ctypes.byref(mesh), ctypes.byref(m)))
+
+    def set_birth_model(self, mesh, kind=None, params=(), table=None):
+        """Blocks born by excision built from the initial model on the device
+        (src/scheme.rs:197-205); kind None clears it."""
+        if kind is None:
+            _check(lib().knc_set_birth_model(self._h, None, None))
+            self._birth_mesh = None
+            return
+        m, _keep = make_model(kind, params, table)
+        _check(lib().knc_set_birth_model(self._h, ctypes.byref(mesh), ctypes.byref(m)))
+        self._birth_mesh = Mesh(mesh.reference_radius, mesh.num_radial_zones, mesh.num_polar_zones,
+                                mesh.block_size, 0)
 
     def remove_block(self, index):
         _check(lib().knc_block_remove(self._h, int(index)))
@@ -352,6 +431,13 @@ class Context:
         for k in [k for k in self.shapes if n.value == 0 or k < lo.value or k > hi.value]:
             self.shapes.pop(k)
             self.vertices.pop(k, None)
+        mesh = getattr(self, "_birth_mesh", None)
+        if n.value and mesh is not None:
+            for k in range(lo.value, hi.value + 1):
+                if k not in self.shapes:
+                    rv, qv = mesh_block_vertices(mesh, k)
+                    self.shapes[k] = (rv.size - 1, qv.size - 1)
+                    self.vertices[k] = (rv, qv)
 
     def excision_counts(self):
         r, a = ctypes.c_int64(), ctypes.c_int64()
@@ -438,6 +524,31 @@ def max_signal_speed(h, primitive):
     out = np.zeros(prim.shape[0])
     _check(lib().knc_hydro_max_signal_speed(ctypes.byref(h), prim.shape[0], _p(prim), _p(out)))
     return out
+
+
+def model_eval(kind, params, r, theta, time, table=None, ctx=None):
+    """primitive_at (n, 4) and scalar_at (n) of an initial model at points
+    (r, theta) and `time`, evaluated on the device."""
+    r, q = _f64(r).ravel(), _f64(theta).ravel()
+    prim = np.empty((r.size, 4))
+    scal = np.empty(r.size)
+    m, _keep = make_model(kind, params, table)
+    own = ctx is None
+    if own:
+        ctx = Context(relativistic_hydro())
+    try:
+        rc = lib().knc_model_eval(ctx._h, ctypes.byref(m), r.size, _p(r), _p(q), float(time), _p(prim), _p(scal))
+        if rc == ERR_MODEL:
+            raise ModelError(lib().knc_last_error_message().decode())
+        _check(rc)
+    finally:
+        if own:
+            ctx.close()
+    return prim, scal
+
+
+class ModelError(RuntimeError):
+    """An initial model evaluated where the reference panics."""
 
 
 def plm_gradient(plm_theta, a, b, c):

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define KNC_ABI_VERSION 2
+#define KNC_ABI_VERSION 3
 
 /* hydrodynamics system and options */
 enum { KNC_EULER = 0, KNC_SRHD = 1 };
@@ -57,7 +57,11 @@ enum {
     KNC_ERR_INVALID_ARGUMENT = 10,
     KNC_ERR_HIP = 11,
     KNC_ERR_NO_BLOCK = 12,
-    KNC_ERR_UNSUPPORTED = 13
+    KNC_ERR_UNSUPPORTED = 13,
+    /* an initial model was evaluated where the reference panics: wind_shock
+     * outside its flares without a table or outside the table's range,
+     * halo_kilonova below the galactic midplane */
+    KNC_ERR_MODEL = 14
 };
 
 /* Hydrodynamics configuration: the fields of RelativisticHydro
@@ -215,6 +219,85 @@ typedef struct knc_excision {
 int knc_set_excision(knc_context* ctx, const knc_excision* excision);
 /* Blocks removed and added by excision since the context was created. */
 int knc_excision_counts(knc_context* ctx, int64_t* removed, int64_t* added);
+
+/* ---- initial models on the device (the files of src/models) ----------- */
+/* The InitialModel implementations of the reference, evaluated on the GPU:
+ * kinetic_bomb, jet_in_cloud, jet_in_star, halo_kilonova and wind_shock.
+ * params holds the fields of the model's struct in declaration order:
+ *   KINETIC_BOMB   external_medium_density, launch_radius, shell_thickness,
+ *                  kinetic_energy, shell_mass (src/models/kinetic_bomb.rs:16-22)
+ *   JET_IN_CLOUD   cloud_mass, engine_delay, engine_duration, engine_strength,
+ *                  engine_theta, engine_u, envelop_m1, envelop_slowest_beta,
+ *                  envelop_fastest_beta, envelop_psi (src/models/jet_in_cloud.rs:18-49)
+ *   JET_IN_STAR    star_mass, engine_duration, engine_energy, engine_theta,
+ *                  engine_u, envelope_radius, envelope_mass, volume_factor
+ *                  (src/models/jet_in_star.rs:37-62)
+ *   HALO_KILONOVA  altitude, launch_radius, shell_thickness, kinetic_energy,
+ *                  shell_mass, radial_distance (src/models/halo_kilonova.rs:14-25)
+ *   WIND_SHOCK     wind_mass_outflow_rate, wind_gamma_beta, wind_pressure,
+ *                  post_shock_pressure, shock_location, post_shock_gamma_beta,
+ *                  flare_outflow_rate, flare_gamma_beta, flare_time,
+ *                  flare_duration (src/models/wind_shock.rs:14-59); table
+ *                  holds table_rows rows (radius, gamma-beta, density,
+ *                  specific enthalpy) of the initial data table, the first
+ *                  column strictly increasing (LookupTable::from_rows,
+ *                  src/lookup_table_v2.rs:35-48), or table_rows = 0. */
+enum {
+    KNC_MODEL_KINETIC_BOMB = 1,
+    KNC_MODEL_JET_IN_CLOUD = 2,
+    KNC_MODEL_JET_IN_STAR = 3,
+    KNC_MODEL_HALO_KILONOVA = 4,
+    KNC_MODEL_WIND_SHOCK = 5
+};
+#define KNC_MODEL_NPARAM 12
+typedef struct knc_model {
+    int32_t kind;
+    int32_t table_rows;
+    double params[KNC_MODEL_NPARAM];
+    const double* table;        /* copied in; may be NULL when table_rows = 0 */
+} knc_model;
+
+/* The reference's radial block mesh (Mesh, src/mesh.rs:84-113): block i
+ * spans [R (1 + b)^i, R (1 + b)^(i+1)] with b = block_size * zone_dlogr,
+ * zone_dlogr = 1/num_radial_zones, or pi/num_polar_zones when
+ * num_radial_zones = 0 (None); block_size log-spaced radial zones; polar
+ * extent [0, pi], or pi/2 -+ zone_dlogr when num_polar_zones = 1
+ * (src/mesh.rs:341-375, 202-229). */
+typedef struct knc_mesh {
+    double reference_radius;
+    int32_t num_radial_zones;
+    int32_t num_polar_zones;
+    int32_t block_size;
+    int32_t pad;
+} knc_mesh;
+
+/* Vertices of block `index` of the mesh: block_size+1 radial and
+ * num_polar_zones+1 polar values (SphericalPolarGrid::geometry,
+ * src/mesh.rs:246-264).  Host only; no context needed. */
+int knc_mesh_block_vertices(const knc_mesh* mesh, int32_t index, double* radial_vertices,
+                            double* polar_vertices);
+
+/* primitive_at / scalar_at at n points (r, theta) and `time` on the device:
+ * primitive (n, 4) as (rho, u_r, u_theta, p), scalar (n). */
+int knc_model_eval(knc_context* ctx, const knc_model* model, int64_t n, const double* r, const double* theta,
+                   double time, double* primitive, double* scalar);
+
+/* Inserts block `index` of `mesh` built on the device by
+ * BlockState::from_model at `time` (src/state.rs:55-69). */
+int knc_block_insert_model(knc_context* ctx, int32_t index, const knc_mesh* mesh, const knc_model* model,
+                           double time);
+
+/* Makes `side` a model boundary evaluated on the device: at every stage's
+ * input time the ghost rows are BlockState::from_model on the two rows of
+ * mesh block inner_index - 1 (side 0) or outer_index + 1 (side 1) next to
+ * the live mesh, then try_to_primitive (src/scheme.rs:30-52).  No host call
+ * per stage.  model = NULL returns the side to the rows of knc_set_boundary.
+ * The mesh's polar grid must be the blocks'. */
+int knc_set_boundary_model(knc_context* ctx, int32_t side, const knc_mesh* mesh, const knc_model* model);
+
+/* Block births of excision (src/scheme.rs:197-205) from the model on the
+ * device, for a context whose excision has no block source; NULL clears. */
+int knc_set_birth_model(knc_context* ctx, const knc_mesh* mesh, const knc_model* model);
 
 /* ---- autotuner (no counterpart in the reference) ------------------------ */
 /* Times the stage kernels of the context's current mesh for each supported

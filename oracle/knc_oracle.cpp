@@ -30,6 +30,8 @@
 #include <thread>
 #include <vector>
 
+#include "knc_model_oracle.hpp"
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -545,6 +547,12 @@ struct Sim {
     Boundary inner, outer;
     BoundarySource source[2] = {nullptr, nullptr};
     void* source_user[2] = {nullptr, nullptr};
+    // model boundaries: ghost blocks of the mesh from the initial model
+    bool model_on[2] = {false, false};
+    om::Model model[2]{};
+    std::vector<double> model_table[2];
+    om::Mesh mesh[2]{};
+    int first_index = 0;          // block index of blocks[0]
     double time = 0.0;
     long long iteration = 0;
     int nthreads = 1;
@@ -792,27 +800,93 @@ ErrRec time_step(Sim& sim, double* dt)
 // RK1: U1; RK2: avg(L(L(U0)), U0; 1/2); RK3: b = 3/4 then 1/3.
 // Refreshes the ghost rows of the sides that have a boundary source, at
 // the time of the state the next stage starts from.
-int refresh_boundaries(Sim& sim, double t)
+// Ghost rows of a model side at time t: BlockState::from_model on the two
+// rows of mesh block first-1 (side 0) or last+1 (side 1) next to the live
+// mesh, then try_to_primitive and s = M/dV/Gamma (src/scheme.rs:30-52),
+// with U/dV as U (1/volA)(1/dcost) like the blocks.
+ErrRec model_rows(Sim& sim, int side, double t, std::vector<PS>& rows)
+{
+    const om::Mesh& mesh = sim.mesh[side];
+    const int nq = sim.blocks.front().nq;
+    const std::vector<double>& qv = sim.blocks.front().qv;
+    const int index = side ? sim.first_index + (int)sim.blocks.size() : sim.first_index - 1;
+    std::vector<double> rv((size_t)mesh.block_size + 1);
+    om::block_vertices(mesh, index, rv.data(), nullptr);
+    const int i0 = side ? 0 : mesh.block_size - 2;
+    rows.assign((size_t)2 * nq, PS{{0, 0, 0, 0}, 0});
+    ErrRec err;
+    for (int i = 0; i < 2; ++i) {
+        const RadialRow R = radial_row(rv[i0 + i], rv[i0 + i + 1]);
+        for (int j = 0; j < nq; ++j) {
+            const PolarCol C = polar_col(qv[j], qv[j + 1]);
+            double P[4], X = 0.0;
+            ErrRec e;
+            e.block = index;
+            e.i = i0 + i;
+            e.j = j;
+            e.r = R.rc;
+            e.theta = 0.5 * (qv[j] + qv[j + 1]);
+            if (!om::primitive_at(sim.model[side], R.rc, 0.5 * (qv[j] + qv[j + 1]), t, P, &X)) {
+                e.code = 14;
+                if (err.code == OK) err = e;
+                continue;
+            }
+            const Cons c = p2c(sim.h, Prim{P[0], P[1], P[2], P[3]});
+            const double dv = R.volA * C.dcost;
+            const Cons U{c.d * dv, c.sr * dv, c.sq * dv, c.tau * dv};
+            const double M = U.d * X;
+            const double rdv = R.rvolA * C.rdcost;
+            const Cons u{U.d * rdv, U.sr * rdv, U.sq * rdv, U.tau * rdv};
+            PS q{{0, 0, 0, 0}, 0};
+            double rW = 1.0;
+            int rc, it = 0;
+            if (sim.h.system == SYS_EULER) {
+                rc = c2p_euler(u, sim.h.gamma_law_index, &q.P);
+                e.value = u.d;
+            } else {
+                rc = c2p_srhd(u, sim.h.gamma_law_index, 0.0, &q.P, &it, &rW);
+                e.value = rc == NEGATIVE_DENSITY ? u.d : u.tau;
+            }
+            if (rc == NEGATIVE_PRESSURE) rc = OK;
+            if (rc != OK) {
+                e.code = rc;
+                e.u = u;
+                if (err.code == OK) err = e;
+                continue;
+            }
+            q.s = (M * rdv) * rW;
+            rows[(size_t)i * nq + j] = q;
+        }
+    }
+    return err;
+}
+
+// Refreshes the ghost rows of the sides with a boundary source or model,
+// at the time of the state the next stage starts from.
+ErrRec refresh_boundaries(Sim& sim, double t)
 {
     for (int side = 0; side < 2; ++side) {
-        if (!sim.source[side]) continue;
         Boundary& bd = side == 0 ? sim.inner : sim.outer;
         const int nq = sim.blocks.empty() ? 0 : sim.blocks.front().nq;
+        if (sim.model_on[side]) {
+            ErrRec e = model_rows(sim, side, t, bd.rows);
+            if (e.code != OK) return e;
+            bd.mode = BND_FIXED;
+            continue;
+        }
+        if (!sim.source[side]) continue;
         std::vector<double> prim((size_t)8 * nq), scalar((size_t)2 * nq);
-        if (sim.source[side](sim.source_user[side], side, t, nq, prim.data(), scalar.data())) return 1;
+        if (sim.source[side](sim.source_user[side], side, t, nq, prim.data(), scalar.data())) {
+            ErrRec e;
+            e.code = 10;
+            return e;
+        }
         bd.mode = BND_FIXED;
         bd.rows.resize((size_t)2 * nq);
         for (int k = 0; k < 2 * nq; ++k)
             bd.rows[k] = PS{{prim[4 * k], prim[4 * k + 1], prim[4 * k + 2], prim[4 * k + 3]}, scalar[k]};
     }
-    return 0;
-}
-
-ErrRec boundary_source_failed()
-{
-    ErrRec e;
-    e.code = 10;
-    return e;
+    return ErrRec{};
 }
 
 ErrRec rk_step(Sim& sim, double dt)
@@ -827,7 +901,7 @@ ErrRec rk_step(Sim& sim, double dt)
     const int order = sim.h.runge_kutta_order;
     // each stage's ghost rows come from the model at its input state's time:
     // t0, t0 + dt, and (RK3) the averaged time of the second stage's output
-    if (refresh_boundaries(sim, t0)) return boundary_source_failed();
+    if ((e = refresh_boundaries(sim, t0)).code) return e;
     if (order == 1) {
         e = stage(sim, U0, M0, nullptr, nullptr, 0.0, dt, U1, M1);
         if (e.code) return e;
@@ -835,7 +909,7 @@ ErrRec rk_step(Sim& sim, double dt)
     } else if (order == 2) {
         e = stage(sim, U0, M0, nullptr, nullptr, 0.0, dt, U1, M1);
         if (e.code) return e;
-        if (refresh_boundaries(sim, t0 + dt)) return boundary_source_failed();
+        if ((e = refresh_boundaries(sim, t0 + dt)).code) return e;
         e = stage(sim, U1, M1, &U0, &M0, 0.5, dt, U2, M2);
         if (e.code) return e;
         U1.swap(U2); M1.swap(M2);
@@ -844,13 +918,13 @@ ErrRec rk_step(Sim& sim, double dt)
     } else {
         e = stage(sim, U0, M0, nullptr, nullptr, 0.0, dt, U1, M1);
         if (e.code) return e;
-        if (refresh_boundaries(sim, t0 + dt)) return boundary_source_failed();
+        if ((e = refresh_boundaries(sim, t0 + dt)).code) return e;
         e = stage(sim, U1, M1, &U0, &M0, 0.75, dt, U2, M2);
         if (e.code) return e;
         const double t1 = t0 + dt;
         const double t2 = (t1 + dt) * (-0.75 + 1.0) + t0 * 0.75;
         const double b3 = 1.0 / 3.0;
-        if (refresh_boundaries(sim, t2)) return boundary_source_failed();
+        if ((e = refresh_boundaries(sim, t2)).code) return e;
         e = stage(sim, U2, M2, &U0, &M0, b3, dt, U1, M1);
         if (e.code) return e;
         t = (t2 + dt) * (-b3 + 1.0) + t0 * b3;
@@ -1040,6 +1114,77 @@ int ko_sim_set_boundary_source(void* p, int side, BoundarySource fn, void* user)
     Sim* s = (Sim*)p;
     s->source[side] = fn;
     s->source_user[side] = user;
+    return 0;
+}
+
+// Model boundary of `side` from a mesh and an initial model (the table is
+// copied); m = NULL clears it.
+int ko_sim_set_boundary_model(void* p, int side, const om::Model* m, const om::Mesh* mesh)
+{
+    Sim& sim = *(Sim*)p;
+    if (side < 0 || side > 1) return 1;
+    if (!m) {
+        sim.model_on[side] = false;
+        return 0;
+    }
+    sim.model[side] = *m;
+    sim.mesh[side] = *mesh;
+    const int rows = m->kind == 5 ? m->table_rows : 0;
+    sim.model_table[side].assign(m->table, m->table + (size_t)4 * rows);
+    sim.model[side].table = sim.model_table[side].data();
+    sim.model[side].table_rows = rows;
+    sim.model_on[side] = true;
+    return 0;
+}
+
+// Block index of the innermost block (for the ghost blocks of models).
+int ko_sim_set_first_index(void* p, int index)
+{
+    ((Sim*)p)->first_index = index;
+    return 0;
+}
+
+// BlockState::from_model of mesh block `index` at time t
+// (src/state.rs:55-69): vertices rv (block_size+1), qv (num_polar_zones+1),
+// conserved (nr, nq, 4) = to_conserved(P) dV, scalar_mass = D dV X.
+// Returns -1, or the first zone where the model cannot be evaluated.
+long long ko_block_from_model(const Hydro* h, const om::Model* m, const om::Mesh* mesh, int index, double t,
+                              double* rv, double* qv, double* cons, double* scalar_mass)
+{
+    om::block_vertices(*mesh, index, rv, qv);
+    const int nr = mesh->block_size, nq = mesh->num_polar_zones;
+    long long bad = -1;
+    for (int i = 0; i < nr; ++i) {
+        const RadialRow R = radial_row(rv[i], rv[i + 1]);
+        for (int j = 0; j < nq; ++j) {
+            const PolarCol C = polar_col(qv[j], qv[j + 1]);
+            const size_t k = (size_t)i * nq + j;
+            double P[4] = {0, 0, 0, 0}, X = 0.0;
+            if (!om::primitive_at(*m, R.rc, 0.5 * (qv[j] + qv[j + 1]), t, P, &X) && bad < 0) bad = (long long)k;
+            const Cons c = p2c(*h, Prim{P[0], P[1], P[2], P[3]});
+            const double dv = R.volA * C.dcost;
+            const double d = c.d * dv;
+            cons[4 * k] = d;
+            cons[4 * k + 1] = c.sr * dv;
+            cons[4 * k + 2] = c.sq * dv;
+            cons[4 * k + 3] = c.tau * dv;
+            scalar_mass[k] = d * X;
+        }
+    }
+    return bad;
+}
+
+// Replaces the state of block bi (same shape).
+int ko_sim_set_block(void* p, int bi, const double* cons, const double* scalar_mass)
+{
+    Sim& sim = *(Sim*)p;
+    if (bi < 0 || bi >= (int)sim.blocks.size()) return 1;
+    Block& b = sim.blocks[bi];
+    const size_t n = (size_t)b.nr * b.nq;
+    for (size_t k = 0; k < n; ++k) {
+        b.U[k] = Cons{cons[4 * k], cons[4 * k + 1], cons[4 * k + 2], cons[4 * k + 3]};
+        b.M[k] = scalar_mass[k];
+    }
     return 0;
 }
 

@@ -42,7 +42,7 @@ EULER, SRHD = 0, 1
 HLLE, HLLC = 0, 1
 FIXED, OUTFLOW = 0, 1
 CODES = {0: "ok", 1: "NegativeDensity", 2: "NegativePressure", 3: "NegativeEnergyDensity",
-         4: "RootFinderFailed"}
+         4: "RootFinderFailed", 14: "ModelPanics"}
 
 
 class KoHydro(ctypes.Structure):
@@ -63,6 +63,37 @@ def hydro(system=SRHD, gamma=4.0 / 3.0, plm_theta=1.5, cfl=0.4, rk=2, riemann=HL
     if light_speed is None:
         light_speed = LIGHT_SPEED if system == SRHD else 1.0
     return KoHydro(system, riemann, rk, int(adaptive), gamma, plm_theta, cfl, light_speed)
+
+
+class OModel(ctypes.Structure):
+    """om::Model (knc_model_oracle.hpp): kind, table rows, 12 parameters in
+    the reference struct's field order, table pointer."""
+    _fields_ = [("kind", ctypes.c_int), ("table_rows", ctypes.c_int), ("params", ctypes.c_double * 12),
+                ("table", ctypes.c_void_p)]
+
+
+class OMesh(ctypes.Structure):
+    """om::Mesh: reference_radius, num_radial_zones (0 = None),
+    num_polar_zones, block_size."""
+    _fields_ = [("reference_radius", ctypes.c_double), ("num_radial_zones", ctypes.c_int),
+                ("num_polar_zones", ctypes.c_int), ("block_size", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
+def omodel(kind, params, table=None):
+    m = OModel()
+    m.kind = int(kind)
+    for k, v in enumerate(params):
+        m.params[k] = float(v)
+    tab = None
+    if table is not None:
+        tab = np.ascontiguousarray(table, dtype=np.float64).reshape(-1, 4)
+        m.table_rows = tab.shape[0]
+        m.table = tab.ctypes.data
+    return m, tab
+
+
+def omesh(reference_radius, num_radial_zones, num_polar_zones, block_size):
+    return OMesh(float(reference_radius), int(num_radial_zones), int(num_polar_zones), int(block_size), 0)
 
 
 _lib = None
@@ -93,6 +124,19 @@ def lib():
         L.ko_plm_gradient.argtypes = [ctypes.c_double, ctypes.c_int64, P, P, P, P]
         L.ko_sim_set_boundary_source.restype = ctypes.c_int
         L.ko_sim_set_boundary_source.argtypes = [P, ctypes.c_int, BOUNDARY_SOURCE, P]
+        L.ko_model_eval.restype = ctypes.c_longlong
+        L.ko_model_eval.argtypes = [ctypes.POINTER(OModel), ctypes.c_longlong, P, P, ctypes.c_double, P, P]
+        L.ko_mesh_block_vertices.restype = None
+        L.ko_mesh_block_vertices.argtypes = [ctypes.POINTER(OMesh), ctypes.c_int, P, P]
+        L.ko_block_from_model.restype = ctypes.c_longlong
+        L.ko_block_from_model.argtypes = [ctypes.POINTER(KoHydro), ctypes.POINTER(OModel), ctypes.POINTER(OMesh),
+                                          ctypes.c_int, ctypes.c_double, P, P, P, P]
+        L.ko_sim_set_boundary_model.restype = ctypes.c_int
+        L.ko_sim_set_boundary_model.argtypes = [P, ctypes.c_int, ctypes.POINTER(OModel), ctypes.POINTER(OMesh)]
+        L.ko_sim_set_block.restype = ctypes.c_int
+        L.ko_sim_set_block.argtypes = [P, ctypes.c_int, P, P]
+        L.ko_sim_set_first_index.restype = ctypes.c_int
+        L.ko_sim_set_first_index.argtypes = [P, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -106,6 +150,37 @@ def _f64(a, shape=None):
     if shape is not None:
         a = a.reshape(shape)
     return a
+
+
+# ---- initial models and the block mesh (knc_model_oracle.cpp) -------------
+def model_eval(kind, params, r, theta, t, table=None):
+    """primitive_at (n, 4), scalar_at (n) and the index of the first point
+    the reference would panic at (-1 if none)."""
+    r, q = _f64(r).ravel(), _f64(theta).ravel()
+    prim = np.zeros((r.size, 4))
+    scal = np.zeros(r.size)
+    m, _keep = omodel(kind, params, table)
+    bad = lib().ko_model_eval(ctypes.byref(m), r.size, _p(r), _p(q), float(t), _p(prim), _p(scal))
+    return prim, scal, bad
+
+
+def mesh_block_vertices(mesh, index):
+    rv = np.zeros(mesh.block_size + 1)
+    qv = np.zeros(mesh.num_polar_zones + 1)
+    lib().ko_mesh_block_vertices(ctypes.byref(mesh), int(index), _p(rv), _p(qv))
+    return rv, qv
+
+
+def block_from_model(h, mesh, kind, params, index, t, table=None):
+    """BlockState::from_model of mesh block `index`: (rv, qv, conserved,
+    scalar_mass, first failing zone or -1)."""
+    nr, nq = mesh.block_size, mesh.num_polar_zones
+    rv, qv = np.zeros(nr + 1), np.zeros(nq + 1)
+    cons, sm = np.zeros((nr, nq, 4)), np.zeros((nr, nq))
+    m, _keep = omodel(kind, params, table)
+    bad = lib().ko_block_from_model(ctypes.byref(h), ctypes.byref(m), ctypes.byref(mesh), int(index), float(t),
+                                    _p(rv), _p(qv), _p(cons), _p(sm))
+    return rv, qv, cons, sm, bad
 
 
 # ---- per-zone trait methods --------------------------------------------------
@@ -250,6 +325,7 @@ class Sim:
         if add:
             rv, cons, sm = source(outer_index + 1, t)
             self.push_block(rv, self.qv, cons, sm)
+        self.set_first_index(first_index)
         return first_index
 
     def scheme_advance(self, fold, ex=None, source=None, first_index=0):
@@ -267,6 +343,19 @@ class Sim:
 
     def num_blocks(self):
         return lib().ko_sim_num_blocks(self._p)
+
+    def set_first_index(self, index):
+        """Block index of the innermost block (ghost blocks of model sides)."""
+        lib().ko_sim_set_first_index(self._p, int(index))
+
+    def set_boundary_model(self, side, mesh, kind=None, params=(), table=None):
+        """Ghost rows of `side` from the initial model at every stage's input
+        time (src/scheme.rs:30-52); kind None clears."""
+        if kind is None:
+            lib().ko_sim_set_boundary_model(self._p, side, None, None)
+            return
+        m, _keep = omodel(kind, params, table)
+        lib().ko_sim_set_boundary_model(self._p, side, ctypes.byref(m), ctypes.byref(mesh))
 
     def set_boundary_source(self, side, fn):
         """Ghost rows of `side` from fn(side, time) -> (prim (2, nq, 4),
@@ -295,6 +384,12 @@ class Sim:
             prim = _f64(prim).reshape(2 * nq, 4)
             scalar = _f64(scalar).reshape(2 * nq)
         lib().ko_sim_set_boundary(self._p, side, mode, nq, _p(prim), _p(scalar))
+
+    def set_block(self, b, cons, scalar_mass):
+        """Replaces the state of block b (same shape)."""
+        cons = _f64(cons)
+        sm = _f64(scalar_mass)
+        lib().ko_sim_set_block(self._p, int(b), _p(cons), _p(sm))
 
     def get_block(self, b, nr, nq):
         cons = np.zeros((nr, nq, 4))

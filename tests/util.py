@@ -49,3 +49,51 @@ def rel_l1(state, ref):
 
 def max_rel_l1(state, ref):
     return max(rel_l1(state, ref).values())
+
+
+def oracle_mesh(preset):
+    m = preset.knc_mesh
+    return oracle.omesh(m.reference_radius, m.num_radial_zones, m.num_polar_zones, m.block_size)
+
+
+def oracle_from_preset(preset, nthreads=8):
+    """App::from_config on the oracle: the blocks of grid_blocks(start_time)
+    from the model, model ghost rows on both sides."""
+    sim = oracle.Sim(oracle_hydro_from_capi(preset.hydro), nthreads)
+    mesh = oracle_mesh(preset)
+    t0 = preset.control["start_time"]
+    for i in preset.grid_block_indexes(t0):
+        rv, qv, cons, sm, bad = oracle.block_from_model(sim.h, mesh, preset.model_kind, preset.model_params, i, t0,
+                                                        preset.model_table)
+        assert bad == -1, f"model fails in block {i}"
+        sim.push_block(rv, qv, cons, sm)
+    sim.time = t0
+    sim.set_first_index(0)
+    for side in (0, 1):
+        sim.set_boundary_model(side, mesh, preset.model_kind, preset.model_params, preset.model_table)
+    return sim
+
+
+def oracle_birth_source(sim, preset):
+    """Block births from the model for Sim.scheme_advance."""
+    mesh = oracle_mesh(preset)
+
+    def source(index, t):
+        rv, qv, cons, sm, bad = oracle.block_from_model(sim.h, mesh, preset.model_kind, preset.model_params, index,
+                                                        t, preset.model_table)
+        assert bad == -1
+        return rv, cons, sm
+    return source
+
+
+def oracle_hydro_from_capi(h):
+    return oracle.KoHydro(h.system, h.riemann_solver, h.runge_kutta_order, h.adaptive_time_step, h.gamma_law_index,
+                          h.plm_theta, h.cfl_number, h.light_speed)
+
+
+def preset_excision(preset):
+    """The Excision of a preset's mesh for Sim.scheme_advance."""
+    from knc.setups import Excision
+    m = preset.mesh
+    return Excision(m["inner_radius"], m["outer_radius"], m["inner_excision_speed"], m["outer_excision_speed"],
+                    m["excision_delay"] or 0.0, 1)
