@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/knc_gpu.h"
+#include "knc_context.hpp"
 #include "knc_device.hpp"
 #include "knc_kernels.hpp"
 #include "knc_mlp_host.hpp"
@@ -35,7 +36,9 @@ namespace {
 
 thread_local std::string g_last_error;
 
-int fail(int code, const char* fmt, ...)
+}  // namespace
+
+int knc::fail(int code, const char* fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -46,13 +49,7 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return fail(KNC_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                        __LINE__);                                                                 \
-    } while (0)
+namespace {
 
 int check_hydro(const knc_hydro* h)
 {
@@ -110,107 +107,7 @@ std::vector<double> radial_table(int nr, const double* rv)
     return t;
 }
 
-// A device copy of an initial model (knc_model) and the mesh its ghost
-// blocks and births come from.
-struct DevModel {
-    bool on = false;
-    ModelDev dev{};
-    double* d_table = nullptr;
-    knc_mesh mesh{};
-};
-
-struct HostBlock {
-    int32_t index = 0;
-    int nr = 0, nq = 0;
-    double* mem = nullptr;      // NSLOT * 5 * nr * nq doubles
-    double* rgeo = nullptr;     // device radial table
-    float* guess = nullptr;     // learned-guess plane, nr * nq floats
-    double* prim = nullptr;     // primitive planes of the two-pass stage, 5 * nr * nq
-    std::vector<double> rv;
-};
-
 }  // namespace
-
-struct knc_context {
-    knc_hydro h{};
-    knc_options o{};
-    ZoneParams zp{};
-    hipStream_t stream = nullptr;
-    int num_cu = 0;
-
-    std::map<int32_t, HostBlock> blocks;  // ordered by radial index
-    int cur = 0;                          // slot holding U^n
-    double time = 0.0;
-    int64_t iteration = 0;
-
-    // polar table
-    int nq = 0;
-    std::vector<double> qv;
-    double* d_theta = nullptr;
-    ThetaTable th{};
-
-    // boundary rows
-    int inner_mode = BND_OUTFLOW, outer_mode = BND_OUTFLOW;
-    double* d_bnd = nullptr;   // 2 x (5 planes x 2nq)
-
-    // launch tables
-    bool dirty = true;
-    int rows_per_item = 16;
-    BlockDesc* d_blocks = nullptr;
-    int cap_blocks = 0;
-    WorkItem* d_items = nullptr;
-    int cap_items = 0;
-    int32_t* d_nitems = nullptr;
-    int nitems = 0;
-    std::vector<std::pair<int, int>> block_items;  // (offset, count) per table entry
-    int threads = 64;
-    int persistent_grid = 0;
-
-    // per-call device state
-    ErrLog* d_err = nullptr;
-    C2PStats* d_stats = nullptr;
-    double* d_dt_partial = nullptr;       // per-workgroup minima of the CFL pass
-    double* d_dt = nullptr;
-
-    // graphs keyed by (starting slot, steps, first fold step for model boundaries)
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
-
-    // learned c2p guess: device copy of the network's operand fragments
-    void* d_mlp = nullptr;
-    MlpDev mlp{};
-    bool has_mlp = false;
-
-    // model boundaries: ghost rows per stage of a fold, side-major within a
-    // stage (5 planes x 2nq each), filled on the host and uploaded per segment
-    knc_boundary_fn bsrc[2] = {nullptr, nullptr};
-    void* buser[2] = {nullptr, nullptr};
-    double* h_bseq = nullptr;   // pinned
-    double* d_bseq = nullptr;
-    size_t cap_bseq = 0;        // doubles
-    bool trial = false;         // autotuner trial steps: fixed rows only
-
-    // initial models evaluated on the device: per side for model boundaries,
-    // one for excision births; stage times of the fold's ghost-row sets
-    DevModel bmodel[2];
-    DevModel birth;
-    double* d_qc = nullptr;          // polar zone centres 0.5 (q0 + q1)
-    std::vector<double> stage_times; // per (fold step, stage)
-
-    // moving excision surfaces
-    bool exc_on = false;
-    knc_excision exc{};
-    int64_t exc_removed = 0, exc_added = 0;
-
-    // timing
-    bool timing = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_stage_ms = 0.0;
-    int last_launches = 0;
-};
-
-int insert_block(knc_context* c, int32_t index, int32_t nr, int32_t nq, const double* rv, const double* qv,
-                 const double* conserved, const double* scalar_mass);
-int insert_block_model(knc_context* c, int32_t index, const DevModel& dm, double t);
 
 namespace {
 
@@ -667,8 +564,9 @@ int enqueue_time_step(knc_context* c)
     return KNC_OK;
 }
 
-// Reads the error log after a synchronised call; fills err and returns the code.
-int collect_error(knc_context* c, knc_error* err)
+}  // namespace
+
+int knc::collect_error(knc_context* c, knc_error* err)
 {
     unsigned long long key = 0;
     HIP_TRY(hipMemcpy(&key, &c->d_err->min_key, sizeof key, hipMemcpyDeviceToHost));
@@ -706,6 +604,15 @@ int collect_error(knc_context* c, knc_error* err)
     return fail(code, "hydro error %d at block %d zone (%d,%d) (r, theta) = (%.4e, %.4f)", code, e.block, e.i, e.j,
                 e.r, e.theta);
 }
+
+int knc::reset_errors(knc_context* c)
+{
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
+    HIP_TRY(hipGetLastError());
+    return KNC_OK;
+}
+
+namespace {
 
 int require_blocks(knc_context* c)
 {
@@ -1203,7 +1110,7 @@ int knc_block_insert(knc_context* c, int32_t index, int32_t nr, int32_t nq, cons
 
 // knc_block_insert; with conserved NULL the block starts at zero in every
 // slot (the caller fills slot c->cur on the device).
-extern "C++" int insert_block(knc_context* c, int32_t index, int32_t nr, int32_t nq, const double* rv, const double* qv,
+extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, int32_t nq, const double* rv, const double* qv,
                  const double* conserved, const double* scalar_mass)
 {
     if (nr < 2) return fail(KNC_ERR_INVALID_ARGUMENT, "must have at least 2 radial zones per block");
@@ -1436,7 +1343,7 @@ int knc_model_eval(knc_context* c, const knc_model* model, int64_t n, const doub
 
 // BlockState::from_model of mesh block `index` at time t, inserted as a new
 // block (src/state.rs:55-69 on the vertices of src/mesh.rs:202-264).
-extern "C++" int insert_block_model(knc_context* c, int32_t index, const DevModel& dm, double t)
+extern "C++" int knc::insert_block_model(knc_context* c, int32_t index, const DevModel& dm, double t)
 {
     const knc_mesh& m = dm.mesh;
     const int nr = m.block_size, nq = m.num_polar_zones;

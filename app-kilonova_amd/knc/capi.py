@@ -33,6 +33,8 @@ GUESS_PLAIN, GUESS_LEARNED, GUESS_LEARNED_STEP = 0, 1, 2
 ERROR_KINDS = {1: "NegativeDensity", 2: "NegativePressure", 3: "NegativeEnergyDensity",
                4: "RootFinderFailed"}
 ERR_MODEL = 14   # an initial model evaluated where the reference panics
+ERR_IO = 15      # a file could not be opened, read or written
+ERR_FORMAT = 16  # a checkpoint is not an App of the expected shape
 
 # initial models on the device (src/models/*.rs); params in struct field order
 KINETIC_BOMB, JET_IN_CLOUD, JET_IN_STAR, HALO_KILONOVA, WIND_SHOCK = 1, 2, 3, 4, 5
@@ -123,6 +125,28 @@ def mesh_block_vertices(mesh, index):
     return rv, qv
 
 
+class RecordMeta(ctypes.Structure):
+    _fields_ = [("config", ctypes.c_void_p), ("config_len", ctypes.c_uint64), ("tasks", ctypes.c_void_p),
+                ("tasks_len", ctypes.c_uint64), ("version", ctypes.c_char_p)]
+
+
+class CheckpointInfo(ctypes.Structure):
+    _fields_ = [("system", ctypes.c_int32), ("num_blocks", ctypes.c_int32), ("time", ctypes.c_double),
+                ("iteration_numer", ctypes.c_int64), ("iteration_denom", ctypes.c_int64),
+                ("config", ctypes.c_void_p), ("config_len", ctypes.c_uint64), ("tasks", ctypes.c_void_p),
+                ("tasks_len", ctypes.c_uint64), ("version", ctypes.c_char_p)]
+
+
+def _record_meta(config, tasks, version):
+    """A RecordMeta over copies of the CBOR items; returns (meta, buffers to keep)."""
+    cbuf = ctypes.create_string_buffer(bytes(config), len(config))
+    tbuf = ctypes.create_string_buffer(bytes(tasks), len(tasks)) if tasks is not None else None
+    meta = RecordMeta(ctypes.cast(cbuf, ctypes.c_void_p), len(config),
+                      ctypes.cast(tbuf, ctypes.c_void_p) if tbuf is not None else None,
+                      len(tasks) if tasks is not None else 0, version.encode())
+    return meta, (cbuf, tbuf)
+
+
 class C2PStats(ctypes.Structure):
     _fields_ = [("hist", ctypes.c_uint64 * 64), ("zones", ctypes.c_uint64), ("fallbacks", ctypes.c_uint64),
                 ("floors", ctypes.c_uint64), ("learned_iters", ctypes.c_uint64)]
@@ -203,6 +227,15 @@ def lib():
     L.knc_block_insert_model.argtypes = [P, I32, ctypes.POINTER(Mesh), ctypes.POINTER(Model), D]
     L.knc_set_boundary_model.argtypes = [P, I32, ctypes.POINTER(Mesh), ctypes.POINTER(Model)]
     L.knc_set_birth_model.argtypes = [P, ctypes.POINTER(Mesh), ctypes.POINTER(Model)]
+    L.knc_write_checkpoint.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(RecordMeta)]
+    L.knc_write_products.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(RecordMeta), ctypes.POINTER(Error)]
+    L.knc_checkpoint_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(P)]
+    L.knc_checkpoint_get_info.argtypes = [P, ctypes.POINTER(CheckpointInfo)]
+    L.knc_checkpoint_block_shape.argtypes = [P, I32, P, P, P]
+    L.knc_checkpoint_block_data.argtypes = [P, I32, P, P]
+    L.knc_checkpoint_load.argtypes = [P, P, ctypes.POINTER(Mesh)]
+    L.knc_checkpoint_close.argtypes = [P]
+    L.knc_checkpoint_close.restype = None
     _lib = L
     return L
 
@@ -439,6 +472,22 @@ class Context:
                     self.shapes[k] = (rv.size - 1, qv.size - 1)
                     self.vertices[k] = (rv, qv)
 
+    # -- output files (src/io.rs, src/main.rs:50-68) -------------------------
+    def write_checkpoint(self, path, config, tasks, version):
+        """chkpt.NNNN.cbor: App {state, tasks, config, version} with the state
+        encoded from HBM; config and tasks are CBOR items (bytes)."""
+        meta, _keep = _record_meta(config, tasks, version)
+        _check(lib().knc_write_checkpoint(self._h, os.fsencode(path), ctypes.byref(meta)))
+
+    def write_products(self, path, config, version):
+        """prods.NNNN.cbor: Products {time, blocks, config, version} with
+        try_to_primitive on the device; raises HydroError if a zone fails."""
+        meta, _keep = _record_meta(config, None, version)
+        err = Error()
+        rc = lib().knc_write_products(self._h, os.fsencode(path), ctypes.byref(meta), ctypes.byref(err))
+        if rc:
+            self._raise(rc, err)
+
     def excision_counts(self):
         r, a = ctypes.c_int64(), ctypes.c_int64()
         _check(lib().knc_excision_counts(self._h, ctypes.byref(r), ctypes.byref(a)))
@@ -478,6 +527,61 @@ class Context:
 
 
 # ---- per-zone Hydrodynamics methods ------------------------------------------
+class Checkpoint:
+    """A checkpoint file (App, src/app.rs:179-186) parsed by the library:
+    ``system``, ``time``, ``iteration`` (numer, denom), ``config`` and
+    ``tasks`` (the CBOR items, bytes), ``version`` and the blocks."""
+
+    def __init__(self, path):
+        h = ctypes.c_void_p()
+        _check(lib().knc_checkpoint_open(os.fsencode(path), ctypes.byref(h)))
+        self._h = h
+        info = CheckpointInfo()
+        _check(lib().knc_checkpoint_get_info(self._h, ctypes.byref(info)))
+        self.system = info.system
+        self.time = info.time
+        self.iteration = (info.iteration_numer, info.iteration_denom)
+        self.config = ctypes.string_at(info.config, info.config_len)
+        self.tasks = ctypes.string_at(info.tasks, info.tasks_len)
+        self.version = info.version.decode()
+        self.blocks = []
+        for k in range(info.num_blocks):
+            i, nr, nq = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+            _check(lib().knc_checkpoint_block_shape(self._h, k, ctypes.byref(i), ctypes.byref(nr), ctypes.byref(nq)))
+            self.blocks.append((i.value, nr.value, nq.value))
+
+    def block_data(self, k):
+        """(conserved (nr, nq, 4), scalar_mass (nr, nq)) of block k."""
+        _, nr, nq = self.blocks[k]
+        cons = np.zeros((nr, nq, 4))
+        sm = np.zeros((nr, nq))
+        _check(lib().knc_checkpoint_block_data(self._h, int(k), _p(cons), _p(sm)))
+        return cons, sm
+
+    def load(self, ctx, mesh):
+        """Every block into an empty Context with the mesh's vertices; sets
+        the context's time and iteration."""
+        _check(lib().knc_checkpoint_load(self._h, ctx._h, ctypes.byref(mesh)))
+        for index, nr, nq in self.blocks:
+            rv, qv = mesh_block_vertices(mesh, index)
+            ctx.shapes[index] = (nr, nq)
+            ctx.vertices[index] = (rv, qv)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().knc_checkpoint_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def try_to_primitive(h, conserved, guess=None):
     cons = _f64(conserved).reshape(-1, 4)
     n = cons.shape[0]

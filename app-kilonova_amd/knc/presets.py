@@ -18,6 +18,7 @@ all evaluated on the GPU (``knc_block_insert_model``,
 """
 import copy
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -90,6 +91,10 @@ class Preset:
     model_table: np.ndarray = None
     mesh: dict = field(default_factory=dict)
     control: dict = field(default_factory=dict)
+    # the model struct's serialised fields in declaration order (serde skips
+    # halo_kilonova's external_medium_density; wind_shock ends with
+    # initial_data_table)
+    model_fields: dict = field(default_factory=dict)
 
     @property
     def knc_mesh(self):
@@ -129,6 +134,37 @@ class Preset:
     def moving_excision(self):
         return self.mesh["inner_excision_speed"] > 0.0 or self.mesh["outer_excision_speed"] > 0.0
 
+    def to_serde(self):
+        """The Configuration as serde serialises it (src/app.rs:163-172): the
+        hydro and model enums externally tagged in snake_case, every struct's
+        fields in declaration order, f64 fields as floats, usize fields as
+        ints, absent options as None."""
+        h = self.hydro
+        rk = f"RK{h.runge_kutta_order}"
+        common = {"gamma_law_index": float(h.gamma_law_index), "plm_theta": float(h.plm_theta),
+                  "cfl_number": float(h.cfl_number), "runge_kutta_order": rk}
+        if h.system == capi.SRHD:
+            hydro = {"relativistic": dict(common, riemann_solver="HLLC" if h.riemann_solver == capi.HLLC else "HLLE",
+                                          adaptive_time_step=bool(h.adaptive_time_step))}
+        else:
+            hydro = {"newtonian": common}
+        m = self.mesh
+        opt_f = lambda v: None if v is None else float(v)
+        opt_i = lambda v: None if v is None else int(v)
+        mesh = {"reference_radius": float(m["reference_radius"]), "inner_radius": float(m["inner_radius"]),
+                "outer_radius": float(m["outer_radius"]), "inner_excision_speed": float(m["inner_excision_speed"]),
+                "outer_excision_speed": float(m["outer_excision_speed"]),
+                "num_radial_zones": opt_i(m["num_radial_zones"]), "num_polar_zones": int(m["num_polar_zones"]),
+                "block_size": int(m["block_size"]), "excision_delay": opt_f(m["excision_delay"])}
+        c = self.control
+        control = {"start_time": float(c["start_time"]), "final_time": float(c["final_time"]),
+                   "checkpoint_interval": float(c["checkpoint_interval"]),
+                   "products_interval": opt_f(c["products_interval"]), "fold": int(c["fold"]),
+                   "num_threads": opt_i(c["num_threads"]), "snappy_compression": bool(c["snappy_compression"]),
+                   "output_directory": str(c["output_directory"])}
+        return {"hydro": hydro, "model": {self.model_name: dict(self.model_fields)}, "mesh": mesh,
+                "control": control}
+
 
 def parse(config, table_loader=None):
     """A Preset from a parsed setup mapping (serde's checks restated)."""
@@ -161,6 +197,12 @@ def parse(config, table_loader=None):
     mf = _fields(msec, f"model.{mname}", names if not optional else [n for n in names if n not in optional],
                  optional)
     params = [_f(mf[n], f"model.{mname}.{n}") for n in names]
+    model_fields = dict(zip(names, params))
+    if mname == "wind_shock":
+        path = mf.get("initial_data_table")
+        if path is not None and not isinstance(path, str):
+            raise ConfigError("model.wind_shock.initial_data_table: expected a string")
+        model_fields["initial_data_table"] = path
     table = None
     if mname == "wind_shock" and mf.get("initial_data_table"):
         if table_loader is None:
@@ -182,7 +224,17 @@ def parse(config, table_loader=None):
         raise ConfigError("must have at least 2 radial zones per block")
     if mesh["num_polar_zones"] == 1 and not mesh["num_radial_zones"]:
         raise ConfigError("num_radial_zones is not optional when num_polar_zones=1")
-    return Preset(hydro, mname, kind, params, table, mesh, co)
+    for k in ("products_interval",):
+        if co[k] is not None:
+            co[k] = _f(co[k], f"control.{k}")
+    for k in ("num_threads",):
+        if co[k] is not None:
+            co[k] = _i(co[k], f"control.{k}")
+    if not isinstance(co["snappy_compression"], bool):
+        raise ConfigError("control.snappy_compression: expected a bool")
+    if not isinstance(co["output_directory"], str):
+        raise ConfigError("control.output_directory: expected a string")
+    return Preset(hydro, mname, kind, params, table, mesh, co, model_fields)
 
 
 def ascii_table(path):
@@ -230,18 +282,12 @@ def load(path, overrides=(), table_loader=ascii_table):
     return parse(apply_overrides(config, overrides), table_loader=table_loader)
 
 
-def context_from_preset(preset, **options):
-    """App::from_config on the GPU: the blocks of Mesh::grid_blocks at
-    control.start_time built from the model on the device, model ghost rows
+def configure_context(ctx, preset):
+    """The parts of App::from_config that are not the state: model ghost rows
     on both sides and, with moving surfaces, excision with births from the
-    model.  Returns the Context (time = start_time)."""
-    ctx = capi.Context(preset.hydro, **options)
+    model -- all evaluated on the device."""
     mesh = preset.knc_mesh
-    t0 = float(preset.control["start_time"])
     args = (preset.model_kind, preset.model_params)
-    for i in preset.grid_block_indexes(t0):
-        ctx.insert_block_model(i, mesh, *args, t0, table=preset.model_table)
-    ctx.set_time(t0, 0)
     for side in (capi.INNER, capi.OUTER):
         ctx.set_boundary_model(side, mesh, *args, table=preset.model_table)
     if preset.moving_excision:
@@ -250,3 +296,128 @@ def context_from_preset(preset, **options):
                          m["outer_excision_speed"], m["excision_delay"] or 0.0, 1, None)
         ctx.set_birth_model(mesh, *args, table=preset.model_table)
     return ctx
+
+
+def context_from_preset(preset, **options):
+    """App::from_config on the GPU: the blocks of Mesh::grid_blocks at
+    control.start_time built from the model on the device, then
+    configure_context.  Returns the Context (time = start_time)."""
+    ctx = capi.Context(preset.hydro, **options)
+    mesh = preset.knc_mesh
+    t0 = float(preset.control["start_time"])
+    for i in preset.grid_block_indexes(t0):
+        ctx.insert_block_model(i, mesh, preset.model_kind, preset.model_params, t0, table=preset.model_table)
+    ctx.set_time(t0, 0)
+    return configure_context(ctx, preset)
+
+
+def validate(preset):
+    """Configuration::validate (src/app.rs:273-279): the hydro's ranges
+    (src/physics/relativistic_hydro.rs:46-54), the models' own checks
+    (src/models/kinetic_bomb.rs:66-76, halo_kilonova.rs:47-59,
+    wind_shock.rs:75-82), Mesh::validate (src/mesh.rs:273-298) and
+    Control::validate (src/app.rs:202-213).  Raises ConfigError."""
+    h = preset.hydro
+    if not 1.0 <= h.plm_theta <= 2.0:
+        raise ConfigError("plm_theta must be in the range [1, 2]")
+    if not 0.0 <= h.cfl_number <= 0.7:
+        raise ConfigError("cfl_number must be in the range [0.0, 0.7]")
+    f = preset.model_fields
+    if preset.model_name in ("kinetic_bomb", "halo_kilonova"):
+        v = math.sqrt(2.0 * f["kinetic_energy"] / f["shell_mass"])
+        if v > 0.25 * capi.LIGHT_SPEED:
+            raise ConfigError(f"The shell is moving faster (v/c = {v / capi.LIGHT_SPEED}) than 0.25 c, but this "
+                              "problem assumes Newtonian expressions for the kinetic energy. Consider reducing the "
+                              "kinetic energy or increasing the shell mass.")
+    if preset.model_name == "wind_shock" and f["wind_gamma_beta"] < 0.0:
+        raise ConfigError("the wind four-velocity must be positive")
+    m = preset.mesh
+    t0 = float(preset.control["start_time"])
+    if m["reference_radius"] <= 0.0 or m["inner_radius"] < 0.0 or m["outer_radius"] < 0.0:
+        raise ConfigError("all radii must be positive")
+    if m["reference_radius"] > preset.excision_surface(1, t0):
+        raise ConfigError("the reference radius is outside the outer excision surface")
+    if (m["excision_delay"] or 0.0) < 0.0:
+        raise ConfigError("the excision delay time must be non-negative")
+    if m["inner_excision_speed"] < 0.0 or m["outer_excision_speed"] < 0.0:
+        raise ConfigError("the excision surface speeds must be non-negative")
+    if m["outer_excision_speed"] < m["inner_excision_speed"]:
+        raise ConfigError("outer_excision_speed < inner_excision_speed (the IES would eventually overtake the OES)")
+    if m["num_polar_zones"] != 1 and m["num_polar_zones"] < 16:
+        raise ConfigError("must have at least 2 radial zones per block")   # the reference's message
+    c = preset.control
+    threads = c["num_threads"] if c["num_threads"] is not None else 2 * (os.cpu_count() or 1)
+    if threads == 0 or threads >= 1024:
+        raise ConfigError("num_threads must be > 0 and < 1024")
+    if c["checkpoint_interval"] < 0.0:
+        raise ConfigError("checkpoint_interval <= 0.0")
+    if (c["products_interval"] or 0.0) < 0.0:
+        raise ConfigError("products_interval <= 0.0")
+    return preset
+
+
+# The reference's preset setups (setups/*.yaml, embedded by App::presets,
+# src/app.rs:385-393), restated as mappings.  jet_in_star carries the field
+# eta_0 and lacks envelope_mass and volume_factor, so that setup is rejected
+# here as serde rejects it.
+SETUPS = {
+    "jet_in_cloud": {
+        "hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.5, "cfl_number": 0.4,
+                                   "runge_kutta_order": "RK2", "riemann_solver": "HLLC"}},
+        "model": {"jet_in_cloud": {"cloud_mass": 1e32, "engine_delay": 1.0, "engine_duration": 1.0,
+                                   "engine_strength": 1.0, "engine_theta": 0.1, "engine_u": 30.0,
+                                   "envelop_m1": 2e27, "envelop_slowest_beta": 0.1, "envelop_fastest_beta": 0.995,
+                                   "envelop_psi": 0.25}},
+        "mesh": {"inner_radius": 0.0, "outer_radius": 0.0, "inner_excision_speed": 3e8, "outer_excision_speed": 3e10,
+                 "reference_radius": 3e8, "num_polar_zones": 512, "block_size": 4},
+        "control": {"final_time": 5.0, "start_time": 1.0, "checkpoint_interval": 0.1, "fold": 100,
+                    "num_threads": None, "output_directory": "data"},
+    },
+    "jet_in_star": {
+        "hydro": {"relativistic": {"gamma_law_index": 1.33, "plm_theta": 1.5, "cfl_number": 0.3,
+                                   "runge_kutta_order": "RK2", "riemann_solver": "HLLC"}},
+        "model": {"jet_in_star": {"star_mass": 1e34, "engine_duration": 10.0, "engine_energy": 1e51,
+                                  "engine_theta": 0.1, "engine_u": 50.0, "envelope_radius": 1e11, "eta_0": 1.0}},
+        "mesh": {"inner_radius": 1e9, "outer_radius": 1e12, "excision_delay": 10.0, "inner_excision_speed": 1e9,
+                 "outer_excision_speed": 3e10, "reference_radius": 1e9, "num_polar_zones": 64, "block_size": 4},
+        "control": {"final_time": 3.0, "start_time": 0.0, "checkpoint_interval": 0.1, "fold": 100,
+                    "num_threads": None, "output_directory": "data"},
+    },
+    "halo_kilonova": {
+        "hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.1, "cfl_number": 0.2,
+                                   "runge_kutta_order": "RK2", "riemann_solver": "HLLC",
+                                   "adaptive_time_step": True}},
+        "model": {"halo_kilonova": {"altitude": 1e20, "external_medium_density": 1e-23, "launch_radius": 1e18,
+                                    "shell_thickness": 1e18, "kinetic_energy": 1e51, "shell_mass": 2e32,
+                                    "radial_distance": 1e22}},
+        "mesh": {"inner_radius": 1e17, "outer_radius": 1e20, "inner_excision_speed": 0.0,
+                 "outer_excision_speed": 0.0, "reference_radius": 1e18, "num_polar_zones": 256, "block_size": 4},
+        "control": {"final_time": 1e10, "start_time": 0.0, "checkpoint_interval": 1e8, "output_directory": "data",
+                    "num_threads": 16, "fold": 10},
+    },
+    "wind_shock": {
+        "hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.5, "cfl_number": 0.1,
+                                   "runge_kutta_order": "RK2", "riemann_solver": "HLLC"}},
+        "model": {"wind_shock": {"wind_mass_outflow_rate": 1e20, "flare_outflow_rate": 1e33, "flare_gamma_beta": 1e2,
+                                 "wind_gamma_beta": 1e2, "post_shock_gamma_beta": 1e-2, "wind_pressure": 1e-17,
+                                 "post_shock_pressure": 1e-9, "shock_location": 5e8, "flare_time": 1.0,
+                                 "flare_duration": 0.01}},
+        "mesh": {"inner_radius": 5e8, "outer_radius": 50e9, "inner_excision_speed": 0.0,
+                 "outer_excision_speed": 0.0, "reference_radius": 2e8, "num_polar_zones": 1,
+                 "num_radial_zones": 32, "block_size": 32},
+        "control": {"final_time": 8.0, "start_time": 0.0, "checkpoint_interval": 0.05, "fold": 10,
+                    "num_threads": 4, "output_directory": "data"},
+    },
+    "kinetic_bomb": {
+        "hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.5, "cfl_number": 0.3,
+                                   "runge_kutta_order": "RK2", "riemann_solver": "HLLC",
+                                   "adaptive_time_step": True}},
+        "model": {"kinetic_bomb": {"external_medium_density": 1e-23, "launch_radius": 1e18,
+                                   "shell_thickness": 1e18, "kinetic_energy": 1e51, "shell_mass": 2e32}},
+        "mesh": {"inner_radius": 1e18, "outer_radius": 2e19, "inner_excision_speed": 0.0,
+                 "outer_excision_speed": 0.0, "reference_radius": 1e18, "num_polar_zones": 1,
+                 "num_radial_zones": 512, "block_size": 512},
+        "control": {"final_time": 1e9, "start_time": 0.0, "checkpoint_interval": 1e7, "output_directory": "data",
+                    "num_threads": None, "fold": 10},
+    },
+}

@@ -4,7 +4,7 @@
  * Drop-in boundary for the hot path of clemson-cal/app-kilonova:
  *   - the per-scheme seam  scheme::advance      (src/scheme.rs:212-240),
  *                          State::time_step     (src/state.rs:143-162),
- *                          the block set of State (src/state.rs:26-43) and
+ *                          the block set of State (src/state.rs:24-43) and
  *                          add_remove_blocks    (src/scheme.rs:179-206);
  *   - the per-zone seam    trait Hydrodynamics  (src/traits.rs:41-136),
  *     implemented by RelativisticHydro (src/physics/relativistic_hydro.rs)
@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define KNC_ABI_VERSION 3
+#define KNC_ABI_VERSION 4
 
 /* hydrodynamics system and options */
 enum { KNC_EULER = 0, KNC_SRHD = 1 };
@@ -61,7 +61,12 @@ enum {
     /* an initial model was evaluated where the reference panics: wind_shock
      * outside its flares without a table or outside the table's range,
      * halo_kilonova below the galactic midplane */
-    KNC_ERR_MODEL = 14
+    KNC_ERR_MODEL = 14,
+    /* a file could not be opened, read or written (io::Error, src/io.rs:16-17) */
+    KNC_ERR_IO = 15,
+    /* a checkpoint is not an App of the expected shape (ciborium::de::Error,
+     * src/io.rs:13-14) */
+    KNC_ERR_FORMAT = 16
 };
 
 /* Hydrodynamics configuration: the fields of RelativisticHydro
@@ -298,6 +303,70 @@ int knc_set_boundary_model(knc_context* ctx, int32_t side, const knc_mesh* mesh,
 /* Block births of excision (src/scheme.rs:197-205) from the model on the
  * device, for a context whose excision has no block source; NULL clears. */
 int knc_set_birth_model(knc_context* ctx, const knc_mesh* mesh, const knc_model* model);
+
+/* ---- checkpoints and products (src/io.rs, src/app.rs:174-186,
+ *      src/products.rs) ---------------------------------------------------- */
+/* The reference's output files, serde-serialised through ciborium 0.1.0
+ * (src/io.rs:24-35): every number in its shortest lossless CBOR form,
+ * structs as maps keyed by field name, enums externally tagged, ndarray
+ * arrays as {"v": 1, "dim": [...], "data": [...]}, BlockIndex map keys as
+ * [i, j], the Rational64 iteration as [numer, denom].  The library encodes
+ * the state from HBM; the Configuration (src/app.rs:163-172) and the Tasks
+ * (src/tasks.rs:10-47) are small and arrive already CBOR-encoded from the
+ * caller, copied into the file verbatim. */
+typedef struct knc_record_meta {
+    const uint8_t* config;      /* CBOR of app::Configuration */
+    uint64_t config_len;
+    const uint8_t* tasks;       /* CBOR of tasks::Tasks (checkpoints only) */
+    uint64_t tasks_len;
+    const char* version;        /* the version string of App / Products */
+} knc_record_meta;
+
+/* chkpt.NNNN.cbor (side_effects, src/main.rs:61-68): App {state, tasks,
+ * config, version} (src/app.rs:179-186) with state = AnyState {"Relativistic" | "Newtonian":
+ * State {time, iteration, solution {[i, 0]: BlockState {conserved,
+ * scalar_mass}}}} (src/state.rs:24-43); conserved elements are 4-tuples
+ * (D, S_r, S_theta, tau).  The blocks stream from HBM through pinned
+ * buffers while the previous block is encoded. */
+int knc_write_checkpoint(knc_context* ctx, const char* path, const knc_record_meta* meta);
+
+/* prods.NNNN.cbor (src/main.rs:50-59): Products {time, blocks {[i, 0]:
+ * BlockProducts {radial_vertices, polar_vertices, primitive, scalar}},
+ * config, version} (src/products.rs:14-37, 44-91): try_to_primitive of every zone
+ * on the device, primitive elements in AnyPrimitive order (u_r, u_theta,
+ * rho, p) (src/physics/mod.rs:36-75), scalar = scalar_mass / D.  A failed
+ * c2p writes nothing and returns its KNC_HYDRO_* code with err filled. */
+int knc_write_products(knc_context* ctx, const char* path, const knc_record_meta* meta, knc_error* err);
+
+/* Reading a checkpoint (App::from_file on a .cbor, src/app.rs:344-350).
+ * The handle parses and checks the whole file when opened; block data are
+ * decoded on request.  Byte ranges and strings returned by
+ * knc_checkpoint_get_info stay valid until knc_checkpoint_close. */
+typedef struct knc_checkpoint knc_checkpoint;
+typedef struct knc_checkpoint_info {
+    int32_t system;             /* KNC_SRHD (Relativistic) or KNC_EULER (Newtonian) */
+    int32_t num_blocks;
+    double time;
+    int64_t iteration_numer;
+    int64_t iteration_denom;
+    const uint8_t* config;      /* the CBOR item of the config field */
+    uint64_t config_len;
+    const uint8_t* tasks;       /* the CBOR item of the tasks field */
+    uint64_t tasks_len;
+    const char* version;
+} knc_checkpoint_info;
+int knc_checkpoint_open(const char* path, knc_checkpoint** out);
+int knc_checkpoint_get_info(const knc_checkpoint* ck, knc_checkpoint_info* out);
+/* Block k (0 <= k < num_blocks, increasing block index). */
+int knc_checkpoint_block_shape(const knc_checkpoint* ck, int32_t k, int32_t* index, int32_t* nr, int32_t* nq);
+/* conserved (nr, nq, 4) and scalar_mass (nr, nq), as knc_block_download. */
+int knc_checkpoint_block_data(const knc_checkpoint* ck, int32_t k, double* conserved, double* scalar_mass);
+/* Inserts every block of the checkpoint into an empty context, with the
+ * vertices of `mesh` (knc_mesh_block_vertices; the blocks' shapes must be
+ * block_size x num_polar_zones), and sets the context's time and iteration
+ * (the iteration must be whole). */
+int knc_checkpoint_load(const knc_checkpoint* ck, knc_context* ctx, const knc_mesh* mesh);
+void knc_checkpoint_close(knc_checkpoint* ck);
 
 /* ---- autotuner (no counterpart in the reference) ------------------------ */
 /* Times the stage kernels of the context's current mesh for each supported

@@ -1051,6 +1051,50 @@ int ko_geometry(int nr, int nq, const double* rv, const double* qv, double* Ar, 
 }
 
 // Whole-mesh simulation: a radially ordered chain of blocks.
+// BlockProducts::try_from_block_state (src/products.rs:44-66) for one block:
+// try_to_primitive of U/dV per zone (the NegativePressure floor accepted),
+// written in AnyPrimitive order (velocity_r, velocity_q, mass_density,
+// gas_pressure; src/physics/mod.rs:36-75 and Hydrodynamics::any,
+// relativistic_hydro.rs:114-121), and scalar = scalar_mass / lab_frame_mass
+// (M / D).  Returns 0, or the first failure (row-major) in err.
+int ko_products(const Hydro* h, int nr, int nq, const double* rv, const double* qv, const double* cons,
+                const double* scalar_mass, double* prim, double* scalar, ko_error* err)
+{
+    for (int i = 0; i < nr; ++i) {
+        const RadialRow R = radial_row(rv[i], rv[i + 1]);
+        for (int j = 0; j < nq; ++j) {
+            const PolarCol C = polar_col(qv[j], qv[j + 1]);
+            const size_t k = (size_t)i * nq + j;
+            const double rdv = R.rvolA * C.rdcost;
+            const Cons u{cons[4 * k] * rdv, cons[4 * k + 1] * rdv, cons[4 * k + 2] * rdv, cons[4 * k + 3] * rdv};
+            Prim P{0, 0, 0, 0};
+            int rc, it = 0;
+            double rW = 1.0;
+            if (h->system == SYS_EULER) rc = c2p_euler(u, h->gamma_law_index, &P);
+            else rc = c2p_srhd(u, h->gamma_law_index, 0.0, &P, &it, &rW);
+            if (rc == NEGATIVE_PRESSURE) rc = OK;
+            if (rc != OK) {
+                ErrRec e;
+                e.code = rc;
+                e.i = i;
+                e.j = j;
+                e.r = R.rc;
+                e.theta = 0.5 * (qv[j] + qv[j + 1]);
+                e.value = (h->system == SYS_EULER || rc == NEGATIVE_DENSITY) ? u.d : u.tau;
+                e.u = u;
+                put_err(e, err);
+                return rc;
+            }
+            prim[4 * k] = P.ur;
+            prim[4 * k + 1] = P.uq;
+            prim[4 * k + 2] = P.rho;
+            prim[4 * k + 3] = P.p;
+            scalar[k] = scalar_mass[k] / cons[4 * k];
+        }
+    }
+    return 0;
+}
+
 void* ko_sim_create(const Hydro* h, int nthreads)
 {
     Sim* s = new Sim();

@@ -248,6 +248,23 @@ def plm_gradient(theta, a, b, c):
     return out
 
 
+def products(h, rv, qv, cons, scalar_mass):
+    """BlockProducts of one block: (primitive (nr, nq, 4) in AnyPrimitive order
+    (u_r, u_theta, rho, p), scalar (nr, nq)); raises OracleError on a c2p
+    failure."""
+    rv, qv = _f64(rv), _f64(qv)
+    nr, nq = rv.size - 1, qv.size - 1
+    cons = _f64(cons).reshape(nr, nq, 4)
+    sm = _f64(scalar_mass).reshape(nr, nq)
+    prim = np.zeros((nr, nq, 4))
+    scal = np.zeros((nr, nq))
+    err = KoError()
+    if lib().ko_products(ctypes.byref(h), nr, nq, _p(rv), _p(qv), _p(cons), _p(sm), _p(prim), _p(scal),
+                         ctypes.byref(err)):
+        raise OracleError(err)
+    return prim, scal
+
+
 def geometry(rv, qv):
     rv = _f64(rv)
     qv = _f64(qv)

@@ -26,6 +26,10 @@ STRUCTS = {
                                           "from_cache", "ms_per_stage", "default_ms_per_stage"]),
     "knc_model": (capi.Model, ["kind", "table_rows", "params", "table"]),
     "knc_mesh": (capi.Mesh, ["reference_radius", "num_radial_zones", "num_polar_zones", "block_size"]),
+    "knc_record_meta": (capi.RecordMeta, ["config", "config_len", "tasks", "tasks_len", "version"]),
+    "knc_checkpoint_info": (capi.CheckpointInfo, ["system", "num_blocks", "time", "iteration_numer",
+                                                  "iteration_denom", "config", "config_len", "tasks", "tasks_len",
+                                                  "version"]),
 }
 
 ENUMS = {"KNC_EULER": capi.EULER, "KNC_SRHD": capi.SRHD, "KNC_HLLE": capi.HLLE, "KNC_HLLC": capi.HLLC,
@@ -35,7 +39,8 @@ ENUMS = {"KNC_EULER": capi.EULER, "KNC_SRHD": capi.SRHD, "KNC_HLLE": capi.HLLE, 
          "KNC_GUESS_LEARNED_STEP": capi.GUESS_LEARNED_STEP, "KNC_ERR_MODEL": capi.ERR_MODEL,
          "KNC_MODEL_KINETIC_BOMB": capi.KINETIC_BOMB, "KNC_MODEL_JET_IN_CLOUD": capi.JET_IN_CLOUD,
          "KNC_MODEL_JET_IN_STAR": capi.JET_IN_STAR, "KNC_MODEL_HALO_KILONOVA": capi.HALO_KILONOVA,
-         "KNC_MODEL_WIND_SHOCK": capi.WIND_SHOCK, "KNC_MODEL_NPARAM": capi.MODEL_NPARAM}
+         "KNC_MODEL_WIND_SHOCK": capi.WIND_SHOCK, "KNC_MODEL_NPARAM": capi.MODEL_NPARAM,
+         "KNC_ERR_IO": capi.ERR_IO, "KNC_ERR_FORMAT": capi.ERR_FORMAT}
 
 
 @pytest.fixture(scope="module")

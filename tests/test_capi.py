@@ -32,7 +32,7 @@ def test_library_exports_every_declared_function():
 
 
 def test_abi_version():
-    assert knc.lib().knc_abi_version() == 3
+    assert knc.lib().knc_abi_version() == 4
 
 
 def test_invalid_hydro_is_rejected_before_any_device_call():

@@ -187,3 +187,29 @@ def test_geometry_telescopes_and_closed_forms():
     assert np.allclose(g["rc"], np.sqrt(rv[:-1] * rv[1:])[:, None])
     dq = np.diff(qv)
     assert np.allclose(g["dl"], np.minimum(np.diff(rv)[:, None], dq[None, :] * rv[:-1, None]))
+
+
+def test_oracle_products_recover_the_model_primitive():
+    """BlockProducts (src/products.rs:44-66): a block built as from_model does
+    (conserved = to_conserved(P) dV, scalar_mass = D dV X) gives back P in
+    AnyPrimitive's array order [velocity_r, velocity_q, mass_density,
+    gas_pressure] (src/physics/mod.rs:59-63) and the concentration X; a
+    negative density is reported at its zone."""
+    rng = np.random.default_rng(11)
+    rv = np.geomspace(1.0, 3.0, 6)
+    qv = np.linspace(0.0, np.pi, 5)
+    dV = oracle.geometry(rv, qv)["dV"]
+    for system, gamma in ((oracle.SRHD, 4.0 / 3.0), (oracle.EULER, 5.0 / 3.0)):
+        h = oracle.hydro(system, gamma=gamma)
+        P = np.stack([10 ** rng.uniform(-3, 0, (5, 4)), rng.uniform(-3, 3, (5, 4)), rng.uniform(-1, 1, (5, 4)),
+                      10 ** rng.uniform(-4, -1, (5, 4))], -1)
+        X = rng.uniform(0, 1, (5, 4))
+        cons = oracle.to_conserved(h, P).reshape(5, 4, 4) * dV[..., None]
+        sm = cons[..., 0] * X
+        prim, scal = oracle.products(h, rv, qv, cons, sm)
+        assert np.allclose(prim, P[..., [1, 2, 0, 3]], rtol=1e-11, atol=0)
+        assert np.allclose(scal, X, rtol=1e-15, atol=1e-300)
+        cons[3, 2, 0] = -abs(cons[3, 2, 0])
+        with pytest.raises(oracle.OracleError) as e:
+            oracle.products(h, rv, qv, cons, sm)
+        assert (e.value.code, e.value.i, e.value.j) == (1, 3, 2)
