@@ -189,6 +189,43 @@ def c2p_roofline(zones, stage_ms, mean_iterations):
             "ms_at_fp64_peak": 1e3 * t, "frac_of_stage": 1e3 * t / stage_ms, "peak_tflops": FP64_VECTOR_PEAK / 1e12}
 
 
+def io_rates(ctx, zones):
+    """The reference's output files of the bench state, once, outside the
+    timed region: knc_write_checkpoint (state streamed from HBM and encoded
+    on the host), knc_write_products (device c2p, then the same), and the
+    checkpoint reader (parse and check, then decode every block)."""
+    import tempfile
+
+    from knc import app as kapp
+    from knc import capi, cbor
+    cfg = cbor.encode({"workload": "bomb_jet"})
+    tasks = cbor.encode(kapp.Tasks(0.0).to_serde())
+    out = {"state_MB": 40.0 * zones / 1e6}
+    with tempfile.TemporaryDirectory() as d:
+        for name, write in (("checkpoint", lambda p: ctx.write_checkpoint(p, cfg, tasks, kapp.VERSION)),
+                            ("products", lambda p: ctx.write_products(p, cfg, kapp.VERSION))):
+            path = os.path.join(d, name + ".cbor")
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            write(path)
+            dt = time.perf_counter() - t0
+            size = os.path.getsize(path) / 1e6
+            out[name] = {"write_s": dt, "file_MB": size, "file_MB_per_s": size / dt}
+        path = os.path.join(d, "checkpoint.cbor")
+        t0 = time.perf_counter()
+        with capi.Checkpoint(path) as ck:
+            t1 = time.perf_counter()
+            for k in range(len(ck.blocks)):
+                ck.block_data(k)
+            t2 = time.perf_counter()
+        out["checkpoint"].update({"open_s": t1 - t0, "decode_s": t2 - t1,
+                                  "read_MB_per_s": out["checkpoint"]["file_MB"] / (t2 - t0)})
+    log(f"bench: io checkpoint {out['checkpoint']['file_MB']:.0f} MB written in {out['checkpoint']['write_s']:.3f} s,"
+        f" read in {out['checkpoint']['open_s'] + out['checkpoint']['decode_s']:.3f} s; products "
+        f"{out['products']['file_MB']:.0f} MB in {out['products']['write_s']:.3f} s")
+    return out
+
+
 def copy_probe(mib=512, repeats=5):
     """Device-to-device copies of a known size, so that FETCH_SIZE and
     WRITE_SIZE per dispatch can be scaled to bytes."""
@@ -217,6 +254,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-stats", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
+    ap.add_argument("--no-io", action="store_true", help="skip the checkpoint/products file measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run five 512 MiB device-to-device copies (calibration of the HBM traffic counters)")
     args = ap.parse_args()
@@ -276,6 +314,7 @@ def main():
         plain = {"value": rate_p, "ms_per_step": 1e3 * wall_p / args.steps, "stage_ms": stage_ms_p}
 
     newton = newton_stats(setup, args.fold, net) if not args.no_stats else None
+    io = io_rates(ctx_b, zones) if not args.no_io else None
 
     # headline: the faster learned-guess form (batched plain guess without weights)
     head = max(learned, key=lambda k: learned[k]["value"]) if learned else None
@@ -335,6 +374,7 @@ def main():
                              if plain else None),
         "rel_l1_vs_plain": parity or None,
         "newton": newton,
+        "io": io,
         "stage_ms": head_stage,
     }
     print(json.dumps(out), flush=True)
