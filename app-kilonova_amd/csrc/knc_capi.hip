@@ -111,10 +111,16 @@ std::vector<double> radial_table(int nr, const double* rv)
 
 namespace {
 
+// The network supplies guesses (k_guess passes, a network must be attached).
 bool learned(const knc_context* c)
 {
     return (c->o.c2p_guess == KNC_GUESS_LEARNED || c->o.c2p_guess == KNC_GUESS_LEARNED_STEP) &&
            c->h.system == KNC_SRHD;
+}
+// The stage's c2p starts Newton from the guess planes (learned or previous).
+bool guess_read(const knc_context* c)
+{
+    return learned(c) || (c->o.c2p_guess == KNC_GUESS_PREVIOUS && c->h.system == KNC_SRHD);
 }
 bool two_pass(const knc_context* c) { return c->o.stage_passes == 2; }
 // A side whose ghost rows are refreshed at every stage time (host callback
@@ -293,7 +299,7 @@ int choose_rows(knc_context* c)
     int64_t rows = 0;
     for (auto& kv : c->blocks) rows += kv.second.nr;
     const int tiles = (c->nq + c->threads - 1) / c->threads;
-    const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, learned(c),
+    const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, guess_read(c),
                                                 two_pass(c), c->threads));
     const int64_t want = (int64_t)c->num_cu * occ;
     int64_t r = rows * tiles / std::max<int64_t>(1, want);
@@ -368,7 +374,7 @@ int rebuild_tables(knc_context* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     // Persistent grid: every CU holds as many stage workgroups as fit.
     const int per_cu = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
-                                                   learned(c), two_pass(c), c->threads));
+                                                   guess_read(c), two_pass(c), c->threads));
     c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
     c->dirty = false;
     return KNC_OK;
@@ -398,6 +404,7 @@ StageArgs base_args(knc_context* c)
     a.step = 0;
     a.rows = c->rows_per_item;
     a.mlp = c->mlp;
+    a.write_guess = c->o.c2p_guess == KNC_GUESS_PREVIOUS;
     return a;
 }
 
@@ -430,10 +437,11 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage)
 {
     const bool stats = c->o.collect_stats != 0;
     const bool prims = two_pass(c);
-    const bool lg = learned(c);
+    const bool lg = guess_read(c);
     // KNC_GUESS_LEARNED_STEP evaluates the network on the first stage's input
     // only; the later stages of the step start Newton from the same guesses.
-    const bool guess_pass = lg && (first_stage || c->o.c2p_guess == KNC_GUESS_LEARNED);
+    // KNC_GUESS_PREVIOUS has no pass: each stage stores its own pressures.
+    const bool guess_pass = learned(c) && (first_stage || c->o.c2p_guess == KNC_GUESS_LEARNED);
     auto pre_pass = [&](const StageArgs& b, int nitems) -> int {
         const int grid = row_pass_grid(c, nitems);
         if (guess_pass) {
@@ -880,7 +888,7 @@ std::map<std::vector<int64_t>, std::vector<int>> g_tune_cache;
 std::vector<int64_t> tune_key(const knc_context* c)
 {
     std::vector<int64_t> k = {c->h.system, c->h.riemann_solver, c->h.runge_kutta_order, c->o.collect_stats,
-                              learned(c) ? c->o.c2p_guess : 0, c->o.launch_mode, c->o.device, c->num_cu, c->nq};
+                              guess_read(c) ? c->o.c2p_guess : 0, c->o.launch_mode, c->o.device, c->num_cu, c->nq};
     for (auto& kv : c->blocks) k.push_back(kv.second.nr);
     return k;
 }
@@ -1035,7 +1043,7 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
         c->o.use_graph = 0;
     }
     if (c->o.c2p_guess != KNC_GUESS_PLAIN && c->o.c2p_guess != KNC_GUESS_LEARNED &&
-        c->o.c2p_guess != KNC_GUESS_LEARNED_STEP) {
+        c->o.c2p_guess != KNC_GUESS_LEARNED_STEP && c->o.c2p_guess != KNC_GUESS_PREVIOUS) {
         delete c;
         return fail(KNC_ERR_INVALID_ARGUMENT, "unknown c2p guess mode");
     }
@@ -1154,6 +1162,8 @@ extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, in
     }
     HIP_TRY(hipMemcpy(b.rgeo, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(b.mem, 0, (size_t)NSLOT * 5 * n * sizeof(double)));
+    // all-ones words are NaN: no guess yet (the plain guess is used)
+    HIP_TRY(hipMemset(b.guess, 0xff, n * sizeof(float)));
     if (conserved) {
         std::vector<double> soa(5 * n);
         for (size_t k = 0; k < n; ++k) {

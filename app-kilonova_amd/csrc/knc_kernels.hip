@@ -361,6 +361,10 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                 }
             }
             const PS Pk = raw_primitive<SYS, STATS, GUESS>(a, src, cur, j, lds_hist);
+            if (GUESS && !PRIMS && a.write_guess && act && k >= it.i0 && k < it.i1) {
+                // this item owns row k: keep its pressure as the next stage's guess
+                gstf(B.guess, (int64_t)k * nq + j, guess_log2(Pk.P.p, cur.v[0] * (src.rvolA * gld(a.th.rdcost, j))));
+            }
             Flux5 Fk;
             if (k >= it.i0) {
                 const PS g = plm5(th, Pm2, Pm1, Pk);                 // gx[k-1]
@@ -509,6 +513,7 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
                 gst(P, 2 * np + j, q.P.uq);
                 gst(P, 3 * np + j, q.P.p);
                 gst(P, 4 * np + j, q.s);
+                if (GUESS && a.write_guess) gstf(B.guess, (int64_t)i * nq + j, guess_log2(q.P.p, u0 * rdv));
             }
         }
     }

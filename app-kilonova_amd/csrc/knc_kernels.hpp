@@ -32,6 +32,7 @@ struct StageArgs {
     int32_t step;                // error-key step number (0 = dt pass)
     int32_t rows;                // rows per work item (row-parallel passes: k_prim, k_guess, k_dt)
     MlpDev mlp;                  // learned c2p guess (learned-guess kernels only)
+    int32_t write_guess;         // guess kernels also store each owned zone's recovered log2(p/D)
 };
 
 // One failed zone into the error log.  tag = (block order << 16) | row, or

@@ -107,14 +107,14 @@ def cpu_baseline(setup, nsteps, nthreads):
 
 
 def newton_stats(setup, fold, net, steps=20):
-    """Newton iteration histogram and fallback rate of the plain and learned
-    guesses over a short instrumented run (not timed).  The two-pass stage
+    """Newton iteration histogram and fallback rate of the plain, learned and
+    previous-pressure guesses over a short instrumented run (not timed).  The two-pass stage
     form is used so that every zone is counted once per stage."""
     import knc
     out = {}
     for name, n, guess in (("plain", None, knc.GUESS_PLAIN), ("learned", net, knc.GUESS_LEARNED),
-                           ("learned_step", net, knc.GUESS_LEARNED_STEP)):
-        if name != "plain" and n is None:
+                           ("learned_step", net, knc.GUESS_LEARNED_STEP), ("previous", None, knc.GUESS_PREVIOUS)):
+        if name.startswith("learned") and n is None:
             continue
         ctx = knc.context_from_setup(setup, launch_mode=knc.BATCHED, c2p_guess=guess, collect_stats=True,
                                      stage_passes=2)
@@ -252,6 +252,7 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="stage workgroup size (0: library choice)")
     ap.add_argument("--rows", type=int, default=0, help="radial rows per work item (0: library choice)")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-previous", action="store_true", help="skip the previous-pressure guess path")
     ap.add_argument("--no-stats", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
     ap.add_argument("--no-io", action="store_true", help="skip the checkpoint/products file measurement")
@@ -296,6 +297,17 @@ def main():
             log(f"bench: {name}+batched {rate / 1e9:.3f} Gzu/s, stage {stage_ms:.4f} ms")
             learned[name] = {"value": rate, "ms_per_step": 1e3 * wall / args.steps, "stage_ms": stage_ms}
             ctx_l[name] = ctx
+
+    # the previous-pressure guess: every stage starts Newton from the
+    # pressure its zone's previous stage recovered (no network, no extra pass)
+    prev = None
+    if not args.no_previous:
+        ctx_v, wall_v, stage_ms_v = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, tune=tune,
+                                            guess=knc.GUESS_PREVIOUS, **shape)
+        rate_v = zones * args.steps / wall_v
+        log(f"bench: previous+batched {rate_v / 1e9:.3f} Gzu/s, stage {stage_ms_v:.4f} ms")
+        prev = {"value": rate_v, "ms_per_step": 1e3 * wall_v / args.steps, "stage_ms": stage_ms_v}
+        ctx_l["previous"] = ctx_v
 
     plain = None
     parity = {}
@@ -368,9 +380,11 @@ def main():
         "cpu_baseline": cpu,
         "paths": {"plain_per_block": plain,
                   "batched": {"value": rate_b, "ms_per_step": 1e3 * wall_b / args.steps, "stage_ms": stage_ms_b},
-                  **{f"{k}_batched": v for k, v in learned.items()}},
+                  **{f"{k}_batched": v for k, v in learned.items()},
+                  **({"previous_batched": prev} if prev else {})},
         "speedup_vs_plain": ({"batched": rate_b / plain["value"],
-                              **{f"{k}_batched": v["value"] / plain["value"] for k, v in learned.items()}}
+                              **{f"{k}_batched": v["value"] / plain["value"] for k, v in learned.items()},
+                              **({"previous_batched": prev["value"] / plain["value"]} if prev else {})}
                              if plain else None),
         "rel_l1_vs_plain": parity or None,
         "newton": newton,

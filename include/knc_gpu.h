@@ -44,8 +44,11 @@ enum { KNC_SIDE_INNER = 0, KNC_SIDE_OUTER = 1 };
 enum { KNC_LAUNCH_PER_BLOCK = 0, KNC_LAUNCH_BATCHED = 1 };
 /* c2p initial pressure: PLAIN p = 0; LEARNED the network's guess for every
  * stage's input; LEARNED_STEP the network evaluated on the first stage's
- * input of each RK step, its guesses reused by the step's later stages. */
-enum { KNC_GUESS_PLAIN = 0, KNC_GUESS_LEARNED = 1, KNC_GUESS_LEARNED_STEP = 2 };
+ * input of each RK step, its guesses reused by the step's later stages;
+ * PREVIOUS the pressure each zone's previous stage recovered (p = 0 for a
+ * zone with none yet: a new block's first stage).  Newton ends in a
+ * guess-independent polish, so every mode gives the plain mode's state. */
+enum { KNC_GUESS_PLAIN = 0, KNC_GUESS_LEARNED = 1, KNC_GUESS_LEARNED_STEP = 2, KNC_GUESS_PREVIOUS = 3 };
 
 /* return codes: hydro errors mirror HydroErrorType (src/physics/mod.rs:82-95) */
 enum {
