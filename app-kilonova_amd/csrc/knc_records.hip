@@ -84,13 +84,18 @@ int products_grid(int64_t n)
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
 }
 
-// Two pinned host buffers, their copy events and a device scratch.
+// Two pinned host buffers, their copy events and a device scratch.  The
+// destructor waits for the stream first, so that no copy is still writing
+// a buffer it frees (an error path can leave one in flight).
 struct Staging {
+    hipStream_t stream = nullptr;
     double* host[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
     double* dev = nullptr;
+    explicit Staging(hipStream_t s) : stream(s) {}
     ~Staging()
     {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (int k = 0; k < 2; ++k) {
             if (host[k]) (void)hipHostFree(host[k]);
             if (ev[k]) (void)hipEventDestroy(ev[k]);
@@ -447,7 +452,7 @@ int knc_write_checkpoint(knc_context* c, const char* path, const knc_record_meta
     w.sint(1);
     w.text("solution");
     w.map(c->blocks.size());
-    Staging st;
+    Staging st(c->stream);
     rc = stream_blocks(
         c, st, [&](const HostBlock& b, size_t) { return b.mem + (size_t)c->cur * 5 * b.nr * b.nq; },
         [&](const HostBlock& b, const double* h) {
@@ -487,7 +492,7 @@ int knc_write_products(knc_context* c, const char* path, const knc_record_meta* 
     if (c->blocks.empty()) return fail(KNC_ERR_NO_BLOCK, "no blocks");
     // c2p of every block into one device scratch, then check for failures
     // before anything is written.
-    Staging st;
+    Staging st(c->stream);
     size_t total = 0;
     std::vector<size_t> offset;
     for (auto& kv : c->blocks) {

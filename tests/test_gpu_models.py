@@ -36,44 +36,12 @@ def wind_table(r0=5e7, r1=2e11, n=400, mdot=1e20, u=100.0, mu=1e-3):
     return np.stack([r, np.full(n, u), d, np.full(n, C * C * (1 + mu))], -1)
 
 
-# the reference's setups/*.yaml, restated as mappings
-SETUP_KB = {"hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.5, "cfl_number": 0.3,
-                                       "runge_kutta_order": "RK2", "riemann_solver": "HLLC",
-                                       "adaptive_time_step": True}},
-            "model": {"kinetic_bomb": {"external_medium_density": 1e-23, "launch_radius": 1e18,
-                                       "shell_thickness": 1e18, "kinetic_energy": 1e51, "shell_mass": 2e32}},
-            "mesh": {"inner_radius": 1e18, "outer_radius": 2e19, "inner_excision_speed": 0.0,
-                     "outer_excision_speed": 0.0, "reference_radius": 1e18, "num_polar_zones": 1,
-                     "num_radial_zones": 512, "block_size": 512},
-            "control": {"final_time": 1e9, "start_time": 0.0, "checkpoint_interval": 1e7, "output_directory": "data",
-                        "num_threads": None, "fold": 10}}
-SETUP_JIC = {"hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.5, "cfl_number": 0.4,
-                                        "runge_kutta_order": "RK2", "riemann_solver": "HLLC"}},
-             "model": {"jet_in_cloud": dict(zip(presets.MODELS["jet_in_cloud"][1], CLOUD))},
-             "mesh": {"inner_radius": 0.0, "outer_radius": 0.0, "inner_excision_speed": 3e8,
-                      "outer_excision_speed": 3e10, "reference_radius": 3e8, "num_polar_zones": 512,
-                      "block_size": 4},
-             "control": {"final_time": 5.0, "start_time": 1.0, "checkpoint_interval": 0.1, "fold": 100,
-                         "num_threads": None, "output_directory": "data"}}
-SETUP_HALO = {"hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.1, "cfl_number": 0.2,
-                                         "runge_kutta_order": "RK2", "riemann_solver": "HLLC",
-                                         "adaptive_time_step": True}},
-              "model": {"halo_kilonova": {"altitude": 1e20, "external_medium_density": 1e-23, "launch_radius": 1e18,
-                                          "shell_thickness": 1e18, "kinetic_energy": 1e51, "shell_mass": 2e32,
-                                          "radial_distance": 1e22}},
-              "mesh": {"inner_radius": 1e17, "outer_radius": 1e20, "inner_excision_speed": 0.0,
-                       "outer_excision_speed": 0.0, "reference_radius": 1e18, "num_polar_zones": 256,
-                       "block_size": 4},
-              "control": {"final_time": 1e10, "start_time": 0.0, "checkpoint_interval": 1e8, "output_directory": "data",
-                          "num_threads": 16, "fold": 10}}
-SETUP_WIND = {"hydro": {"relativistic": {"gamma_law_index": 1.333, "plm_theta": 1.5, "cfl_number": 0.1,
-                                         "runge_kutta_order": "RK2", "riemann_solver": "HLLC"}},
-              "model": {"wind_shock": dict(zip(presets.MODELS["wind_shock"][1], WIND))},
-              "mesh": {"inner_radius": 5e8, "outer_radius": 50e9, "inner_excision_speed": 0.0,
-                       "outer_excision_speed": 0.0, "reference_radius": 2e8, "num_polar_zones": 1,
-                       "num_radial_zones": 32, "block_size": 32},
-              "control": {"final_time": 8.0, "start_time": 0.0, "checkpoint_interval": 0.05, "fold": 10,
-                          "num_threads": 4, "output_directory": "data"}}
+# the reference's setups/*.yaml (knc.presets.SETUPS); jet_in_star's file
+# names eta_0 and lacks two fields, so its runnable form is restated here
+SETUP_KB = presets.SETUPS["kinetic_bomb"]
+SETUP_JIC = presets.SETUPS["jet_in_cloud"]
+SETUP_HALO = presets.SETUPS["halo_kilonova"]
+SETUP_WIND = presets.SETUPS["wind_shock"]
 SETUP_STAR = {"hydro": {"relativistic": {"gamma_law_index": 1.33, "plm_theta": 1.5, "cfl_number": 0.3,
                                          "runge_kutta_order": "RK2", "riemann_solver": "HLLC"}},
               "model": {"jet_in_star": dict(zip(presets.MODELS["jet_in_star"][1], STAR))},
