@@ -1162,8 +1162,8 @@ extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, in
     }
     HIP_TRY(hipMemcpy(b.rgeo, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(b.mem, 0, (size_t)NSLOT * 5 * n * sizeof(double)));
-    // all-ones words are NaN: no guess yet (the plain guess is used)
-    HIP_TRY(hipMemset(b.guess, 0xff, n * sizeof(float)));
+    // log2(p/D) = -inf: no guess yet (2^-inf = 0, the plain guess)
+    HIP_TRY(hipMemsetD32((hipDeviceptr_t)b.guess, 0xff800000u, n));
     if (conserved) {
         std::vector<double> soa(5 * n);
         for (size_t k = 0; k < n; ++k) {

@@ -238,11 +238,12 @@ __device__ inline float mlp_log2_guess_wave(const MlpDev& M, const MlpShared* sh
     return y / M.norm.tscale + M.norm.tshift;
 }
 
-// Pressure guess from its log2 form: p = D 2^l2; a NaN entry (a zone with
-// no guess yet) gives the plain guess p = 0.
+// Pressure guess from its log2 form: p = D 2^l2.  An entry of -inf (a zone
+// with no guess yet) gives the plain guess p = 0; a NaN (a failed network)
+// fails Newton, which then restarts from the plain guess and counts it.
 __device__ inline double mlp_guess_pressure(double D, float l2)
 {
-    return l2 == l2 ? D * (double)__builtin_amdgcn_exp2f(l2) : 0.0;
+    return D * (double)__builtin_amdgcn_exp2f(l2);
 }
 
 // The log2 form of a recovered pressure, log2(p/D) in FP32 (about 1e-7
