@@ -174,10 +174,13 @@ __device__ inline Cons p2c(const Prim& p, const ZoneParams& zp)
     return u;
 }
 
-// Conserved state, physical flux and outer wave speeds along one axis.
+// Conserved state, physical flux and outer wave speeds along one axis.  The
+// speeds also come as fractions lm = nm/den, lp = np/den with den > 0: the
+// HLLE flux combines them with a single division.
 struct FluxState {
     Cons u, f;
     double us, fs, vn, lm, lp, p;
+    double nm, np, den;
 };
 
 template <int SYS, int RS, int AXIS>
@@ -193,6 +196,9 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
         const double cs = sqrt(zp.gamma * P.p / P.rho);
         st.lm = st.vn - cs;
         st.lp = st.vn + cs;
+        st.nm = st.lm;
+        st.np = st.lp;
+        st.den = 1.0;
         st.f.d = st.u.d * st.vn;
         st.f.sr = st.u.sr * st.vn;
         st.f.sq = st.u.sq * st.vn;
@@ -208,14 +214,21 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
     // Outer characteristic speeds of Marti & Mueller (2003) eq. (72) with
     // numerator and denominator multiplied by W^2 rho h (gp = gamma p,
     // e = rho h - gp): lambda = (u_n W e -/+ sqrt(gp (rho h + u_t^2 e))) /
-    // (rho h + u^2 e), one division for both speeds.
+    // (rho h + u^2 e).  HLLE keeps them as fractions; HLLC divides once for
+    // both speeds.
     const double gp = zp.gamma * P.p;
     const double e = rhoh - gp;
-    const double rY = 1.0 / (rhoh + uu * e);
+    const double Y = rhoh + uu * e;
     const double k0 = sqrt(gp * (rhoh + (ut * ut) * e));
     const double b = (un * W) * e;
-    st.lm = (b - k0) * rY;
-    st.lp = (b + k0) * rY;
+    st.nm = b - k0;
+    st.np = b + k0;
+    st.den = Y;
+    if (RS == RS_HLLC) {
+        const double rY = 1.0 / Y;
+        st.lm = st.nm * rY;
+        st.lp = st.np * rY;
+    }
     st.vn = RS == RS_HLLC ? un / W : 0.0;
     st.us = s * W;
     // Fluxes with v_n = u_n/W cancelled against the W of U:
@@ -241,21 +254,29 @@ __device__ inline void riemann(const PS& Lq, const PS& Rq, const ZoneParams& zp,
 {
     const FluxState L = flux_state<SYS, RS, AXIS>(Lq.P, Lq.s, zp);
     const FluxState R = flux_state<SYS, RS, AXIS>(Rq.P, Rq.s, zp);
-    const double am = dmin(L.lm, R.lm);
-    const double ap = dmax(L.lp, R.lp);
 
     if (RS == RS_HLLE) {
-        const double a0 = dmin(am, 0.0);
-        const double a1 = dmax(ap, 0.0);
-        const double rd = 1.0 / (a1 - a0);
-        const double ab = a1 * a0;
-        F->d   = (L.f.d   * a1 - R.f.d   * a0 + (R.u.d   - L.u.d)   * ab) * rd;
-        F->sr  = (L.f.sr  * a1 - R.f.sr  * a0 + (R.u.sr  - L.u.sr)  * ab) * rd;
-        F->sq  = (L.f.sq  * a1 - R.f.sq  * a0 + (R.u.sq  - L.u.sq)  * ab) * rd;
-        F->tau = (L.f.tau * a1 - R.f.tau * a0 + (R.u.tau - L.u.tau) * ab) * rd;
-        *Fs    = (L.fs    * a1 - R.fs    * a0 + (R.us    - L.us)    * ab) * rd;
+        // a0 = min(lm_L, lm_R, 0) = n0/d0 and a1 = max(lp_L, lp_R, 0) = n1/d1,
+        // chosen by cross-multiplication (positive denominators); the flux
+        // (a1 F_L - a0 F_R + a1 a0 (U_R - U_L)) / (a1 - a0) is multiplied
+        // through by d0 d1, leaving one division.
+        const bool left_m = L.nm * R.den <= R.nm * L.den;
+        const double nm = left_m ? L.nm : R.nm, dm = left_m ? L.den : R.den;
+        const bool left_p = L.np * R.den >= R.np * L.den;
+        const double np = left_p ? L.np : R.np, dp = left_p ? L.den : R.den;
+        const double n0 = nm < 0.0 ? nm : 0.0, d0 = nm < 0.0 ? dm : 1.0;
+        const double n1 = np > 0.0 ? np : 0.0, d1 = np > 0.0 ? dp : 1.0;
+        const double c1 = n1 * d0, c0 = n0 * d1, ab = n1 * n0;
+        const double rd = 1.0 / (c1 - c0);
+        F->d   = (L.f.d   * c1 - R.f.d   * c0 + (R.u.d   - L.u.d)   * ab) * rd;
+        F->sr  = (L.f.sr  * c1 - R.f.sr  * c0 + (R.u.sr  - L.u.sr)  * ab) * rd;
+        F->sq  = (L.f.sq  * c1 - R.f.sq  * c0 + (R.u.sq  - L.u.sq)  * ab) * rd;
+        F->tau = (L.f.tau * c1 - R.f.tau * c0 + (R.u.tau - L.u.tau) * ab) * rd;
+        *Fs    = (L.fs    * c1 - R.fs    * c0 + (R.us    - L.us)    * ab) * rd;
         return;
     }
+    const double am = dmin(L.lm, R.lm);
+    const double ap = dmax(L.lp, R.lp);
     if (am >= 0.0) { *F = L.f; *Fs = L.fs; return; }
     if (ap <= 0.0) { *F = R.f; *Fs = R.fs; return; }
 

@@ -210,9 +210,12 @@ Cons p2c(const Hydro& h, const Prim& p)
 
 // Physical flux along axis (0 radial, 1 polar) plus the passive scalar:
 // U_s = s*W, F_s = U_s*v_n.
+// The outer speeds also as fractions lm = nm/den, lp = np/den (den > 0), the
+// form the HLLE flux combines with one division.
 struct FluxState {
     Cons u, f;
     double us, fs, vn, lm, lp, p, ut;
+    double nm, np, den;
 };
 
 FluxState flux_state(const Hydro& h, const Prim& P, double s, int axis)
@@ -229,6 +232,9 @@ FluxState flux_state(const Hydro& h, const Prim& P, double s, int axis)
         const double cs = std::sqrt(g * P.p / P.rho);
         st.lm = st.vn - cs;
         st.lp = st.vn + cs;
+        st.nm = st.lm;
+        st.np = st.lp;
+        st.den = 1.0;
         st.f.d = st.u.d * st.vn;
         st.f.sr = st.u.sr * st.vn;
         st.f.sq = st.u.sq * st.vn;
@@ -248,14 +254,18 @@ FluxState flux_state(const Hydro& h, const Prim& P, double s, int axis)
     //   root         W^2 rho h c_s sqrt((1-v^2)(1 - v^2 c_s^2 - v_n^2 (1 - c_s^2)))
     //                  = sqrt(gp (rho h + u_t^2 e)),
     //   first term   W^2 rho h v_n (1 - c_s^2) = u_n W e,
-    // so one division serves both speeds.
+    // so one division serves both speeds (HLLE keeps the fractions).
     const double gp = g * P.p;
     const double e = rhoh - gp;
-    const double rY = 1.0 / (rhoh + uu * e);
+    const double Y = rhoh + uu * e;
+    const double rY = 1.0 / Y;
     const double k0 = std::sqrt(gp * (rhoh + (ut * ut) * e));
     const double b = (un * W) * e;
-    st.lm = (b - k0) * rY;
-    st.lp = (b + k0) * rY;
+    st.nm = b - k0;
+    st.np = b + k0;
+    st.den = Y;
+    st.lm = st.nm * rY;
+    st.lp = st.np * rY;
     st.vn = un / W;   // HLLC only
     st.us = s * W;
     // Physical fluxes with v_n = u_n/W cancelled against the W of U:
@@ -282,21 +292,29 @@ void riemann(const Hydro& h, const Prim& PL, const Prim& PR, double sL, double s
 {
     const FluxState L = flux_state(h, PL, sL, axis);
     const FluxState R = flux_state(h, PR, sR, axis);
-    const double am = std::min(L.lm, R.lm);
-    const double ap = std::max(L.lp, R.lp);
 
     if (h.riemann_solver == HLLE) {
-        const double a0 = std::min(am, 0.0);
-        const double a1 = std::max(ap, 0.0);
-        const double rd = 1.0 / (a1 - a0);
-        const double ab = a1 * a0;
-        F->d   = (L.f.d   * a1 - R.f.d   * a0 + (R.u.d   - L.u.d)   * ab) * rd;
-        F->sr  = (L.f.sr  * a1 - R.f.sr  * a0 + (R.u.sr  - L.u.sr)  * ab) * rd;
-        F->sq  = (L.f.sq  * a1 - R.f.sq  * a0 + (R.u.sq  - L.u.sq)  * ab) * rd;
-        F->tau = (L.f.tau * a1 - R.f.tau * a0 + (R.u.tau - L.u.tau) * ab) * rd;
-        *Fs    = (L.fs    * a1 - R.fs    * a0 + (R.us    - L.us)    * ab) * rd;
+        // a0 = min(lm_L, lm_R, 0) = n0/d0, a1 = max(lp_L, lp_R, 0) = n1/d1,
+        // selected by cross-multiplication; the HLLE flux
+        // (a1 F_L - a0 F_R + a1 a0 (U_R - U_L)) / (a1 - a0) multiplied through
+        // by d0 d1.
+        const bool left_m = L.nm * R.den <= R.nm * L.den;
+        const double nm = left_m ? L.nm : R.nm, dm = left_m ? L.den : R.den;
+        const bool left_p = L.np * R.den >= R.np * L.den;
+        const double np = left_p ? L.np : R.np, dp = left_p ? L.den : R.den;
+        const double n0 = nm < 0.0 ? nm : 0.0, d0 = nm < 0.0 ? dm : 1.0;
+        const double n1 = np > 0.0 ? np : 0.0, d1 = np > 0.0 ? dp : 1.0;
+        const double c1 = n1 * d0, c0 = n0 * d1, ab = n1 * n0;
+        const double rd = 1.0 / (c1 - c0);
+        F->d   = (L.f.d   * c1 - R.f.d   * c0 + (R.u.d   - L.u.d)   * ab) * rd;
+        F->sr  = (L.f.sr  * c1 - R.f.sr  * c0 + (R.u.sr  - L.u.sr)  * ab) * rd;
+        F->sq  = (L.f.sq  * c1 - R.f.sq  * c0 + (R.u.sq  - L.u.sq)  * ab) * rd;
+        F->tau = (L.f.tau * c1 - R.f.tau * c0 + (R.u.tau - L.u.tau) * ab) * rd;
+        *Fs    = (L.fs    * c1 - R.fs    * c0 + (R.us    - L.us)    * ab) * rd;
         return;
     }
+    const double am = std::min(L.lm, R.lm);
+    const double ap = std::max(L.lp, R.lp);
     if (am >= 0.0) { *F = L.f; *Fs = L.fs; return; }
     if (ap <= 0.0) { *F = R.f; *Fs = R.fs; return; }
 
