@@ -4,8 +4,12 @@
 // (scheme::advance, src/scheme.rs:212-240) fold by fold, reporting FP64
 // zone-updates per second like the reference's Mzps (src/main.rs:42-48).
 //
-//   knc_drive [nr] [nq] [nblocks] [steps] [fold] [batched 0|1]
+//   knc_drive [nr] [nq] [nblocks] [steps] [fold] [batched 0|1] [checkpoint.cbor]
 //
+// With a checkpoint path it then writes the state as the reference's App
+// file (knc_write_checkpoint; empty configuration and tasks items), reads
+// it back (knc_checkpoint_open) and checks every block against the state
+// still resident on the device.
 // Workload: SRHD, gamma = 4/3, r in [1, 100] log-spaced, theta in [0, pi];
 // rho = r^-2 (1 + 0.1 cos 4 theta), radial four-velocity u = 1, p = 1e-2 rho;
 // a fixed inner inflow (u = 3) and an outflow outer boundary; HLLE, RK2,
@@ -14,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../include/knc_gpu.h"
@@ -40,6 +45,44 @@ int check(int rc, const char* what)
     return rc;
 }
 
+// The state as a checkpoint file, read back and compared block by block
+// with the device state.
+int write_and_verify(knc_context* ctx, const char* path, int nblocks, int nr, int nq)
+{
+    const uint8_t empty_map = 0xa0;   // CBOR {}
+    knc_record_meta meta{&empty_map, 1, &empty_map, 1, "knc_drive"};
+    if (check(knc_write_checkpoint(ctx, path, &meta), "knc_write_checkpoint")) return 1;
+    knc_checkpoint* ck = nullptr;
+    if (check(knc_checkpoint_open(path, &ck), "knc_checkpoint_open")) return 1;
+    knc_checkpoint_info info{};
+    int rc = check(knc_checkpoint_get_info(ck, &info), "knc_checkpoint_get_info");
+    double t = 0.0;
+    int64_t iteration = 0;
+    knc_get_time(ctx, &t, &iteration);
+    if (!rc && (info.num_blocks != nblocks || info.time != t || info.iteration_numer != iteration ||
+                info.iteration_denom != 1 || info.system != KNC_SRHD)) {
+        std::fprintf(stderr, "knc_drive: checkpoint header differs from the state\n");
+        rc = 1;
+    }
+    const size_t n = (size_t)nr * nq;
+    std::vector<double> c0(4 * n), s0(n), c1(4 * n), s1(n);
+    for (int k = 0; k < nblocks && !rc; ++k) {
+        int32_t index = -1, bnr = 0, bnq = 0;
+        rc = check(knc_checkpoint_block_shape(ck, k, &index, &bnr, &bnq), "knc_checkpoint_block_shape");
+        if (!rc && (index != k || bnr != nr || bnq != nq)) rc = 1;
+        if (!rc) rc = check(knc_checkpoint_block_data(ck, k, c1.data(), s1.data()), "knc_checkpoint_block_data");
+        if (!rc) rc = check(knc_block_download(ctx, k, c0.data(), s0.data()), "knc_block_download");
+        if (!rc && (std::memcmp(c0.data(), c1.data(), c0.size() * sizeof(double)) ||
+                    std::memcmp(s0.data(), s1.data(), s0.size() * sizeof(double)))) {
+            std::fprintf(stderr, "knc_drive: block %d of the checkpoint differs from the state\n", k);
+            rc = 1;
+        }
+    }
+    knc_checkpoint_close(ck);
+    if (!rc) std::printf("knc_drive: checkpoint %s written and read back, %d blocks identical\n", path, nblocks);
+    return rc;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -50,8 +93,10 @@ int main(int argc, char** argv)
     const int steps = argc > 4 ? std::atoi(argv[4]) : 100;
     const int fold = argc > 5 ? std::atoi(argv[5]) : 10;
     const int batched = argc > 6 ? std::atoi(argv[6]) : 1;
+    const char* checkpoint = argc > 7 ? argv[7] : nullptr;
     if (nr <= 0 || nq <= 0 || nblocks <= 0 || nr % nblocks || steps <= 0 || fold <= 0) {
-        std::fprintf(stderr, "usage: knc_drive [nr] [nq] [nblocks] [steps] [fold] [batched 0|1]\n");
+        std::fprintf(stderr,
+                     "usage: knc_drive [nr] [nq] [nblocks] [steps] [fold] [batched 0|1] [checkpoint.cbor]\n");
         return 2;
     }
     const double gamma = 4.0 / 3.0;
@@ -121,6 +166,7 @@ int main(int argc, char** argv)
                     "%.3f G zone-updates/s, t = %.6e s, iteration %lld\n",
                     nr, nq, nblocks, batched ? "batched" : "per-block", tr.stage_threads, tr.rows_per_item,
                     tr.stage_passes, steps, wall, (double)nr * nq * steps / wall / 1e9, t, (long long)iteration);
+    if (!rc && checkpoint) rc = write_and_verify(ctx, checkpoint, nblocks, nb, nq);
     knc_context_destroy(ctx);
     return rc ? 1 : 0;
 }

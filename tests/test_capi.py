@@ -87,3 +87,22 @@ def test_compiled_driver_advances_a_mesh():
     r = subprocess.run([DRIVER, "512", "128", "4", "40", "10", "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "G zone-updates/s" in r.stdout and "iteration 50" in r.stdout
+
+
+@pytest.mark.gpu
+def test_compiled_driver_writes_and_reads_a_checkpoint(tmp_path):
+    """The compiled caller writes the reference's checkpoint file through
+    the C ABI, reads it back and finds every block equal to the device
+    state; the independent Python decoder reads the same file."""
+    import subprocess
+
+    from knc import cbor
+    path = tmp_path / "chkpt.0000.cbor"
+    r = subprocess.run([DRIVER, "256", "64", "4", "20", "10", "1", str(path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "4 blocks identical" in r.stdout
+    doc = cbor.decode(path.read_bytes())
+    assert list(doc) == ["state", "tasks", "config", "version"] and doc["version"] == "knc_drive"
+    st = doc["state"]["Relativistic"]
+    assert st["iteration"] == [30, 1] and sorted(st["solution"]) == [(k, 0) for k in range(4)]
