@@ -323,9 +323,10 @@ void parse_block(cbor::Reader& r, knc_checkpoint::Block& b)
 
 void parse_state(cbor::Reader& r, knc_checkpoint& ck)
 {
-    // AnyState: {variant: State}
+    // AnyState: {variant: State}, a map of one entry (definite or not)
     const uint64_t n = r.open(cbor::MAP);
-    if (n != 1) throw cbor::Error("state is not a one-variant enum");
+    if (n != 1 && n != UINT64_MAX) throw cbor::Error("state is not a one-variant enum");
+    if (n == UINT64_MAX && r.at_break()) throw cbor::Error("state is not a one-variant enum");
     const std::string variant = r.text();
     if (variant == "Relativistic") ck.system = KNC_SRHD;
     else if (variant == "Newtonian") ck.system = KNC_EULER;
@@ -372,6 +373,7 @@ void parse_state(cbor::Reader& r, knc_checkpoint& ck)
         }
     }
     if (!have_t || !have_i || !have_s) throw cbor::Error("state lacks time, iteration or solution");
+    if (n == UINT64_MAX) r.take_break();
     std::sort(ck.blocks.begin(), ck.blocks.end(),
               [](const knc_checkpoint::Block& a, const knc_checkpoint::Block& b) { return a.index < b.index; });
     for (size_t k = 1; k < ck.blocks.size(); ++k)

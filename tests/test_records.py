@@ -218,8 +218,8 @@ def test_checkpoint_reader_indefinite_containers_and_newtonian(tmp_path):
          "conserved": {"data": cbor.Raw(b"\x9f" + b"".join(cbor.encode([float(v) for v in r]) for r in
                                                            cons.reshape(-1, 4)) + b"\xff"),
                        "dim": [2, 3], "v": 1}}) + b"\xff"
-    state = (b"\xa1" + cbor.encode("Newtonian") + b"\xa3" + cbor.encode("iteration") + cbor.encode([40, 1]) +
-             cbor.encode("solution") + sol + cbor.encode("time") + b"\xc1" + cbor.encode(st["time"]))
+    state = (b"\xbf" + cbor.encode("Newtonian") + b"\xa3" + cbor.encode("iteration") + cbor.encode([40, 1]) +
+             cbor.encode("solution") + sol + cbor.encode("time") + b"\xc1" + cbor.encode(st["time"]) + b"\xff")
     doc = (head + parts[0] + parts[1] + cbor.encode("state") + state + cbor.encode("tasks") +
            cbor.encode(body["tasks"]) + cbor.encode("config") + cbor.encode(body["config"]) +
            cbor.encode("version") + b"\x7f" + cbor.encode("v0.1")[0:0] + b"\x62v0\x62.1" + b"\xff" + b"\xff")
