@@ -670,4 +670,30 @@ int knc_checkpoint_load(const knc_checkpoint* ck, knc_context* c, const knc_mesh
 
 void knc_checkpoint_close(knc_checkpoint* ck) { delete ck; }
 
+int knc_cbor_read_f64(const uint8_t* data, uint64_t len, uint64_t offset, uint64_t count, int32_t arity,
+                      double* out, uint64_t* end)
+{
+    if (!data || !out || arity < 1 || offset > len) return fail(KNC_ERR_INVALID_ARGUMENT, "bad argument");
+    try {
+        cbor::Reader r(data, (size_t)len);
+        r.seek((size_t)offset);
+        if (arity == 1) {
+            r.f64_array(count, out, 1);
+        } else {
+            const uint64_t n = r.open(cbor::ARRAY);
+            uint64_t k = 0;
+            for (; r.more(n, k); ++k) {
+                if (k >= count) throw cbor::Error("array longer than expected");
+                r.f64_array((uint64_t)arity, out + k * (uint64_t)arity, 1);
+            }
+            if (k != count) throw cbor::Error("array shorter than expected");
+        }
+        if (end) *end = r.offset();
+    } catch (const cbor::Error& e) {
+        return fail(KNC_ERR_FORMAT, "not an array of %llu numbers x %d: %s", (unsigned long long)count, arity,
+                    e.what());
+    }
+    return KNC_OK;
+}
+
 }  // extern "C"

@@ -12,8 +12,10 @@ already CBOR.  :func:`decode` reads any well-formed CBOR: arrays become
 lists, maps dicts (array keys become tuples), tags are dropped.
 
 The block data of a checkpoint or products file is encoded and decoded by
-the library (``knc_write_checkpoint``, ``knc_checkpoint_open``); this module
-serves the Configuration and Tasks items and the tests.
+the library (``knc_write_checkpoint``, ``knc_checkpoint_open``).  This
+module serves the Configuration and Tasks items, the tests and
+:mod:`knc.records`, for which ``decode(..., arrays=True)`` returns long
+arrays of numbers as numpy arrays (``knc_cbor_read_f64``).
 """
 import math
 import struct
@@ -115,10 +117,50 @@ def _half(h):
     return -v if s else v
 
 
+def _native_f64(data, offset, count, arity):
+    """The array at data[offset] of `count` numbers (or arity-tuples of
+    numbers) through the library's decoder, as float64 of shape (count,) or
+    (count, arity) and the offset past it; None where the library is absent
+    or the array is not of that form."""
+    try:
+        import ctypes
+
+        import numpy as np
+
+        from . import capi
+        L = capi.lib()
+    except (ImportError, RuntimeError, OSError):
+        return None
+    fn = L.knc_cbor_read_f64
+    fn.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int32,
+                   ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    out = np.empty(count * arity)
+    end = ctypes.c_uint64()
+    if fn(data, len(data), offset, count, arity, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(end)):
+        return None
+    return (out if arity == 1 else out.reshape(count, arity)), end.value
+
+
+_NUMBER_HEADS = set(range(0x00, 0x1c)) | set(range(0x20, 0x3c)) | {0xf9, 0xfa, 0xfb}
+
+
 class _Reader:
-    def __init__(self, data, pos=0):
+    def __init__(self, data, pos=0, arrays=False):
         self.d = memoryview(data)
         self.p = pos
+        self.raw = bytes(data) if arrays else None
+
+    def _arity(self):
+        """1 if the next item is a number, k if it is an array of k <= 8
+        numbers, else 0."""
+        if self.p >= len(self.d):
+            return 0
+        b = self.d[self.p]
+        if b in _NUMBER_HEADS:
+            return 1
+        if 0x81 <= b <= 0x88 and self.p + 1 < len(self.d) and self.d[self.p + 1] in _NUMBER_HEADS:
+            return b - 0x80
+        return 0
 
     def byte(self):
         if self.p >= len(self.d):
@@ -148,7 +190,14 @@ class _Reader:
     def item(self, depth=0):
         if depth > 256:
             raise CBORError("nesting too deep")
+        start = self.p
         major, minor, arg = self.head()
+        if major == 4 and self.raw is not None and arg is not None and arg >= 64:
+            arity = self._arity()
+            got = _native_f64(self.raw, start, arg, arity) if arity else None
+            if got is not None:
+                self.p = got[1]
+                return got[0]
         if arg is None and major in (0, 1, 6):
             raise CBORError("indefinite length on a non-container")
         if major == 0:
@@ -218,9 +267,11 @@ def _freeze(x):
     return tuple(_freeze(v) for v in x) if isinstance(x, list) else x
 
 
-def decode(data, allow_trailing=False):
-    """Decodes one CBOR item from data (bytes-like)."""
-    r = _Reader(data)
+def decode(data, allow_trailing=False, arrays=False):
+    """Decodes one CBOR item from data (bytes-like).  With arrays=True,
+    arrays of 64 or more numbers (or of equal small tuples of numbers) come
+    back as float64 numpy arrays, decoded by the library when it is built."""
+    r = _Reader(data, arrays=arrays)
     obj = r.item()
     if not allow_trailing and r.p != len(r.d):
         raise CBORError("trailing bytes")

@@ -220,6 +220,10 @@ def io_rates(ctx, zones):
             t2 = time.perf_counter()
         out["checkpoint"].update({"open_s": t1 - t0, "decode_s": t2 - t1,
                                   "read_MB_per_s": out["checkpoint"]["file_MB"] / (t2 - t0)})
+        from knc import records
+        t0 = time.perf_counter()
+        records.read_products(os.path.join(d, "products.cbor"))
+        out["products"]["numpy_read_s"] = time.perf_counter() - t0
     log(f"bench: io checkpoint {out['checkpoint']['file_MB']:.0f} MB written in {out['checkpoint']['write_s']:.3f} s,"
         f" read in {out['checkpoint']['open_s'] + out['checkpoint']['decode_s']:.3f} s; products "
         f"{out['products']['file_MB']:.0f} MB in {out['products']['write_s']:.3f} s")

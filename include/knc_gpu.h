@@ -371,6 +371,15 @@ int knc_checkpoint_block_data(const knc_checkpoint* ck, int32_t k, double* conse
 int knc_checkpoint_load(const knc_checkpoint* ck, knc_context* ctx, const knc_mesh* mesh);
 void knc_checkpoint_close(knc_checkpoint* ck);
 
+/* Decoding helper for readers of these files (no counterpart in the
+ * reference; host only, no context): the CBOR array at data[offset] of
+ * exactly `count` items, each a number (arity 1) or an array of `arity`
+ * numbers, any float width or integer, into out[count * arity]; *end
+ * receives the offset just past the array.  KNC_ERR_FORMAT if the item is
+ * not such an array. */
+int knc_cbor_read_f64(const uint8_t* data, uint64_t len, uint64_t offset, uint64_t count, int32_t arity,
+                      double* out, uint64_t* end);
+
 /* ---- autotuner (no counterpart in the reference) ------------------------ */
 /* Times the stage kernels of the context's current mesh for each supported
  * workgroup shape (polar columns per work item), radial rows per work item

@@ -267,3 +267,49 @@ def test_checkpoint_reader_rejects_malformed_files(tmp_path, mutate):
 def test_checkpoint_reader_missing_file(tmp_path):
     with pytest.raises(RuntimeError, match="knc error 15"):
         capi.Checkpoint(tmp_path / "none.cbor")
+
+
+def test_array_fast_path_and_records_reader(tmp_path):
+    """decode(arrays=True) hands long number arrays (and arrays of number
+    tuples) to the library's decoder, in any float width; the result equals
+    the pure-Python decode.  knc.records reads Products and App files into
+    numpy with the reference's layout."""
+    from knc import records
+    rng = np.random.default_rng(4)
+    nr, nq = 40, 8
+    prim = np.round(rng.uniform(-4, 4, (nr, nq, 4)) * 256) / 256
+    prim[:, :, 1] = 0.0                                      # u_theta = 0 -> binary16 zeros among binary64s
+    prim[::3, :, 3] = rng.uniform(0, 1, (len(range(0, nr, 3)), nq))
+    scal = rng.uniform(0, 1, (nr, nq))
+    rv = np.geomspace(1.0, 2.0, nr + 1)
+    qv = np.linspace(0.0, np.pi, nq + 1)
+    cfg = _preset("kinetic_bomb").to_serde()
+    doc = {"time": 3.25, "blocks": {(2, 0): {"radial_vertices": _ndarray((nr + 1,), rv.tolist()),
+                                             "polar_vertices": _ndarray((nq + 1,), qv.tolist()),
+                                             "primitive": _ndarray((nr, nq), prim.reshape(-1, 4).tolist()),
+                                             "scalar": _ndarray((nr, nq), scal.reshape(-1).tolist())}},
+           "config": cfg, "version": "v0.1.2 test"}
+    raw = cbor.encode(doc)
+    fast = cbor.decode(raw, arrays=True)
+    slow = cbor.decode(raw)
+    blk = fast["blocks"][(2, 0)]
+    assert isinstance(blk["primitive"]["data"], np.ndarray) and blk["primitive"]["data"].shape == (nr * nq, 4)
+    assert np.array_equal(blk["primitive"]["data"], np.asarray(slow["blocks"][(2, 0)]["primitive"]["data"]))
+    assert np.array_equal(blk["scalar"]["data"], np.asarray(slow["blocks"][(2, 0)]["scalar"]["data"]))
+    path = tmp_path / "prods.0000.cbor"
+    path.write_bytes(raw)
+    p = records.read_products(path)
+    b = p["blocks"][(2, 0)]
+    assert p["time"] == 3.25 and p["config"] == cfg
+    assert np.array_equal(b["primitive"], prim) and np.array_equal(b["scalar"], scal)
+    assert np.array_equal(b["radial_vertices"], rv) and np.array_equal(b["polar_vertices"], qv)
+    assert np.array_equal(records.radial_profile(p, 3, "gas_pressure"), prim[:, 3, 3])
+    assert np.array_equal(records.radial_profile(p, 0, "vertices"), rv[:-1])
+    # a checkpoint, in every float width
+    blocks = _blocks(5, {0: (16, 8), 1: (16, 8)})
+    ck = tmp_path / "chkpt.0000.cbor"
+    ck.write_bytes(_app_item(blocks, widths=(16, 32, 64)))
+    a = records.read_app(ck)
+    assert a["system"] == "Relativistic" and a["iteration"] == (7, 1) and a["time"] == 2.5
+    for k, (c, m) in blocks.items():
+        assert np.array_equal(a["blocks"][(k, 0)][0], c) and np.array_equal(a["blocks"][(k, 0)][1], m)
