@@ -59,3 +59,12 @@ def test_wind_shock_full_size_excision_every_50_steps():
     assert ctx.excision_counts() == (1, 1)
     err = rel_l1(gpu_state(ctx), sim.blocks())
     assert max(err.values()) <= MANY_STEPS, err
+
+
+def test_jet_in_cloud_hllc_1000_steps_full_size():
+    """The same mesh with HLLC, the solver of every reference setup
+    (setups/*.yaml), for 1000 steps with the inner excision surface."""
+    s = knc.setups.jet_in_cloud(riemann=knc.HLLC)
+    ctx, sim = _pair(s, [10] * 100)
+    err = rel_l1(gpu_state(ctx), sim.blocks())
+    assert max(err.values()) <= MANY_STEPS, err
