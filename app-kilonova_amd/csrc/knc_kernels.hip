@@ -237,7 +237,9 @@ __device__ inline Flux5 scaled(const Cons& F, double Fs, double c, double A)
 // not span every column, its halo columns j0-2, j0-1 and j1-2 .. j1+1 are
 // recovered once per item for all its rows, and the fluxes through its right
 // polar face j1, into LDS before the march; each zone row then costs two
-// workgroup barriers (primitive exchange, polar flux exchange).
+// workgroup barriers (primitive exchange, polar flux exchange).  The
+// primitive rows alternate between two LDS buffers, so a row's writes need
+// no barrier of their own: the buffer was last read two rows earlier.
 constexpr int MAX_ROWS = 64;
 constexpr int NHALO = 6;   // j0-2, j0-1 | j1-2, j1-1, j1, j1+1
 
@@ -254,7 +256,7 @@ template <int SYS, int RS, bool STATS, int NT, bool GUESS, bool PRIMS>
 __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 {
     constexpr int WP = NT + 3;                 // primitive row: 2 left halo, NT own, 1 right halo
-    __shared__ double ldsP[5 * WP];
+    __shared__ double ldsPP[2][5 * WP];      // primitive rows, alternating by row parity
     __shared__ double ldsF[5 * NT];
     __shared__ double ldsH[NHALO * MAX_ROWS * 5];
     __shared__ double ldsR[MAX_ROWS * 5];      // flux through face j1 per row
@@ -381,7 +383,9 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                 // Zone row r = k-2: polar slopes and fluxes (scheme.rs:107-128).
                 const int r = k - 2, rr = r - it.i0;
                 const PS& P = Pm2;
-                __syncthreads();
+                // Row r's buffer was last read by row r-2, before the flux
+                // barrier of row r-1 that every thread has since passed.
+                double* ldsP = ldsPP[r & 1];
                 lds_put(ldsP, WP, tid + 2, P);
                 if (tid < 2 && left_halo) {
                     const double* d = ldsH + (tid * MAX_ROWS + rr) * 5;
