@@ -130,8 +130,10 @@ class App:
         return cbor.encode(self.preset.to_serde())
 
     def write_checkpoint(self, path):
-        """App::package + io::write_cbor."""
-        self.ctx.write_checkpoint(path, self.config_cbor(), cbor.encode(self.tasks.to_serde()), self.version)
+        """App::package + io::write_cbor; like App::package
+        (src/app.rs:368-382) it stamps the running code's version, also after
+        a restart from another version's checkpoint."""
+        self.ctx.write_checkpoint(path, self.config_cbor(), cbor.encode(self.tasks.to_serde()), VERSION)
 
     def write_products(self, path):
         """Products::try_from_state + io::write_cbor."""
