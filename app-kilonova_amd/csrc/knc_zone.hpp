@@ -49,6 +49,33 @@ __host__ __device__ inline double dmin(double a, double b) { return (b < a) ? b 
 __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded FP64 square root.  For x in [2^-767, DBL_MAX] it is the
+// coupled Newton iteration for sqrt(x) and 1/(2 sqrt(x)) (Goldschmidt 1964;
+// Markstein, IBM J. Res. Dev. 34 (1990) 111) started from the hardware's
+// v_rsq_f64: one joint correction g, h <- g + g r, h + h r with
+// r = 1/2 - h g, then two corrections of the root alone, g <- g + (x - g^2) h,
+// the last of which rounds once.  This is the operation sequence the
+// compiler's sqrt performs on such x; its range scaling (x < 2^-767), the
+// zero / infinity pass-through and their selects (seven of its seventeen
+// VALU instructions) are taken only by the lanes outside that range.
+// Every result is the host's sqrt to the bit (test_fp64_arithmetic_rounds_like_host).
+__device__ inline double sqrt_rn(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    if (__builtin_expect(!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023), 0)) g = __builtin_sqrt(x);
+    return g;
+}
+
+// ---------------------------------------------------------------------------
 // c2p, SRHD: Newton-Raphson on p (Marti & Mueller, Living Rev. Relativ. 6
 // (2003) 7, sec. 2.3).  Q = tau + D + p, den = Q^2 - S^2:
 //   rho h = den/Q,  rho = D sqrt(den)/Q,  v^2 = S^2/Q^2,
@@ -73,7 +100,7 @@ __device__ inline double c2p_srhd_newton_step(double D, double S2, double E, con
     const double Q = E + p;
     const double den = Q * Q - S2;
     if (!(den > 0.0)) return p;
-    const double sq = sqrt(den);
+    const double sq = sqrt_rn(den);
     const double pQ = p * Q;
     const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
     return p - F * den / (zp.gamma * p * S2 - den * Q);
@@ -99,7 +126,7 @@ __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const Zon
         const double Q = E + p;
         const double den = Q * Q - S2;
         if (!(den > 0.0)) break;
-        const double sq = sqrt(den);
+        const double sq = sqrt_rn(den);
         const double pQ = p * Q;
         const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
         const double dp = F * den / (zp.gamma * p * S2 - den * Q);
@@ -126,7 +153,7 @@ __device__ inline int c2p_srhd_finish(const Cons& u, double S2, double p, Prim* 
 {
     const double Q = (u.tau + u.d) + p;
     const double den = Q * Q - S2;
-    const double sq = sqrt(den);
+    const double sq = sqrt_rn(den);
     const double r = 1.0 / (Q * sq);
     const double rsq = Q * r;
     *rW = den * r;
@@ -153,6 +180,20 @@ __device__ inline int c2p_euler(const Cons& u, const ZoneParams& zp, Prim* out)
     return E_OK;
 }
 
+// SRHD conserved state from the primitive and its Lorentz factor
+// W = sqrt(1 + u_r^2 + u_q^2).
+__device__ inline Cons p2c_srhd(const Prim& p, double W, const ZoneParams& zp)
+{
+    Cons u;
+    const double rhoh = p.rho + p.p * zp.gfac;
+    const double hw = rhoh * W;
+    u.d = p.rho * W;
+    u.sr = hw * p.ur;
+    u.sq = hw * p.uq;
+    u.tau = (hw * W - p.p) - u.d;
+    return u;
+}
+
 template <int SYS>
 __device__ inline Cons p2c(const Prim& p, const ZoneParams& zp)
 {
@@ -163,13 +204,7 @@ __device__ inline Cons p2c(const Prim& p, const ZoneParams& zp)
         u.sq = p.rho * p.uq;
         u.tau = p.p / zp.gm1 + (0.5 * p.rho) * (p.ur * p.ur + p.uq * p.uq);
     } else {
-        const double W = sqrt(1.0 + (p.ur * p.ur + p.uq * p.uq));
-        const double rhoh = p.rho + p.p * zp.gfac;
-        const double hw = rhoh * W;
-        u.d = p.rho * W;
-        u.sr = hw * p.ur;
-        u.sq = hw * p.uq;
-        u.tau = (hw * W - p.p) - u.d;
+        u = p2c_srhd(p, sqrt_rn(1.0 + (p.ur * p.ur + p.uq * p.uq)), zp);
     }
     return u;
 }
@@ -187,13 +222,13 @@ template <int SYS, int RS, int AXIS>
 __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams& zp)
 {
     FluxState st;
-    st.u = p2c<SYS>(P, zp);
     const double un = AXIS == 0 ? P.ur : P.uq;
     st.p = P.p;
     if (SYS == SYS_EULER) {
+        st.u = p2c<SYS>(P, zp);
         st.vn = un;
         st.us = s;
-        const double cs = sqrt(zp.gamma * P.p / P.rho);
+        const double cs = sqrt_rn(zp.gamma * P.p / P.rho);
         st.lm = st.vn - cs;
         st.lp = st.vn + cs;
         st.nm = st.lm;
@@ -208,7 +243,8 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
         return st;
     }
     const double uu = P.ur * P.ur + P.uq * P.uq;
-    const double W = sqrt(1.0 + uu);
+    const double W = sqrt_rn(1.0 + uu);
+    st.u = p2c_srhd(P, W, zp);
     const double rhoh = P.rho + P.p * zp.gfac;
     const double ut = AXIS == 0 ? P.uq : P.ur;
     // Outer characteristic speeds of Marti & Mueller (2003) eq. (72) with
@@ -219,7 +255,7 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
     const double gp = zp.gamma * P.p;
     const double e = rhoh - gp;
     const double Y = rhoh + uu * e;
-    const double k0 = sqrt(gp * (rhoh + (ut * ut) * e));
+    const double k0 = sqrt_rn(gp * (rhoh + (ut * ut) * e));
     const double b = (un * W) * e;
     st.nm = b - k0;
     st.np = b + k0;
@@ -295,7 +331,7 @@ __device__ inline void riemann(const PS& Lq, const PS& Rq, const ZoneParams& zp,
         const double FEhll = ft + fd;
         const double b = Ehll + fm;
         const double disc = b * b - 4.0 * FEhll * um;
-        lstar = 2.0 * um / (b + sqrt(disc));
+        lstar = 2.0 * um / (b + sqrt_rn(disc));
         pstar = fm - FEhll * lstar;
     } else {
         const double dl = L.u.d * (am - L.vn);
@@ -354,11 +390,11 @@ template <int SYS>
 __device__ inline double max_signal_speed(const Prim& P, const ZoneParams& zp)
 {
     if (SYS == SYS_EULER)
-        return sqrt(P.ur * P.ur + P.uq * P.uq) + sqrt(zp.gamma * P.p / P.rho);
+        return sqrt_rn(P.ur * P.ur + P.uq * P.uq) + sqrt_rn(zp.gamma * P.p / P.rho);
     const double uu = P.ur * P.ur + P.uq * P.uq;
     const double rhoh = P.rho + P.p * zp.gfac;
-    const double cs = sqrt(zp.gamma * P.p / rhoh);
-    const double v = sqrt(uu / (1.0 + uu));
+    const double cs = sqrt_rn(zp.gamma * P.p / rhoh);
+    const double v = sqrt_rn(uu / (1.0 + uu));
     return (v + cs) / (1.0 + v * cs);
 }
 

@@ -16,17 +16,51 @@ ONE_STEP = 1e-12
 MANY_STEPS = 1e-9
 
 
+def _near_midpoint_squares(rng, k):
+    """Doubles x whose square root lies within 2^-40 ulp of a rounding
+    midpoint -- the hardest inputs to round -- and their neighbours.  A
+    midpoint between consecutive doubles is m = t 2^E with t odd and of 54
+    bits; RN(m^2) is that close to m^2 when t^2 is congruent to a small c
+    modulo 2^(bits of t^2 - 53), so t is a square root of c there, found by
+    Hensel lifting (c = 1 mod 8; c < 0 too)."""
+    def roots(c, bits):
+        ts = {1, 3, 5, 7}
+        for j in range(3, bits):
+            mod = 1 << (j + 1)
+            ts = {t if (t * t - c) % mod == 0 else t + (1 << (j - 1)) for t in ts}
+        mod = 1 << bits
+        ts = {t % mod for t in ts} | {(-t) % mod for t in ts}
+        # the 54-bit representatives t + j 2^bits in [2^53, 2^54)
+        return {t + j * mod for t in ts for j in range(4) if (t + j * mod) >> 53 == 1}
+    out = []
+    for c in list(range(1, 8 * k, 8)) + list(range(-7, -8 * k, -8)):
+        for bits in (53, 55):
+            for t in roots(c, bits):
+                if (t * t).bit_length() - 53 != bits:
+                    continue
+                e = int(rng.integers(-436, 458))
+                x = float(t * t) * 2.0 ** (2 * e) if e >= 0 else float(t * t) / 2.0 ** (-2 * e)
+                out += [np.nextafter(x, 0), x, np.nextafter(x, np.inf)]
+    return np.array(out)
+
+
 def test_fp64_arithmetic_rounds_like_host():
-    """Device division, square root and unfused multiply-add round like the
-    host's, over the whole exponent range and at the edges of the device
-    square root's range scaling (2^-767, subnormals, 0, inf)."""
+    """Device division, the stage's square root and unfused multiply-add
+    round like the host's: over the whole exponent range, at the edges of the
+    square root's fast range (2^-767 and DBL_MAX) and of the compiler's range
+    scaling (subnormals, 0, inf, NaN), on 2^20 random normal inputs and on
+    inputs whose root is next to a rounding midpoint."""
     rng = np.random.default_rng(0)
-    n = 1 << 18
+    n = 1 << 20
     a = rng.standard_normal(n) * np.exp(rng.uniform(-300, 300, n) / 10)
     a[: n // 4] = np.ldexp(rng.uniform(1.0, 2.0, n // 4), rng.integers(-1074, 1024, n // 4))
+    a[n // 4: n // 2] = np.ldexp(rng.uniform(1.0, 2.0, n // 4), rng.integers(-767, 1024, n // 4))
     edges = np.array([0.0, -0.0, 5e-324, 2.0 ** -1022, 2.0 ** -767, np.nextafter(2.0 ** -767, 0),
-                      np.nextafter(2.0 ** -767, 1), 1.0, np.nextafter(1.0, 2), 1.7976931348623157e308, np.inf])
-    a[n // 4: n // 4 + edges.size] = edges
+                      np.nextafter(2.0 ** -767, 1), 1.0, np.nextafter(1.0, 2), 1.7976931348623157e308,
+                      np.nextafter(1.7976931348623157e308, 0), np.inf, -np.inf, np.nan, -1.0, -5e-324])
+    a[n // 2: n // 2 + edges.size] = edges
+    hard = _near_midpoint_squares(rng, 1000)
+    a[n // 2 + 64: n // 2 + 64 + hard.size] = hard
     b = rng.standard_normal(n) * np.exp(rng.uniform(-300, 300, n) / 10)
     c = rng.standard_normal(n)
     with np.errstate(all="ignore"):
