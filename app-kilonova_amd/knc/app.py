@@ -25,7 +25,7 @@ import yaml
 from . import capi, cbor, presets
 
 DESCRIPTION = "Clemson Kilonova Code"          # the crate's description (Cargo.toml)
-VERSION = "v0.1.2 knc-mi355x"                   # the crate version this build mirrors, and its name
+VERSION = "v0.1.2 knc-mi355x"                   # the crate version this package mirrors, and its name
 TASK_NAMES = ("write_checkpoint", "write_products", "iteration_message", "report_progress")
 
 
