@@ -589,12 +589,11 @@ __device__ inline GuessIn guess_load(const StageArgs& a, const GuessStep& p)
     return x;
 }
 
-// Four wavefronts per SIMD (at most 128 VGPRs; the pass needs 110) from
-// 128-thread workgroups up: left alone the compiler settles at three, and
-// the extra wavefront hides more of the MFMA -> tanh -> MFMA dependency chain
-// (122 vs 146 us per pass on bomb_jet 4096x1024; five or six wavefronts gain
-// nothing more).  A 64-thread workgroup is held to two per SIMD by its
-// 20 KiB of LDS anyway.
+// At least four wavefronts per SIMD (at most 128 VGPRs) from 128-thread
+// workgroups up: a fourth wavefront hides more of the MFMA -> tanh -> MFMA
+// dependency chain than three (122 vs 146 us per pass on bomb_jet
+// 4096x1024).  The pass now needs 90 VGPRs and runs five.  A 64-thread
+// workgroup is held to two per SIMD by its 20 KiB of LDS anyway.
 template <int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 128 ? 4 : 1))) k_guess(StageArgs a)
 {

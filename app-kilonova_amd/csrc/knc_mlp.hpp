@@ -107,26 +107,38 @@ __device__ inline void mlp_shared_load(MlpShared* sh, const MlpDev& M)
     __syncthreads();
 }
 
-// tanh of two FP32 values as a packed BF16 pair (a in the low half).
-__device__ inline uint32_t tanh_pair(const uint16_t* tab, float a, float b)
+// Table indexes of two FP32 values, packed as two 16-bit halves (a low).
+__device__ inline uint32_t tanh_index_pair(float a, float b)
 {
     const half2_t x = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(a, b));
     half2_t m = __builtin_elementwise_fma(x, (half2_t)(_Float16)0x1p-16f, (half2_t)(_Float16)0x1p-14f);
     m = __builtin_elementwise_max(m, (half2_t)(_Float16)0.0f);
     m = __builtin_elementwise_min(m, (half2_t)(_Float16)(2047 * 0x1p-24f));
-    const uint32_t bits = __builtin_bit_cast(uint32_t, m);
-    short2_t r;
-    r.x = (short)tab[bits & 0xffffu];
-    r.y = (short)tab[bits >> 16];
-    return __builtin_bit_cast(uint32_t, r);
+    return __builtin_bit_cast(uint32_t, m);
 }
 
-// Eight accumulator values -> an MFMA operand of their tanh.
+// Eight accumulator values -> an MFMA operand of their tanh.  The four
+// index pairs are formed first and the eight table reads issued together,
+// so that one LDS round trip covers all eight.
 __device__ inline bf16x8 tanh_operand(const uint16_t* tab, const float* v)
 {
+    uint32_t bits[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bits[q] = tanh_index_pair(v[2 * q], v[2 * q + 1]);
+    uint16_t lo[4], hi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        lo[q] = tab[bits[q] & 0xffffu];
+        hi[q] = tab[bits[q] >> 16];
+    }
     u32x4 w;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) w[q] = tanh_pair(tab, v[2 * q], v[2 * q + 1]);
+    for (int q = 0; q < 4; ++q) {
+        short2_t r;
+        r.x = (short)lo[q];
+        r.y = (short)hi[q];
+        w[q] = __builtin_bit_cast(uint32_t, r);
+    }
     return __builtin_bit_cast(bf16x8, w);
 }
 
