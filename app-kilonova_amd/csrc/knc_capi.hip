@@ -1673,6 +1673,36 @@ int knc_hydro_plm_gradient(double theta, int64_t n, const double* a, const doubl
     return KNC_OK;
 }
 
+int knc_fp64_fma_rate(int chains, int waves_per_simd, int iters, double* fma_per_s, float* ms)
+{
+    if (!(chains == 1 || chains == 2 || chains == 4 || chains == 8) || waves_per_simd < 1 || waves_per_simd > 8 ||
+        iters < 1 || iters > (1 << 20) || !fma_per_s)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    int dev = 0, ncu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // 256-thread workgroups (a wave on each SIMD), waves_per_simd of them per CU
+    const int grid = ncu * waves_per_simd;
+    DevBuf out;
+    ZONE_ALLOC(out, (size_t)grid * 256 * 8);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    if (err == hipSuccess) err = launch_fma_probe(chains, iters, grid, (double*)out.p, nullptr);   // warm-up
+    if (err == hipSuccess) err = hipEventRecord(e0, nullptr);
+    if (err == hipSuccess) err = launch_fma_probe(chains, iters, grid, (double*)out.p, nullptr);
+    if (err == hipSuccess) err = hipEventRecord(e1, nullptr);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    HIP_TRY(err);
+    *fma_per_s = (double)grid * 256 * chains * iters / (t * 1e-3);
+    if (ms) *ms = t;
+    return KNC_OK;
+}
+
 int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
                    double* mad)
 {

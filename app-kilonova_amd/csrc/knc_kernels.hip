@@ -783,6 +783,28 @@ __global__ void k_fp64_probe(int64_t n, const double* a, const double* b, const 
     }
 }
 
+// FP64 FMA issue-rate probe: every lane runs CH independent chains
+// x <- fma(x, b, c) for iters steps (one v_fma_f64 each); the chains' sum
+// is stored so that no step is dead.  With one chain a wave issues only
+// dependent FMAs, so the rate shows what latency leaves of the issue peak at
+// a given number of waves per SIMD.
+template <int CH>
+__global__ void __launch_bounds__(256) k_fma_probe(int iters, double b, double c, double* out)
+{
+    double x[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) x[i] = (double)(threadIdx.x + 64 * i) * 1e-3;
+#pragma unroll 4
+    for (int n = 0; n < iters; ++n) {
+#pragma unroll
+        for (int i = 0; i < CH; ++i) x[i] = fma(x[i], b, c);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) s += x[i];
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Launch wrappers.
 // ---------------------------------------------------------------------------
@@ -910,6 +932,19 @@ hipError_t launch_guess(const StageArgs& a, int grid, int threads, hipStream_t s
     case 128: hipLaunchKernelGGL(k_guess<128>, g, dim3(128), 0, s, a); break;
     case 256: hipLaunchKernelGGL(k_guess<256>, g, dim3(256), 0, s, a); break;
     case 512: hipLaunchKernelGGL(k_guess<512>, g, dim3(512), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fma_probe(int chains, int iters, int grid, double* out, hipStream_t s)
+{
+    const double b = 0.999999, c = 1e-6;
+    switch (chains) {
+    case 1: hipLaunchKernelGGL(k_fma_probe<1>, dim3(grid), dim3(256), 0, s, iters, b, c, out); break;
+    case 2: hipLaunchKernelGGL(k_fma_probe<2>, dim3(grid), dim3(256), 0, s, iters, b, c, out); break;
+    case 4: hipLaunchKernelGGL(k_fma_probe<4>, dim3(grid), dim3(256), 0, s, iters, b, c, out); break;
+    case 8: hipLaunchKernelGGL(k_fma_probe<8>, dim3(grid), dim3(256), 0, s, iters, b, c, out); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -110,6 +110,7 @@ hipError_t launch_block_from_model(int system, const ModelDev& m, const ZonePara
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
                              const double* qc, const double* dcost, const double* rdcost, double t, int step,
                              int tag0, double* out, ErrLog* err, hipStream_t s);
+hipError_t launch_fma_probe(int chains, int iters, int grid, double* out, hipStream_t s);
 hipError_t launch_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
                              double* mad, hipStream_t s);
 

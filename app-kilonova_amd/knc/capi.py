@@ -222,6 +222,7 @@ def lib():
     L.knc_hydro_max_signal_speed.argtypes = [ctypes.POINTER(Hydro), I64, P, P]
     L.knc_hydro_plm_gradient.argtypes = [D, I64, P, P, P, P]
     L.knc_fp64_probe.argtypes = [I64, P, P, P, P, P, P]
+    L.knc_fp64_fma_rate.argtypes = [I32, I32, I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]
     L.knc_mesh_block_vertices.argtypes = [ctypes.POINTER(Mesh), I32, P, P]
     L.knc_model_eval.argtypes = [P, ctypes.POINTER(Model), I64, P, P, D, P, P]
     L.knc_block_insert_model.argtypes = [P, I32, ctypes.POINTER(Mesh), ctypes.POINTER(Model), D]
@@ -660,6 +661,15 @@ def plm_gradient(plm_theta, a, b, c):
     out = np.zeros_like(a)
     _check(lib().knc_hydro_plm_gradient(float(plm_theta), a.size, _p(a), _p(b), _p(c), _p(out)))
     return out
+
+
+def fp64_fma_rate(chains=8, waves_per_simd=8, iters=4096):
+    """FP64 FMAs per second of the device and the timed launch's ms:
+    waves_per_simd 256-thread workgroups per compute unit, each lane running
+    `chains` independent chains of `iters` dependent FMAs."""
+    rate, ms = ctypes.c_double(), ctypes.c_float()
+    _check(lib().knc_fp64_fma_rate(int(chains), int(waves_per_simd), int(iters), ctypes.byref(rate), ctypes.byref(ms)))
+    return rate.value, ms.value
 
 
 def fp64_probe(a, b, c):

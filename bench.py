@@ -37,7 +37,9 @@ of the same mesh (100 RK2 steps after one warm-up step by default, about
 10-20 s on 16 threads), reported as zone-updates/s.
 
 The batched paths run the library's autotuner (knc_autotune) on the mesh
-before timing; the chosen shapes are reported.
+before timing; the chosen shapes are reported.  A short FP64 FMA probe
+(knc_fp64_fma_rate, fma_probe) measures the VALU issue peak the stage is
+compared with, and the rate one dependent chain leaves at two waves per SIMD.
 """
 import argparse
 import json
@@ -153,10 +155,23 @@ def read_traffic(head):
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4   # CUs x SIMDs x clock / cycles per wave64 FP64 instruction
 
 
-def valu_roofline(zones, stage_ms):
+def fma_probe():
+    """FP64 FMA issue rates measured on the device (knc_fp64_fma_rate), in G
+    wave-instructions/s: eight independent chains per lane at eight waves per
+    SIMD (the issue peak as measured), and four chains and one dependent
+    chain at two waves per SIMD, the stage kernel's occupancy."""
+    import knc
+    out = {"unit": "G wave-instructions/s"}
+    for name, chains, waves in (("8_chains_8_waves", 8, 8), ("4_chains_2_waves", 4, 2), ("1_chain_2_waves", 1, 2)):
+        rate, _ = knc.fp64_fma_rate(chains, waves, 4096)
+        out[name] = rate / 64 / 1e9
+    return out
+
+
+def valu_roofline(zones, stage_ms, probe=None):
     """VALU issue rate of the plain-guess stage kernel: instructions per zone
     from the committed SQ_INSTS_VALU summary x zones / measured stage time,
-    against the MI355X issue peak, or None."""
+    against the MI355X issue peak (and the measured FMA issue peak), or None."""
     path = os.path.join(ROOT, "profiles", "k_stage_bomb_jet_sq.txt")
     if not os.path.exists(path):
         return None
@@ -167,9 +182,13 @@ def valu_roofline(zones, stage_ms):
     if per_zone is None:
         return None
     achieved = per_zone * zones / 64 / (stage_ms * 1e-3)
-    return {"kernel": "k_stage, plain guess", "bound": "fp64-valu", "valu_insts_per_zone": per_zone,
-            "achieved": achieved / 1e9, "peak": VALU_PEAK_WAVE_INSTS / 1e9, "unit": "G wave-instructions/s",
-            "frac": achieved / VALU_PEAK_WAVE_INSTS}
+    out = {"kernel": "k_stage, plain guess", "bound": "fp64-valu", "valu_insts_per_zone": per_zone,
+           "achieved": achieved / 1e9, "peak": VALU_PEAK_WAVE_INSTS / 1e9, "unit": "G wave-instructions/s",
+           "frac": achieved / VALU_PEAK_WAVE_INSTS}
+    if probe:
+        out["measured_fma_peak"] = probe["8_chains_8_waves"]
+        out["frac_of_measured_peak"] = achieved / 1e9 / probe["8_chains_8_waves"]
+    return out
 
 
 # FP64 operations of one Newton iteration of the c2p as restated (add, sub,
@@ -331,6 +350,9 @@ def main():
 
     newton = newton_stats(setup, args.fold, net) if not args.no_stats else None
     io = io_rates(ctx_b, zones) if not args.no_io else None
+    probe = fma_probe()
+    log(f"bench: FP64 FMA issue {probe['8_chains_8_waves']:.1f} G wave-instructions/s (8 chains, 8 waves/SIMD), "
+        f"{probe['1_chain_2_waves']:.1f} for one dependent chain at 2 waves/SIMD")
 
     # headline: the faster learned-guess form (batched plain guess without weights)
     head = max(learned, key=lambda k: learned[k]["value"]) if learned else None
@@ -382,7 +404,8 @@ def main():
         "headline_stage_roofline": {"bound": "hbm", "achieved": achieved_head, "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": achieved_head / HBM_PEAK_GBS,
                                     "traffic": read_traffic(head), "stage_ms": head_stage},
-        "valu_roofline": valu_roofline(zones, stage_ms_b),
+        "valu_roofline": valu_roofline(zones, stage_ms_b, probe),
+        "fma_probe": probe,
         "c2p_fp64_roofline": (c2p_roofline(zones, head_stage, newton[head or "plain"]["mean_iterations"])
                               if newton else None),
         "autotuned": {"batched": ctx_b.tuned, **{k: c.tuned for k, c in ctx_l.items()},

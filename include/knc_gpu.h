@@ -462,6 +462,15 @@ int knc_mlp_predict(knc_mlp* mlp, int64_t n, const double* conserved, const doub
  * c2p_guess = KNC_GUESS_LEARNED. */
 int knc_context_set_mlp(knc_context* ctx, const knc_mlp* mlp);
 
+/* FP64 vector FMA rate of the current device: 256-thread workgroups,
+ * waves_per_simd (1..8) of them per compute unit, every lane running
+ * `chains` (1, 2, 4 or 8) independent chains of `iters` dependent FMAs.
+ * Writes FMAs per second and the timed launch's milliseconds (ms may be
+ * NULL).  A measurement utility with no counterpart in the reference: it
+ * confirms the FP64 issue peak that the stage kernel is measured against and
+ * the rate a dependent chain leaves at a given occupancy. */
+int knc_fp64_fma_rate(int chains, int waves_per_simd, int iters, double* fma_per_s, float* ms);
+
 /* FP64 arithmetic probe of the device (a/b, sqrt|a|, a*b+c with two
  * roundings), used to check that device arithmetic rounds like the host's. */
 int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sqrt_abs_a,

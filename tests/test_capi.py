@@ -67,6 +67,10 @@ def test_null_context_arguments_are_errors():
     assert L.knc_autotune(None, 1, 0, None) == 10
     assert L.knc_advance(None, 1, None, None) == 10
     assert L.knc_block_remove(None, 0) == 10
+    rate = ctypes.c_double()
+    for chains, waves, iters in ((3, 2, 16), (8, 0, 16), (8, 9, 16), (8, 2, 0)):   # checked before any device call
+        assert L.knc_fp64_fma_rate(chains, waves, iters, ctypes.byref(rate), None) == 10
+    assert L.knc_fp64_fma_rate(8, 2, 16, None, None) == 10
 
 
 DRIVER = os.path.join(os.path.dirname(capi.LIB_PATH), "knc_drive")

@@ -71,6 +71,19 @@ def test_fp64_arithmetic_rounds_like_host():
         assert np.array_equal(mad[m], (a * b + c)[m])
 
 
+def test_fp64_fma_probe_rates():
+    """The FMA probe behind the stage's VALU roofline: eight independent
+    chains at eight waves per SIMD issue faster than one dependent chain at
+    two, and neither exceeds the MI355X issue peak (614 G wave64
+    instructions/s) by more than clock tolerance."""
+    peak, ms = knc.fp64_fma_rate(8, 8, 1024)
+    dep, _ = knc.fp64_fma_rate(1, 2, 1024)
+    print(f"FP64 FMA issue: {peak / 64 / 1e9:.1f} G/s (8 chains, 8 waves), {dep / 64 / 1e9:.1f} G/s (1 chain, 2 waves)",
+          flush=True)
+    assert ms > 0 and peak > dep > 0
+    assert peak / 64 < 1.2 * 614.4e9
+
+
 def _random_prims(system, n, rng):
     if system == knc.SRHD:
         G = np.exp(rng.uniform(0, np.log(100), n))
