@@ -25,6 +25,24 @@ def test_header_declares_the_boundary():
         assert required in names
 
 
+def _c_arity(name):
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    args = re.search(r"\b" + name + r"\s*\(([^;]*?)\)\s*;", text, re.S).group(1).strip()
+    return 0 if args in ("", "void") else args.count(",") + 1
+
+
+def test_integration_notes_bind_every_entry_point():
+    """INTEGRATION.md declares every header function as a Rust extern, with
+    the C declaration's number of parameters."""
+    notes = open(os.path.join(os.path.dirname(HEADER), "..", "INTEGRATION.md")).read()
+    rust = {m.group(1): m.group(2) for m in re.finditer(r"pub fn (knc_[a-z0-9_]+)\s*\(([^)]*)\)", notes)}
+    missing = [n for n in declared_functions() if n not in rust]
+    assert not missing, missing
+    for n in declared_functions():
+        args = rust[n].strip()
+        assert (0 if not args else args.count(",") + 1) == _c_arity(n), n
+
+
 def test_library_exports_every_declared_function():
     lib = ctypes.CDLL(capi.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
