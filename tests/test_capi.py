@@ -49,6 +49,22 @@ def test_library_exports_every_declared_function():
     assert not missing, missing
 
 
+def test_missing_library_fails_loudly(monkeypatch):
+    """Without the HIP library the host package raises; it has no CPU
+    fallback, and no module of the package imports the oracle."""
+    monkeypatch.setattr(capi, "_lib", None)
+    monkeypatch.setattr(capi, "LIB_PATH", os.path.join(os.path.dirname(capi.LIB_PATH), "absent.so"))
+    with pytest.raises(RuntimeError):
+        capi.lib()
+    with pytest.raises(RuntimeError):
+        knc.Context(knc.relativistic_hydro())
+    pkg = os.path.dirname(os.path.abspath(knc.__file__))
+    for name in os.listdir(pkg):
+        if name.endswith(".py"):
+            src = open(os.path.join(pkg, name)).read()
+            assert not re.search(r"^\s*(import|from)\s+oracle\b", src, re.M), name
+
+
 def test_abi_version():
     assert knc.lib().knc_abi_version() == 4
 
