@@ -161,9 +161,9 @@ def fma_probe():
     SIMD (the issue peak as measured), and four chains and one dependent
     chain at two waves per SIMD, the stage kernel's occupancy."""
     import knc
-    out = {"unit": "G wave-instructions/s"}
+    out = {"unit": "G wave-instructions/s", "note": "best of 3 launches of 16384 FMAs per chain"}
     for name, chains, waves in (("8_chains_8_waves", 8, 8), ("4_chains_2_waves", 4, 2), ("1_chain_2_waves", 1, 2)):
-        rate, _ = knc.fp64_fma_rate(chains, waves, 4096)
+        rate = max(knc.fp64_fma_rate(chains, waves, 16384)[0] for _ in range(3))
         out[name] = rate / 64 / 1e9
     return out
 
