@@ -772,13 +772,14 @@ __global__ void k_plm(double th, int64_t n, const double* a, const double* b, co
 }
 
 // Exact FP64 arithmetic probe: a/b, sqrt(|a|) as the stage computes it
-// (sqrt_rn), a*b+c evaluated as two roundings.
+// (sqrt_rn_ge1 from 1 up, sqrt_rn below), a*b+c evaluated as two roundings.
 __global__ void k_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div,
                              double* sq, double* mad)
 {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         div[k] = a[k] / b[k];
-        sq[k] = sqrt_rn(fabs(a[k]));
+        const double x = fabs(a[k]);
+        sq[k] = x >= 1.0 ? sqrt_rn_ge1(x) : sqrt_rn(x);
         mad[k] = a[k] * b[k] + c[k];
     }
 }

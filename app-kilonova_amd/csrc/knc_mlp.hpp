@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "knc_zone.hpp"
+
 namespace knc {
 
 constexpr int MLP_IN = 4;
@@ -150,7 +152,7 @@ __device__ inline float safe_log2(float x)
 // Raw (unnormalised) features of a per-volume conserved state.
 __device__ inline void mlp_raw_features(double D, double S2, double tau, double gamma, float f[4])
 {
-    const double q = sqrt(D * D + S2);
+    const double q = sqrt_rn(D * D + S2);
     const double th = (tau + D) - q;
     const float l2d = safe_log2((float)D);
     f[0] = l2d;

@@ -75,6 +75,25 @@ __device__ inline double sqrt_rn(double x)
     return g;
 }
 
+// sqrt_rn of a Lorentz factor's square 1 + u^2, which is at least 1 unless
+// it is NaN or +inf: one compare selects the compiler's square root for
+// those two.
+__device__ inline double sqrt_rn_ge1(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    if (__builtin_expect(!(x <= 0x1.fffffffffffffp+1023), 0)) g = __builtin_sqrt(x);
+    return g;
+}
+
 // ---------------------------------------------------------------------------
 // c2p, SRHD: Newton-Raphson on p (Marti & Mueller, Living Rev. Relativ. 6
 // (2003) 7, sec. 2.3).  Q = tau + D + p, den = Q^2 - S^2:
@@ -204,7 +223,7 @@ __device__ inline Cons p2c(const Prim& p, const ZoneParams& zp)
         u.sq = p.rho * p.uq;
         u.tau = p.p / zp.gm1 + (0.5 * p.rho) * (p.ur * p.ur + p.uq * p.uq);
     } else {
-        u = p2c_srhd(p, sqrt_rn(1.0 + (p.ur * p.ur + p.uq * p.uq)), zp);
+        u = p2c_srhd(p, sqrt_rn_ge1(1.0 + (p.ur * p.ur + p.uq * p.uq)), zp);
     }
     return u;
 }
@@ -243,7 +262,7 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
         return st;
     }
     const double uu = P.ur * P.ur + P.uq * P.uq;
-    const double W = sqrt_rn(1.0 + uu);
+    const double W = sqrt_rn_ge1(1.0 + uu);
     st.u = p2c_srhd(P, W, zp);
     const double rhoh = P.rho + P.p * zp.gfac;
     const double ut = AXIS == 0 ? P.uq : P.ur;
