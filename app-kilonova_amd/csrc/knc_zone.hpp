@@ -59,7 +59,7 @@ __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b 
 // zero / infinity pass-through and their selects (seven of its seventeen
 // VALU instructions) are taken only by the lanes outside that range.
 // Every result is the host's sqrt to the bit (test_fp64_arithmetic_rounds_like_host).
-__device__ inline double sqrt_rn(double x)
+__device__ inline double sqrt_rn_core(double x)
 {
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y;
@@ -70,7 +70,12 @@ __device__ inline double sqrt_rn(double x)
     double d = fma(-g, g, x);
     g = fma(d, h, g);
     d = fma(-g, g, x);
-    g = fma(d, h, g);
+    return fma(d, h, g);
+}
+
+__device__ inline double sqrt_rn(double x)
+{
+    double g = sqrt_rn_core(x);
     if (__builtin_expect(!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023), 0)) g = __builtin_sqrt(x);
     return g;
 }
@@ -80,16 +85,7 @@ __device__ inline double sqrt_rn(double x)
 // those two.
 __device__ inline double sqrt_rn_ge1(double x)
 {
-    const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y;
-    double h = y * 0.5;
-    const double r = fma(-h, g, 0.5);
-    g = fma(g, r, g);
-    h = fma(h, r, h);
-    double d = fma(-g, g, x);
-    g = fma(d, h, g);
-    d = fma(-g, g, x);
-    g = fma(d, h, g);
+    double g = sqrt_rn_core(x);
     if (__builtin_expect(!(x <= 0x1.fffffffffffffp+1023), 0)) g = __builtin_sqrt(x);
     return g;
 }
