@@ -49,43 +49,8 @@ int knc::fail(int code, const char* fmt, ...)
     return code;
 }
 
-namespace {
-
-int check_hydro(const knc_hydro* h)
+void knc::fill_radial_table(int nr, const double* rv, double* v)
 {
-    if (!h) return fail(KNC_ERR_INVALID_ARGUMENT, "hydro is NULL");
-    if (h->system != KNC_EULER && h->system != KNC_SRHD) return fail(KNC_ERR_INVALID_ARGUMENT, "unknown system");
-    if (h->riemann_solver != KNC_HLLE && h->riemann_solver != KNC_HLLC)
-        return fail(KNC_ERR_INVALID_ARGUMENT, "unknown riemann solver");
-    if (h->runge_kutta_order < 1 || h->runge_kutta_order > 3)
-        return fail(KNC_ERR_INVALID_ARGUMENT, "runge_kutta_order must be 1, 2 or 3");
-    // Hydrodynamics::validate (relativistic_hydro.rs:46-54, newtonian_hydro.rs:38-46)
-    if (h->plm_theta < 1.0 || h->plm_theta > 2.0)
-        return fail(KNC_ERR_INVALID_ARGUMENT, "plm_theta must be in the range [1, 2]");
-    if (h->cfl_number < 0.0 || h->cfl_number > 0.7)
-        return fail(KNC_ERR_INVALID_ARGUMENT, "cfl_number must be in the range [0.0, 0.7]");
-    if (!(h->gamma_law_index > 1.0)) return fail(KNC_ERR_INVALID_ARGUMENT, "gamma_law_index must exceed 1");
-    if (!(h->light_speed > 0.0)) return fail(KNC_ERR_INVALID_ARGUMENT, "light_speed must be positive");
-    return KNC_OK;
-}
-
-ZoneParams zone_params(const knc_hydro& h)
-{
-    ZoneParams zp;
-    zp.gamma = h.gamma_law_index;
-    zp.gm1 = h.gamma_law_index - 1.0;
-    zp.gfac = h.gamma_law_index / (h.gamma_law_index - 1.0);
-    zp.plm_theta = h.plm_theta;
-    zp.c = h.light_speed;
-    return zp;
-}
-
-// Radial table of one block (layout in knc_device.hpp), from the vertices
-// with the formulas of src/mesh.rs:119-134,157-161.
-std::vector<double> radial_table(int nr, const double* rv)
-{
-    std::vector<double> t((size_t)rgeo_size(nr));
-    double* v = t.data();
     double* rr2 = v + (nr + 1);
     double* volA = rr2 + (nr + 1);
     double* rvolA = volA + nr;
@@ -104,6 +69,83 @@ std::vector<double> radial_table(int nr, const double* rv)
         rb[i] = (KNC_PI * (r0 + r1)) * (r1 - r0);
         dr[i] = r1 - r0;
     }
+}
+
+void knc::fill_theta_table(int nq, const double* qv, double* t)
+{
+    const size_t m = (size_t)nq + 1;
+    double* dcost = t;
+    double* rdcost = dcost + m;
+    double* aq = rdcost + m;
+    double* cq = aq + m;
+    double* cot = cq + m;
+    double* dq = cot + m;
+    for (int j = 0; j <= nq; ++j) {
+        const double q0 = qv[j], q1 = j < nq ? qv[j + 1] : qv[j];
+        const double s0 = std::sin(q0), s1 = std::sin(q1);
+        const double c0 = std::cos(q0), c1 = std::cos(q1);
+        dcost[j] = -(c1 - c0);
+        rdcost[j] = 1.0 / dcost[j];
+        const double ds = s1 - s0, dc = c1 - c0;
+        aq[j] = (KNC_PI * (s0 + s1)) * std::sqrt(ds * ds + dc * dc);
+        cq[j] = s0 * std::sqrt(s0 * s0 + c0 * c0);
+        const double q = 0.5 * (q0 + q1);
+        cot[j] = std::cos(q) / std::sin(q);
+        dq[j] = q1 - q0;
+    }
+}
+
+knc::ThetaTable knc::theta_view(const double* d, int nq)
+{
+    const size_t m = (size_t)nq + 1;
+    ThetaTable th;
+    th.dcost = d;
+    th.rdcost = d + m;
+    th.aq = d + 2 * m;
+    th.cq = d + 3 * m;
+    th.cot = d + 4 * m;
+    th.dq = d + 5 * m;
+    th.nq = nq;
+    return th;
+}
+
+int knc::check_hydro(const knc_hydro* h)
+{
+    if (!h) return fail(KNC_ERR_INVALID_ARGUMENT, "hydro is NULL");
+    if (h->system != KNC_EULER && h->system != KNC_SRHD) return fail(KNC_ERR_INVALID_ARGUMENT, "unknown system");
+    if (h->riemann_solver != KNC_HLLE && h->riemann_solver != KNC_HLLC)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "unknown riemann solver");
+    if (h->runge_kutta_order < 1 || h->runge_kutta_order > 3)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "runge_kutta_order must be 1, 2 or 3");
+    // Hydrodynamics::validate (relativistic_hydro.rs:46-54, newtonian_hydro.rs:38-46)
+    if (h->plm_theta < 1.0 || h->plm_theta > 2.0)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "plm_theta must be in the range [1, 2]");
+    if (h->cfl_number < 0.0 || h->cfl_number > 0.7)
+        return fail(KNC_ERR_INVALID_ARGUMENT, "cfl_number must be in the range [0.0, 0.7]");
+    if (!(h->gamma_law_index > 1.0)) return fail(KNC_ERR_INVALID_ARGUMENT, "gamma_law_index must exceed 1");
+    if (!(h->light_speed > 0.0)) return fail(KNC_ERR_INVALID_ARGUMENT, "light_speed must be positive");
+    return KNC_OK;
+}
+
+ZoneParams knc::zone_params(const knc_hydro& h)
+{
+    ZoneParams zp;
+    zp.gamma = h.gamma_law_index;
+    zp.gm1 = h.gamma_law_index - 1.0;
+    zp.gfac = h.gamma_law_index / (h.gamma_law_index - 1.0);
+    zp.plm_theta = h.plm_theta;
+    zp.c = h.light_speed;
+    return zp;
+}
+
+namespace {
+
+// Radial table of one block (layout in knc_device.hpp), from the vertices
+// with the formulas of src/mesh.rs:119-134,157-161.
+std::vector<double> radial_table(int nr, const double* rv)
+{
+    std::vector<double> t((size_t)rgeo_size(nr));
+    fill_radial_table(nr, rv, t.data());
     return t;
 }
 
@@ -249,37 +291,12 @@ int upload_theta(knc_context* c, int nq, const double* qv)
 {
     c->nq = nq;
     c->qv.assign(qv, qv + nq + 1);
-    const size_t m = (size_t)nq + 1;
-    std::vector<double> t(6 * m, 0.0);
-    double* dcost = t.data();
-    double* rdcost = dcost + m;
-    double* aq = rdcost + m;
-    double* cq = aq + m;
-    double* cot = cq + m;
-    double* dq = cot + m;
-    for (int j = 0; j <= nq; ++j) {
-        const double q0 = qv[j], q1 = j < nq ? qv[j + 1] : qv[j];
-        const double s0 = std::sin(q0), s1 = std::sin(q1);
-        const double c0 = std::cos(q0), c1 = std::cos(q1);
-        dcost[j] = -(c1 - c0);
-        rdcost[j] = 1.0 / dcost[j];
-        const double ds = s1 - s0, dc = c1 - c0;
-        aq[j] = (KNC_PI * (s0 + s1)) * std::sqrt(ds * ds + dc * dc);
-        cq[j] = s0 * std::sqrt(s0 * s0 + c0 * c0);
-        const double q = 0.5 * (q0 + q1);
-        cot[j] = std::cos(q) / std::sin(q);
-        dq[j] = q1 - q0;
-    }
+    std::vector<double> t((size_t)theta_table_size(nq), 0.0);
+    fill_theta_table(nq, qv, t.data());
     if (c->d_theta) HIP_TRY(hipFree(c->d_theta));
     HIP_TRY(hipMalloc(&c->d_theta, t.size() * sizeof(double)));
     HIP_TRY(hipMemcpy(c->d_theta, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-    c->th.dcost = c->d_theta;
-    c->th.rdcost = c->d_theta + m;
-    c->th.aq = c->d_theta + 2 * m;
-    c->th.cq = c->d_theta + 3 * m;
-    c->th.cot = c->d_theta + 4 * m;
-    c->th.dq = c->d_theta + 5 * m;
-    c->th.nq = nq;
+    c->th = theta_view(c->d_theta, nq);
     std::vector<double> qc(nq);
     for (int j = 0; j < nq; ++j) qc[j] = 0.5 * (qv[j] + qv[j + 1]);
     if (c->d_qc) HIP_TRY(hipFree(c->d_qc));

@@ -132,6 +132,19 @@ struct knc_context {
 
 namespace knc {
 
+// Hydrodynamics::validate of a configuration (and its enum fields), and the
+// per-zone parameters derived from it.
+int check_hydro(const knc_hydro* h);
+ZoneParams zone_params(const knc_hydro& h);
+
+// Geometry tables from the vertices (layouts in knc_device.hpp), with the
+// formulas of src/mesh.rs:119-134,157-161: the radial table of a block
+// (rgeo_size(nr) doubles) and the polar table (theta_table_size(nq) doubles,
+// six arrays of nq + 1), and a ThetaTable over a device copy of the latter.
+void fill_radial_table(int nr, const double* rv, double* out);
+void fill_theta_table(int nq, const double* qv, double* out);
+ThetaTable theta_view(const double* d_table, int nq);
+
 // Inserts a block from host data (conserved may be NULL: the slot is left for
 // the caller to fill on the device).
 int insert_block(knc_context* c, int32_t index, int32_t nr, int32_t nq, const double* rv, const double* qv,

@@ -56,7 +56,7 @@ __host__ __device__ inline const double* rg_rvolA(const BlockDesc& b) { return b
 __host__ __device__ inline const double* rg_rinv(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 2 * b.nr; }
 __host__ __device__ inline const double* rg_rb(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 3 * b.nr; }
 __host__ __device__ inline const double* rg_dr(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 4 * b.nr; }
-inline int64_t rgeo_size(int nr) { return 2 * (int64_t)(nr + 1) + 5 * (int64_t)nr; }
+constexpr int64_t rgeo_size(int nr) { return 2 * (int64_t)(nr + 1) + 5 * (int64_t)nr; }
 
 // Polar table shared by every block, nq+1 entries per array:
 //   dcost = -(cos q1 - cos q0), rdcost = 1/dcost,
@@ -72,6 +72,8 @@ struct ThetaTable {
     const double* dq;
     int32_t nq;
 };
+
+constexpr int64_t theta_table_size(int nq) { return 6 * ((int64_t)nq + 1); }
 
 enum { BND_FIXED = 0, BND_OUTFLOW = 1 };
 

@@ -651,10 +651,11 @@ __global__ void __launch_bounds__(256) k_dt(StageArgs a, double* partial)
         const double dr = gld(rg_dr(B), i);
         const double r0 = gld(rg_rv(B), i);
         for (int j = it.j0 + (int)threadIdx.x; j < it.j1; j += blockDim.x) {
+            // the four conserved planes only: the scalar plays no part in dt
             const double* s = slot_of(B, a.in_slot) + (int64_t)i * nq + j;
             const double rdv = rvolA * gld(a.th.rdcost, j);
             const PS q = zone_primitive<SYS, false, false>(a, gld(s, 0), gld(s, np), gld(s, 2 * np), gld(s, 3 * np),
-                                                           gld(s, 4 * np), rdv, (B.order << 16) | i, j, nullptr, 0.0);
+                                                           0.0, rdv, (B.order << 16) | i, j, nullptr, 0.0);
             const double dl = dmin(dr, gld(a.th.dq, j) * r0);
             m = dmin(m, dl / (max_signal_speed<SYS>(q.P, a.zp) * a.zp.c));
         }

@@ -204,6 +204,8 @@ def lib():
     L.knc_set_time.argtypes = [P, D, I64]
     L.knc_get_time.argtypes = [P, P, P]
     L.knc_time_step.argtypes = [P, P, ctypes.POINTER(Error)]
+    L.knc_block_time_step_noalloc.argtypes = [ctypes.POINTER(Hydro), I32, I32, I32, P, P, P, P, P,
+                                              ctypes.POINTER(Error)]
     L.knc_advance_steps.argtypes = [P, D, I32, ctypes.POINTER(Error)]
     L.knc_advance.argtypes = [P, I32, P, ctypes.POINTER(Error)]
     L.knc_get_c2p_stats.argtypes = [P, ctypes.POINTER(C2PStats)]
@@ -581,6 +583,25 @@ class Checkpoint:
 
     def __exit__(self, *exc):
         self.close()
+
+
+NOALLOC_MAX_NR, NOALLOC_MAX_NQ = 8192, 1024
+
+
+def block_time_step_noalloc(h, index, radial_vertices, polar_vertices, conserved_device, stream=None):
+    """State::time_step of one block whose conserved planes D | S_r | S_theta
+    | tau (each nr x nq, theta fastest) are at the device address
+    `conserved_device`, through the entry point that allocates nothing."""
+    rv, qv = _f64(radial_vertices), _f64(polar_vertices)
+    dt = ctypes.c_double()
+    err = Error()
+    rc = lib().knc_block_time_step_noalloc(ctypes.byref(h), int(index), rv.size - 1, qv.size - 1, _p(rv), _p(qv),
+                                           ctypes.c_void_p(conserved_device), stream, ctypes.byref(dt),
+                                           ctypes.byref(err))
+    if rc in ERROR_KINDS:
+        raise HydroError(rc, err)
+    _check(rc)
+    return dt.value
 
 
 def try_to_primitive(h, conserved, guess=None):

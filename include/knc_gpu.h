@@ -176,6 +176,31 @@ int knc_get_time(knc_context* ctx, double* time, int64_t* iteration);
 /* State::time_step (src/state.rs:143-162). */
 int knc_time_step(knc_context* ctx, double* dt, knc_error* err);
 
+/* ---- allocation-free time step (no counterpart in the reference) ------- */
+/* For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The CFL
+ * time step that knc_time_step (State::time_step, src/state.rs:143-162)
+ * returns for a context holding this one block, of a block whose state lives
+ * in the caller's device memory: `conserved` is a device pointer, on the
+ * current device, to the block's volume-integrated conserved planes
+ * D | S_r | S_theta | tau, each nr*nq doubles with theta fastest (the layout
+ * a context keeps resident); the vertices are host arrays (nr+1, nq+1).  The
+ * work runs on `stream` (a hipStream_t of the current device, or NULL) and
+ * the call returns after it.  The library allocates nothing: the kernels'
+ * tables, partial minima, dt and error log live in a file-scope device array
+ * of KNC_NOALLOC_SCRATCH_BYTES that each device holds until the process
+ * ends; nr above KNC_NOALLOC_MAX_NR or nq above KNC_NOALLOC_MAX_NQ is
+ * KNC_ERR_INVALID_ARGUMENT.  Every call on a device shares that device's
+ * array, which keeps the last call's contents: calls from one process take
+ * turns under a lock, and calls that overlapped on one device would race on
+ * it.  Errors come back as from knc_time_step, with err->block = index. */
+#define KNC_NOALLOC_MAX_NR 8192
+#define KNC_NOALLOC_MAX_NQ 1024
+#define KNC_NOALLOC_SCRATCH_BYTES (576 * 1024)
+int knc_block_time_step_noalloc(const knc_hydro* hydro, int32_t index, int32_t nr, int32_t nq,
+                                const double* radial_vertices, const double* polar_vertices,
+                                const double* conserved, void* stream, double* dt, knc_error* err);
+
 /* nsteps RK steps (RungeKuttaOrder::try_advance_async with the stage of
  * try_advance_rk, src/scheme.rs:16-173) with a fixed dt, without excision. */
 int knc_advance_steps(knc_context* ctx, double dt, int32_t nsteps, knc_error* err);
