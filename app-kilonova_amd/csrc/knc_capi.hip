@@ -422,6 +422,8 @@ StageArgs base_args(knc_context* c)
     a.rows = c->rows_per_item;
     a.mlp = c->mlp;
     a.write_guess = c->o.c2p_guess == KNC_GUESS_PREVIOUS;
+    a.guess_in = c->cur;
+    a.guess_out = c->cur;
     return a;
 }
 
@@ -516,6 +518,21 @@ void stage_boundaries(knc_context* c, StageArgs& a, int fold_step, int stage)
     }
 }
 
+// Guess planes of a stage: the c2p starts from the plane of the stage's
+// input slot (with KNC_GUESS_LEARNED_STEP, the plane the network wrote for
+// the step's first stage) and KNC_GUESS_PREVIOUS stores the recovered
+// pressures into the plane of the output slot, which no workgroup of the
+// launch reads.
+void stage_slots(knc_context* c, StageArgs& a, int in, int out, int avg, double bweight)
+{
+    a.in_slot = in;
+    a.out_slot = out;
+    a.avg_slot = avg;
+    a.bweight = bweight;
+    a.guess_in = c->o.c2p_guess == KNC_GUESS_LEARNED_STEP ? c->cur : in;
+    a.guess_out = out;
+}
+
 int enqueue_rk_step(knc_context* c, int step, int fold_step)
 {
     const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
@@ -524,28 +541,28 @@ int enqueue_rk_step(knc_context* c, int step, int fold_step)
     int rc;
     switch (c->h.runge_kutta_order) {
     case 1:
-        a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
         if ((rc = launch_one_stage(c, a, true))) return rc;
         c->cur = f1;
         break;
     case 2:
-        a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
         if ((rc = launch_one_stage(c, a, true))) return rc;
-        a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.5;
+        stage_slots(c, a, f1, f2, c->cur, 0.5);
         stage_boundaries(c, a, fold_step, 1);
         if ((rc = launch_one_stage(c, a, false))) return rc;
         c->cur = f2;
         break;
     default:
-        a.in_slot = c->cur; a.out_slot = f1; a.avg_slot = -1;
+        stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
         if ((rc = launch_one_stage(c, a, true))) return rc;
-        a.in_slot = f1; a.out_slot = f2; a.avg_slot = c->cur; a.bweight = 0.75;
+        stage_slots(c, a, f1, f2, c->cur, 0.75);
         stage_boundaries(c, a, fold_step, 1);
         if ((rc = launch_one_stage(c, a, false))) return rc;
-        a.in_slot = f2; a.out_slot = f1; a.avg_slot = c->cur; a.bweight = 1.0 / 3.0;
+        stage_slots(c, a, f2, f1, c->cur, 1.0 / 3.0);
         stage_boundaries(c, a, fold_step, 2);
         if ((rc = launch_one_stage(c, a, false))) return rc;
         c->cur = f1;
@@ -1166,7 +1183,7 @@ extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, in
         (void)hipFree(b.mem);
         return fail(KNC_ERR_HIP, "hipMalloc failed");
     }
-    if (hipMalloc(&b.guess, n * sizeof(float)) != hipSuccess) {
+    if (hipMalloc(&b.guess, (size_t)NSLOT * n * sizeof(float)) != hipSuccess) {
         (void)hipFree(b.mem);
         (void)hipFree(b.rgeo);
         return fail(KNC_ERR_HIP, "hipMalloc failed");
@@ -1180,7 +1197,7 @@ extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, in
     HIP_TRY(hipMemcpy(b.rgeo, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(b.mem, 0, (size_t)NSLOT * 5 * n * sizeof(double)));
     // log2(p/D) = -inf: no guess yet (2^-inf = 0, the plain guess)
-    HIP_TRY(hipMemsetD32((hipDeviceptr_t)b.guess, 0xff800000u, n));
+    HIP_TRY(hipMemsetD32((hipDeviceptr_t)b.guess, 0xff800000u, (size_t)NSLOT * n));
     if (conserved) {
         std::vector<double> soa(5 * n);
         for (size_t k = 0; k < n; ++k) {

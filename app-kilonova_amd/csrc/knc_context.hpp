@@ -45,7 +45,7 @@ struct HostBlock {
     int nr = 0, nq = 0;
     double* mem = nullptr;      // NSLOT * 5 * nr * nq doubles
     double* rgeo = nullptr;     // device radial table
-    float* guess = nullptr;     // learned-guess plane, nr * nq floats
+    float* guess = nullptr;     // guess planes, NSLOT * nr * nq floats
     double* prim = nullptr;     // primitive planes of the two-pass stage, 5 * nr * nq
     std::vector<double> rv;
 };

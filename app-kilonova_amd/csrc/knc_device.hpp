@@ -32,7 +32,7 @@ constexpr int MAX_NQ = 1024;
 struct BlockDesc {
     double* slot[NSLOT];
     const double* rgeo;
-    float* guess;    // learned c2p guess log2(p/D) per zone (nr x nq), written by k_guess
+    float* guess;    // c2p guesses log2(p/D), one nr x nq plane per state slot (guess_plane)
     double* prim;    // primitive planes rho | u_r | u_q | p | rho X (5 x nr x nq), written by k_prim
     int32_t nr;
     int32_t nq;
@@ -47,6 +47,14 @@ struct BlockDesc {
 __host__ __device__ inline double* slot_of(const BlockDesc& b, int s)
 {
     return s == 0 ? b.slot[0] : (s == 1 ? b.slot[1] : b.slot[2]);
+}
+
+// Guess plane `s` of a block.  A stage reads the plane of its input slot and
+// writes the plane of its output slot (KNC_GUESS_PREVIOUS), so no launch reads
+// a plane that it also writes.
+__host__ __device__ inline float* guess_plane(const BlockDesc& b, int s)
+{
+    return b.guess + (int64_t)s * b.nr * b.nq;
 }
 
 __host__ __device__ inline const double* rg_rv(const BlockDesc& b) { return b.rgeo; }

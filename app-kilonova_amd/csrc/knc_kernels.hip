@@ -119,7 +119,7 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
 // workgroup, so the resolution and every branch on it are uniform.
 struct RowSrc {
     const double* base;   // first plane at (row, 0): U planes, or boundary prim planes
-    const float* guess;   // learned guess row (block rows of learned-guess stages)
+    const float* guess;   // guess row (block rows of guess-reading stages)
     double rvolA;         // 1/volA of the row (conserved rows)
     int32_t np;           // plane stride (nr nq < 2^26, or 2 nq for boundary rows)
     int32_t tag;          // (block order << 16) | row, or -1 for primitive rows (no c2p)
@@ -143,7 +143,7 @@ __device__ inline RowSrc block_row(const StageArgs& a, const BlockDesc& S, int k
 {
     RowSrc r;
     r.base = (PRIMS ? S.prim : slot_of(S, a.in_slot)) + (int64_t)k * S.nq;
-    r.guess = S.guess + (int64_t)k * S.nq;
+    r.guess = guess_plane(S, a.guess_in) + (int64_t)k * S.nq;
     r.rvolA = PRIMS ? 1.0 : gld(rg_rvolA(S), k);
     r.np = S.nr * S.nq;
     r.tag = PRIMS ? -1 : (S.order << 16) | k;
@@ -365,7 +365,8 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
             const PS Pk = raw_primitive<SYS, STATS, GUESS>(a, src, cur, j, lds_hist);
             if (GUESS && !PRIMS && a.write_guess && act && k >= it.i0 && k < it.i1) {
                 // this item owns row k: keep its pressure as the next stage's guess
-                gstf(B.guess, (int64_t)k * nq + j, guess_log2(Pk.P.p, cur.v[0] * (src.rvolA * gld(a.th.rdcost, j))));
+                gstf(guess_plane(B, a.guess_out), (int64_t)k * nq + j,
+                     guess_log2(Pk.P.p, cur.v[0] * (src.rvolA * gld(a.th.rdcost, j))));
             }
             Flux5 Fk;
             if (k >= it.i0) {
@@ -508,7 +509,8 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
             const double rdv = rvolA * gld(a.th.rdcost, j);
             const double u0 = gld(U, j), u1 = gld(U, np + j), u2 = gld(U, 2 * np + j), u3 = gld(U, 3 * np + j);
             const double u4 = gld(U, 4 * np + j);
-            const double guess = GUESS ? mlp_guess_pressure(u0 * rdv, gldf(B.guess, (int64_t)i * nq + j)) : 0.0;
+            const double guess =
+                GUESS ? mlp_guess_pressure(u0 * rdv, gldf(guess_plane(B, a.guess_in), (int64_t)i * nq + j)) : 0.0;
             {
                 const PS q = zone_primitive<SYS, STATS, GUESS>(a, u0, u1, u2, u3, u4, rdv, (B.order << 16) | i, j,
                                                                lds_hist, guess);
@@ -517,7 +519,8 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
                 gst(P, 2 * np + j, q.P.uq);
                 gst(P, 3 * np + j, q.P.p);
                 gst(P, 4 * np + j, q.s);
-                if (GUESS && a.write_guess) gstf(B.guess, (int64_t)i * nq + j, guess_log2(q.P.p, u0 * rdv));
+                if (GUESS && a.write_guess)
+                    gstf(guess_plane(B, a.guess_out), (int64_t)i * nq + j, guess_log2(q.P.p, u0 * rdv));
             }
         }
     }
@@ -614,7 +617,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1
         const int j = p.c0 + (int)threadIdx.x;
         if (j < p.j1) {
             const BlockDesc& B = a.blocks[p.block];
-            gstf(B.guess, (int64_t)p.i * B.nq + j, l2);
+            gstf(guess_plane(B, a.guess_in), (int64_t)p.i * B.nq + j, l2);
         }
         if (!more) break;
         p = q;

@@ -33,6 +33,8 @@ struct StageArgs {
     int32_t rows;                // rows per work item (row-parallel passes: k_prim, k_guess, k_dt)
     MlpDev mlp;                  // learned c2p guess (learned-guess kernels only)
     int32_t write_guess;         // guess kernels also store each owned zone's recovered log2(p/D)
+    int32_t guess_in;            // guess plane the c2p starts from (k_guess writes it)
+    int32_t guess_out;           // guess plane write_guess stores into (never guess_in)
 };
 
 // One failed zone into the error log.  tag = (block order << 16) | row, or
