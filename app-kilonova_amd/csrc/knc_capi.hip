@@ -51,22 +51,14 @@ int knc::fail(int code, const char* fmt, ...)
 
 void knc::fill_radial_table(int nr, const double* rv, double* v)
 {
-    double* rr2 = v + (nr + 1);
-    double* volA = rr2 + (nr + 1);
-    double* rvolA = volA + nr;
-    double* rinv = rvolA + nr;
-    double* rb = rinv + nr;
-    double* dr = rb + nr;
-    for (int i = 0; i <= nr; ++i) {
-        v[i] = rv[i];
-        rr2[i] = rv[i] * rv[i];
-    }
+    double* volA = v + (nr + 1);
+    double* rinv = volA + nr;
+    double* dr = rinv + nr;
+    for (int i = 0; i <= nr; ++i) v[i] = rv[i];
     for (int i = 0; i < nr; ++i) {
         const double r0 = rv[i], r1 = rv[i + 1];
         volA[i] = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
-        rvolA[i] = 1.0 / volA[i];
         rinv[i] = 1.0 / std::sqrt(r0 * r1);
-        rb[i] = (KNC_PI * (r0 + r1)) * (r1 - r0);
         dr[i] = r1 - r0;
     }
 }
@@ -75,24 +67,72 @@ void knc::fill_theta_table(int nq, const double* qv, double* t)
 {
     const size_t m = (size_t)nq + 1;
     double* dcost = t;
-    double* rdcost = dcost + m;
-    double* aq = rdcost + m;
-    double* cq = aq + m;
-    double* cot = cq + m;
+    double* cot = dcost + m;
     double* dq = cot + m;
     for (int j = 0; j <= nq; ++j) {
         const double q0 = qv[j], q1 = j < nq ? qv[j + 1] : qv[j];
-        const double s0 = std::sin(q0), s1 = std::sin(q1);
-        const double c0 = std::cos(q0), c1 = std::cos(q1);
-        dcost[j] = -(c1 - c0);
-        rdcost[j] = 1.0 / dcost[j];
-        const double ds = s1 - s0, dc = c1 - c0;
-        aq[j] = (KNC_PI * (s0 + s1)) * std::sqrt(ds * ds + dc * dc);
-        cq[j] = s0 * std::sqrt(s0 * s0 + c0 * c0);
+        dcost[j] = -(std::cos(q1) - std::cos(q0));
         const double q = 0.5 * (q0 + q1);
         cot[j] = std::cos(q) / std::sin(q);
         dq[j] = q1 - q0;
     }
+}
+
+// face_area of src/mesh.rs:125-134 between the corners (r0, q0) and (r1, q1),
+// written as there: pi (s0 + s1) sqrt(ds^2 + dz^2), s = r sin q, z = r cos q.
+static double face_area(double r0, double sin0, double cos0, double r1, double sin1, double cos1)
+{
+    const double s0 = r0 * sin0;
+    const double s1 = r1 * sin1;
+    const double z0 = r0 * cos0;
+    const double z1 = r1 * cos1;
+    const double ds = s1 - s0;
+    const double dz = z1 - z0;
+    return KNC_PI * (s0 + s1) * std::sqrt(ds * ds + dz * dz);
+}
+
+void knc::fill_zone_geometry(int nr, int nq, const double* rv, const double* qv, double* out)
+{
+    std::vector<double> sn((size_t)nq + 1), cs((size_t)nq + 1), dcost((size_t)nq);
+    for (int j = 0; j <= nq; ++j) {
+        sn[j] = std::sin(qv[j]);
+        cs[j] = std::cos(qv[j]);
+    }
+    for (int j = 0; j < nq; ++j) dcost[j] = -(cs[j + 1] - cs[j]);
+    double* ar = out;
+    double* aq = ar + (size_t)(nr + 1) * nq;
+    double* rdv = aq + (size_t)nr * (nq + 1);
+    for (int i = 0; i <= nr; ++i)   // zone(i, j).face_area_r(): corners (r_i, q_j), (r_i, q_j+1)
+        for (int j = 0; j < nq; ++j)
+            ar[(size_t)i * nq + j] = face_area(rv[i], sn[j], cs[j], rv[i], sn[j + 1], cs[j + 1]);
+    for (int i = 0; i < nr; ++i)    // zone(i, j).face_area_q(): corners (r_i, q_j), (r_i+1, q_j)
+        for (int j = 0; j <= nq; ++j)
+            aq[(size_t)i * (nq + 1) + j] = face_area(rv[i], sn[j], cs[j], rv[i + 1], sn[j], cs[j]);
+    for (int i = 0; i < nr; ++i) {  // cell_volume (mesh.rs:119-123): 2 pi (r1^3 - r0^3) / 3 * dcost
+        const double r0 = rv[i], r1 = rv[i + 1];
+        const double volA = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
+        for (int j = 0; j < nq; ++j) rdv[(size_t)i * nq + j] = 1.0 / (volA * dcost[j]);
+    }
+}
+
+int knc::check_cell_volumes(int nr, int nq, const double* rv, const double* qv)
+{
+    double vlo = HUGE_VAL, vhi = 0.0, clo = HUGE_VAL, chi = 0.0;
+    for (int i = 0; i < nr; ++i) {
+        const double r0 = rv[i], r1 = rv[i + 1];
+        const double volA = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
+        vlo = std::min(vlo, volA);
+        vhi = std::max(vhi, volA);
+    }
+    for (int j = 0; j < nq; ++j) {
+        const double dcost = -(std::cos(qv[j + 1]) - std::cos(qv[j]));
+        clo = std::min(clo, dcost);
+        chi = std::max(chi, dcost);
+    }
+    if (!(vlo * clo >= 2.0 * DIV_RCP_DMIN && vhi * chi <= 0.5 * DIV_RCP_DMAX))
+        return fail(KNC_ERR_INVALID_ARGUMENT, "cell volumes outside [2^-399, 2^399] (or not positive): %.3e .. %.3e",
+                    vlo * clo, vhi * chi);
+    return KNC_OK;
 }
 
 knc::ThetaTable knc::theta_view(const double* d, int nq)
@@ -100,11 +140,8 @@ knc::ThetaTable knc::theta_view(const double* d, int nq)
     const size_t m = (size_t)nq + 1;
     ThetaTable th;
     th.dcost = d;
-    th.rdcost = d + m;
-    th.aq = d + 2 * m;
-    th.cq = d + 3 * m;
-    th.cot = d + 4 * m;
-    th.dq = d + 5 * m;
+    th.cot = d + m;
+    th.dq = d + 2 * m;
     th.nq = nq;
     return th;
 }
@@ -222,7 +259,6 @@ GhostRows ghost_rows(const knc_mesh& m, int side, int32_t inner, int32_t outer)
         const double r0 = rv[i0 + i], r1 = rv[i0 + i + 1];
         g.rc[i] = std::sqrt(r0 * r1);
         g.volA[i] = 2.0 * KNC_PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
-        g.rvolA[i] = 1.0 / g.volA[i];
     }
     return g;
 }
@@ -345,6 +381,7 @@ int rebuild_tables(knc_context* c)
         const size_t plane = (size_t)5 * b.nr * b.nq;
         for (int s = 0; s < NSLOT; ++s) d.slot[s] = b.mem + s * plane;
         d.rgeo = b.rgeo;
+        d.zgeo = b.zgeo;
         d.guess = b.guess;
         d.prim = b.prim;
         d.nr = b.nr;
@@ -827,7 +864,7 @@ int flush_steps(knc_context* c, int first, int n)
                     double* out = c->d_bseq + (((size_t)s * order + st) * 2 + side) * 10 * c->nq;
                     const int tag0 = (side ? GHOST_OUTER_ORDER : GHOST_INNER_ORDER) << 16;
                     HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g[side], c->d_qc,
-                                              c->th.dcost, c->th.rdcost, c->stage_times[(size_t)s * order + st],
+                                              c->th.dcost, c->stage_times[(size_t)s * order + st],
                                               s - first + 1, tag0, out, c->d_err, c->stream));
                 }
     }
@@ -1126,6 +1163,7 @@ void knc_context_destroy(knc_context* c)
     for (auto& kv : c->blocks) {
         (void)hipFree(kv.second.mem);
         (void)hipFree(kv.second.rgeo);
+        (void)hipFree(kv.second.zgeo);
         (void)hipFree(kv.second.guess);
         (void)hipFree(kv.second.prim);
     }
@@ -1159,6 +1197,7 @@ extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, in
     if (nr >= 65536) return fail(KNC_ERR_INVALID_ARGUMENT, "nr must be below 65536");
     if (nq < 1 || nq > MAX_NQ) return fail(KNC_ERR_INVALID_ARGUMENT, "nq must be in [1, %d]", MAX_NQ);
     if (c->blocks.count(index)) return fail(KNC_ERR_INVALID_ARGUMENT, "block %d already present", index);
+    if (int rc = check_cell_volumes(nr, nq, rv, qv)) return rc;
     if (c->blocks.size() >= 65535) return fail(KNC_ERR_INVALID_ARGUMENT, "too many blocks");
     if (c->blocks.empty()) {
         int rc = upload_theta(c, nq, qv);
@@ -1195,6 +1234,18 @@ extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, in
         return fail(KNC_ERR_HIP, "hipMalloc failed");
     }
     HIP_TRY(hipMemcpy(b.rgeo, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice));
+    {
+        std::vector<double> zg((size_t)zgeo_size(nr, nq));
+        fill_zone_geometry(nr, nq, rv, qv, zg.data());
+        if (hipMalloc(&b.zgeo, zg.size() * sizeof(double)) != hipSuccess) {
+            (void)hipFree(b.mem);
+            (void)hipFree(b.rgeo);
+            (void)hipFree(b.guess);
+            (void)hipFree(b.prim);
+            return fail(KNC_ERR_HIP, "hipMalloc failed");
+        }
+        HIP_TRY(hipMemcpy(b.zgeo, zg.data(), zg.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemset(b.mem, 0, (size_t)NSLOT * 5 * n * sizeof(double)));
     // log2(p/D) = -inf: no guess yet (2^-inf = 0, the plain guess)
     HIP_TRY(hipMemsetD32((hipDeviceptr_t)b.guess, 0xff800000u, (size_t)NSLOT * n));
@@ -1225,6 +1276,7 @@ int knc_block_remove(knc_context* c, int32_t index)
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(it->second.mem));
     HIP_TRY(hipFree(it->second.rgeo));
+    HIP_TRY(hipFree(it->second.zgeo));
     HIP_TRY(hipFree(it->second.guess));
     HIP_TRY(hipFree(it->second.prim));
     c->blocks.erase(it);
@@ -1738,26 +1790,29 @@ int knc_fp64_fma_rate(int chains, int waves_per_simd, int iters, double* fma_per
 }
 
 int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
-                   double* mad)
+                   double* mad, double* quot)
 {
-    if (n < 0 || (n && (!a || !b || !c || !div || !sq || !mad))) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n < 0 || (n && (!a || !b || !c || !div || !sq || !mad || !quot)))
+        return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
     if (n == 0) return KNC_OK;
-    DevBuf da, db, dc, d1, d2, d3;
+    DevBuf da, db, dc, d1, d2, d3, d4;
     ZONE_ALLOC(da, n * 8);
     ZONE_ALLOC(db, n * 8);
     ZONE_ALLOC(dc, n * 8);
     ZONE_ALLOC(d1, n * 8);
     ZONE_ALLOC(d2, n * 8);
     ZONE_ALLOC(d3, n * 8);
+    ZONE_ALLOC(d4, n * 8);
     HIP_TRY(hipMemcpy(da.p, a, n * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(db.p, b, n * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dc.p, c, n * 8, hipMemcpyHostToDevice));
     HIP_TRY(launch_fp64_probe(n, (const double*)da.p, (const double*)db.p, (const double*)dc.p, (double*)d1.p,
-                              (double*)d2.p, (double*)d3.p, nullptr));
+                              (double*)d2.p, (double*)d3.p, (double*)d4.p, nullptr));
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(div, d1.p, n * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(sq, d2.p, n * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(mad, d3.p, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(quot, d4.p, n * 8, hipMemcpyDeviceToHost));
     return KNC_OK;
 }
 

@@ -45,6 +45,7 @@ struct HostBlock {
     int nr = 0, nq = 0;
     double* mem = nullptr;      // NSLOT * 5 * nr * nq doubles
     double* rgeo = nullptr;     // device radial table
+    double* zgeo = nullptr;     // device per-zone geometry (face areas, 1/dV)
     float* guess = nullptr;     // guess planes, NSLOT * nr * nq floats
     double* prim = nullptr;     // primitive planes of the two-pass stage, 5 * nr * nq
     std::vector<double> rv;
@@ -144,6 +145,14 @@ ZoneParams zone_params(const knc_hydro& h);
 void fill_radial_table(int nr, const double* rv, double* out);
 void fill_theta_table(int nq, const double* qv, double* out);
 ThetaTable theta_view(const double* d_table, int nq);
+// Per-zone geometry of a block (layout in knc_device.hpp, zgeo_size(nr, nq)
+// doubles): face areas and 1/dV from the vertices, as src/mesh.rs:119-134
+// writes them.
+void fill_zone_geometry(int nr, int nq, const double* rv, const double* qv, double* out);
+// U/dV divides through div_rcp, whose three-operation path is exact for
+// volumes in [DIV_RCP_DMIN, DIV_RCP_DMAX]: fails unless every cell volume of
+// the block lies within a factor of two of that range.
+int check_cell_volumes(int nr, int nq, const double* rv, const double* qv);
 
 // Inserts a block from host data (conserved may be NULL: the slot is left for
 // the caller to fill on the device).

@@ -5,11 +5,13 @@
 // planes, D | S_r | S_q | tau | M (volume-integrated conserved quantities and
 // scalar mass, the SoA form of BlockState, src/state.rs:26-30), each
 // nr*nq doubles, contiguous.  Each block owns three such state slots that the
-// RK stages rotate through.  Geometry is kept as 1-D factors: per-block radial
-// tables and one theta table shared by every block (the reference gives all
-// blocks the same polar grid, src/mesh.rs:358-379); the 2-D face areas and
-// volumes of GridGeometry (src/mesh.rs:21-29) are rebuilt from them in
-// registers with the same floating-point operations.
+// RK stages rotate through.  Geometry: the face areas of GridGeometry
+// (src/mesh.rs:21-29) and the reciprocal cell volumes as per-zone tables,
+// evaluated on the host with the reference's own expressions, and 1-D
+// factors for the rest -- per-block radial tables and one theta table shared
+// by every block (the reference gives all blocks the same polar grid,
+// src/mesh.rs:358-379); the cell volume is volA[i] * dcost[j], the
+// reference's expression itself.
 #pragma once
 
 #include <stdint.h>
@@ -21,17 +23,13 @@ constexpr int MAX_NQ = 1024;
 
 // Radial geometry table of one block (BlockDesc::rgeo), built on the host:
 //   rv[nr+1]    vertices
-//   rr2[nr+1]   rv^2                     radial face area = rr2[f] * aq[j]
-//   volA[nr]    ((2 pi)(r1^3 - r0^3))/3  dV = volA[i] * dcost[j]
-//   rvolA[nr]   1/volA                   1/dV ~ rvolA[i] * rdcost[j]
+//   volA[nr]    ((2 pi)(r1^3 - r0^3))/3  dV = volA[i] * dcost[j] (mesh.rs:119-123)
 //   rinv[nr]    1/sqrt(r0 r1)            1/r at the zone centre (mesh.rs:157-161)
-//   rb[nr]      (pi (r0 + r1))(r1 - r0)  polar face area = rb[i] * cq[j]
 //   dr[nr]      r1 - r0
-// The frustum area pi (s0+s1) sqrt(ds^2+dz^2) of mesh.rs:125-134 is used in
-// its exact factored forms (radial x polar).
 struct BlockDesc {
     double* slot[NSLOT];
     const double* rgeo;
+    const double* zgeo;   // per-zone geometry (zg_* below)
     float* guess;    // c2p guesses log2(p/D), one nr x nq plane per state slot (guess_plane)
     double* prim;    // primitive planes rho | u_r | u_q | p | rho X (5 x nr x nq), written by k_prim
     int32_t nr;
@@ -58,30 +56,40 @@ __host__ __device__ inline float* guess_plane(const BlockDesc& b, int s)
 }
 
 __host__ __device__ inline const double* rg_rv(const BlockDesc& b) { return b.rgeo; }
-__host__ __device__ inline const double* rg_rr2(const BlockDesc& b) { return b.rgeo + (b.nr + 1); }
-__host__ __device__ inline const double* rg_volA(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1); }
-__host__ __device__ inline const double* rg_rvolA(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + b.nr; }
-__host__ __device__ inline const double* rg_rinv(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 2 * b.nr; }
-__host__ __device__ inline const double* rg_rb(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 3 * b.nr; }
-__host__ __device__ inline const double* rg_dr(const BlockDesc& b) { return b.rgeo + 2 * (b.nr + 1) + 4 * b.nr; }
-constexpr int64_t rgeo_size(int nr) { return 2 * (int64_t)(nr + 1) + 5 * (int64_t)nr; }
+__host__ __device__ inline const double* rg_volA(const BlockDesc& b) { return b.rgeo + (b.nr + 1); }
+__host__ __device__ inline const double* rg_rinv(const BlockDesc& b) { return b.rgeo + (b.nr + 1) + b.nr; }
+__host__ __device__ inline const double* rg_dr(const BlockDesc& b) { return b.rgeo + (b.nr + 1) + 2 * b.nr; }
+constexpr int64_t rgeo_size(int nr) { return (int64_t)(nr + 1) + 3 * (int64_t)nr; }
 
-// Polar table shared by every block, nq+1 entries per array:
-//   dcost = -(cos q1 - cos q0), rdcost = 1/dcost,
-//   aq = (pi (sin q0 + sin q1)) sqrt(dsin^2 + dcos^2),
-//   cq = sin q0 sqrt(sin^2 q0 + cos^2 q0)   (index nq: the last vertex),
-//   cot = cos(qc)/sin(qc) at qc = (q0+q1)/2, dq = q1 - q0.
+// Per-zone geometry of one block (BlockDesc::zgeo), built on the host with the
+// expressions of src/mesh.rs:119-134 as they are written there:
+//   ar[(nr+1) nq]   radial face areas, face_area of zone (i, j)'s inner face
+//   aq[nr (nq+1)]   polar face areas, face_area of zone (i, j)'s lower face
+//   rdv[nr nq]      1/dV of cell_volume (dV = volA[i] dcost[j], the same
+//                   expression), the y of div_rcp for U/dV
+__host__ __device__ inline const double* zg_ar(const BlockDesc& b) { return b.zgeo; }
+__host__ __device__ inline const double* zg_aq(const BlockDesc& b) { return b.zgeo + (int64_t)(b.nr + 1) * b.nq; }
+__host__ __device__ inline const double* zg_rdv(const BlockDesc& b)
+{
+    return b.zgeo + (int64_t)(b.nr + 1) * b.nq + (int64_t)b.nr * (b.nq + 1);
+}
+constexpr int64_t zgeo_size(int nr, int nq)
+{
+    return (int64_t)(nr + 1) * nq + (int64_t)nr * (nq + 1) + (int64_t)nr * nq;
+}
+
+// Polar table shared by every block, nq+1 entries per array (index nq: the
+// last vertex alone):
+//   dcost = -(cos q1 - cos q0), cot = cos(qc)/sin(qc) at qc = (q0+q1)/2,
+//   dq = q1 - q0.
 struct ThetaTable {
     const double* dcost;
-    const double* rdcost;
-    const double* aq;
-    const double* cq;
     const double* cot;
     const double* dq;
     int32_t nq;
 };
 
-constexpr int64_t theta_table_size(int nq) { return 6 * ((int64_t)nq + 1); }
+constexpr int64_t theta_table_size(int nq) { return 3 * ((int64_t)nq + 1); }
 
 enum { BND_FIXED = 0, BND_OUTFLOW = 1 };
 

@@ -50,15 +50,17 @@ __device__ inline void gstf(float* p, int64_t k, float v)
     ((__attribute__((address_space(1))) float*)p)[k] = v;
 }
 
-// c2p of one zone (BlockState::try_to_primitive, src/state.rs:76-94, and the
-// scalar concentration s = M/dV/Gamma of src/scheme.rs:35).  u is the
-// volume-integrated state; rdv the reciprocal volume factor.
+// c2p of one zone (BlockState::try_to_primitive of U/dV, src/state.rs:76-94,
+// and the scalar concentration s = M/dV/Gamma of src/scheme.rs:35, each
+// quotient correctly rounded as the reference divides).  u is the
+// volume-integrated state, dv the cell volume and rdv = 1/dv correctly
+// rounded.
 template <int SYS, bool STATS, bool GUESS>
 __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, double usq, double utau,
-                                    double um, double rdv, int tag, int col, unsigned int* lds_hist,
+                                    double um, double dv, double rdv, int tag, int col, unsigned int* lds_hist,
                                     double guess)
 {
-    const Cons u{ud * rdv, usr * rdv, usq * rdv, utau * rdv};
+    const Cons u{div_rcp(ud, dv, rdv), div_rcp(usr, dv, rdv), div_rcp(usq, dv, rdv), div_rcp(utau, dv, rdv)};
     PS out;
     int code;
     double value = u.d;
@@ -109,7 +111,9 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
         out.s = 0.0;
         return out;
     }
-    out.s = (um * rdv) * rW;
+    // lorentz_factor = lorentz_factor_squared().sqrt() (relativistic_hydro.rs:162-164), 1 for Euler
+    const double m = div_rcp(um, dv, rdv);
+    out.s = SYS == SYS_EULER ? m : m / sqrt_rn_ge1(1.0 + out.P.ur * out.P.ur + out.P.uq * out.P.uq);
     return out;
 }
 
@@ -120,7 +124,8 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
 struct RowSrc {
     const double* base;   // first plane at (row, 0): U planes, or boundary prim planes
     const float* guess;   // guess row (block rows of guess-reading stages)
-    double rvolA;         // 1/volA of the row (conserved rows)
+    const double* rdv;    // 1/dV row (conserved rows)
+    double volA;          // volA of the row (conserved rows): dV = volA dcost
     int32_t np;           // plane stride (nr nq < 2^26, or 2 nq for boundary rows)
     int32_t tag;          // (block order << 16) | row, or -1 for primitive rows (no c2p)
 };
@@ -130,7 +135,8 @@ __device__ inline RowSrc fixed_rows(const double* rows, int k2, int nq)
     RowSrc r;
     r.base = rows + (int64_t)k2 * nq;
     r.guess = nullptr;
-    r.rvolA = 1.0;
+    r.rdv = nullptr;
+    r.volA = 1.0;
     r.np = 2 * nq;
     r.tag = -1;
     return r;
@@ -144,7 +150,8 @@ __device__ inline RowSrc block_row(const StageArgs& a, const BlockDesc& S, int k
     RowSrc r;
     r.base = (PRIMS ? S.prim : slot_of(S, a.in_slot)) + (int64_t)k * S.nq;
     r.guess = guess_plane(S, a.guess_in) + (int64_t)k * S.nq;
-    r.rvolA = PRIMS ? 1.0 : gld(rg_rvolA(S), k);
+    r.rdv = zg_rdv(S) + (int64_t)k * S.nq;
+    r.volA = PRIMS ? 1.0 : gld(rg_volA(S), k);
     r.np = S.nr * S.nq;
     r.tag = PRIMS ? -1 : (S.order << 16) | k;
     return r;
@@ -172,20 +179,22 @@ __device__ inline RowSrc resolve_row(const StageArgs& a, const BlockDesc& B, int
     return block_row<PRIMS>(a, B, k);
 }
 
-// One zone as loaded from a row: the five planes and, for learned-guess
-// stages, the guessed log2(p/D).
+// One zone as loaded from a row: the five planes, 1/dV for conserved rows
+// and, for guess-reading stages, the guessed log2(p/D).
 struct Raw5 {
     double v[5];
+    double rdv;
     float g;
 };
 
-template <bool GUESS>
+template <bool GUESS, bool PRIMS>
 __device__ inline Raw5 load_raw(const RowSrc& r, int j)
 {
     const double* p = r.base + j;
     Raw5 x;
 #pragma unroll
     for (int f = 0; f < 5; ++f) x.v[f] = gld(p, f * r.np);
+    x.rdv = !PRIMS && r.tag >= 0 ? gld(r.rdv, j) : 1.0;
     x.g = GUESS && r.tag >= 0 ? gldf(r.guess, j) : 0.0f;
     return x;
 }
@@ -200,10 +209,10 @@ __device__ inline PS raw_primitive(const StageArgs& a, const RowSrc& r, const Ra
         q.s = x.v[4];
         return q;
     }
-    const double rdv = r.rvolA * gld(a.th.rdcost, j);
-    const double guess = GUESS ? mlp_guess_pressure(x.v[0] * rdv, x.g) : 0.0;
-    return zone_primitive<SYS, STATS, GUESS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], rdv, r.tag, j, lds_hist,
-                                             guess);
+    const double dv = r.volA * gld(a.th.dcost, j);
+    const double guess = GUESS ? mlp_guess_pressure(x.v[0] * x.rdv, x.g) : 0.0;
+    return zone_primitive<SYS, STATS, GUESS>(a, x.v[0], x.v[1], x.v[2], x.v[3], x.v[4], dv, x.rdv, r.tag, j,
+                                             lds_hist, guess);
 }
 
 __device__ inline void lds_put(double* L, int W, int j, const PS& q)
@@ -247,7 +256,7 @@ template <int SYS, bool STATS, bool GUESS, bool PRIMS>
 __device__ inline PS column_primitive(const StageArgs& a, const BlockDesc& B, int row, int col, unsigned int* lds_hist)
 {
     const RowSrc src = block_row<PRIMS>(a, B, row);
-    return raw_primitive<SYS, STATS, GUESS>(a, src, load_raw<GUESS && !PRIMS>(src, col), col, lds_hist);
+    return raw_primitive<SYS, STATS, GUESS>(a, src, load_raw<GUESS && !PRIMS, PRIMS>(src, col), col, lds_hist);
 }
 
 // PRIMS: the primitives come from the planes written by k_prim (the two-pass
@@ -282,7 +291,8 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
         const int j0 = it.j0, j1 = it.j1;
         const int nrow = it.i1 - it.i0;
         const int64_t np = (int64_t)B.nr * nq;
-        const double* rr2 = rg_rr2(B);
+        const double* area_r = zg_ar(B);
+        const double* area_q = zg_aq(B);
         const double* Uin = slot_of(B, a.in_slot);
         double* Uout = slot_of(B, a.out_slot);
         const double* U0 = slot_of(B, avg ? a.avg_slot : a.in_slot);
@@ -324,7 +334,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     Cons G;
                     double Gs;
                     riemann<SYS, RS, 1>(add_half(q[1], gl), sub_half(q[2], gr), a.zp, &G, &Gs);
-                    const Flux5 f = scaled(G, Gs, c, gld(rg_rb(B), it.i0 + rr) * gld(a.th.cq, j1));
+                    const Flux5 f = scaled(G, Gs, c, gld(zg_aq(B), (int64_t)(it.i0 + rr) * (nq + 1) + j1));
                     double* d = ldsR + rr * 5;
                     d[0] = f.d; d[1] = f.sr; d[2] = f.sq; d[3] = f.tau; d[4] = f.s;
                 }
@@ -335,7 +345,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
         PS Pm2, Pm1, gm2;
         Flux5 Fm;
         RowSrc src = resolve_row<PRIMS>(a, B, it.i0 - 2);
-        Raw5 cur = load_raw<GUESS && !PRIMS>(src, j), nxt;
+        Raw5 cur = load_raw<GUESS && !PRIMS, PRIMS>(src, j), nxt;
         // The fused stage keeps the conserved rows k-1 and k-2 it loaded for
         // the c2p: row k-2 is also the update operand U of this step.
         Raw5 rm1, rm2;
@@ -345,9 +355,13 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
             RowSrc nsrc = src;
             if (k + 1 < it.i1 + 2) {
                 nsrc = resolve_row<PRIMS>(a, B, k + 1);
-                nxt = load_raw<GUESS && !PRIMS>(nsrc, j);
+                nxt = load_raw<GUESS && !PRIMS, PRIMS>(nsrc, j);
             }
+            // This step's face areas: radial face k-1 and the polar faces of
+            // row k-2, consumed after the c2p.
+            const double ar = k >= it.i0 + 1 ? gld(area_r, (int64_t)(k - 1) * nq + j) : 0.0;
             const bool update = k >= it.i0 + 2;
+            const double aqr = update ? gld(area_q, (int64_t)(k - 2) * (nq + 1) + j) : 0.0;
             Raw5 ui, u0;
             if (update) {
                 const int64_t o = (int64_t)(k - 2) * nq + j;
@@ -365,8 +379,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
             const PS Pk = raw_primitive<SYS, STATS, GUESS>(a, src, cur, j, lds_hist);
             if (GUESS && !PRIMS && a.write_guess && act && k >= it.i0 && k < it.i1) {
                 // this item owns row k: keep its pressure as the next stage's guess
-                gstf(guess_plane(B, a.guess_out), (int64_t)k * nq + j,
-                     guess_log2(Pk.P.p, cur.v[0] * (src.rvolA * gld(a.th.rdcost, j))));
+                gstf(guess_plane(B, a.guess_out), (int64_t)k * nq + j, guess_log2(Pk.P.p, cur.v[0] * cur.rdv));
             }
             Flux5 Fk;
             if (k >= it.i0) {
@@ -376,7 +389,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     Cons F;
                     double Fs;
                     riemann<SYS, RS, 0>(add_half(Pm2, gm2), sub_half(Pm1, g), a.zp, &F, &Fs);
-                    Fk = scaled(F, Fs, c, gld(rr2, k - 1) * gld(a.th.aq, j));
+                    Fk = scaled(F, Fs, c, ar);
                 }
                 gm2 = g;
             }
@@ -408,7 +421,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     Cons G;
                     double Gs;
                     riemann<SYS, RS, 1>(add_half(Pl, gl), sub_half(P, gy), a.zp, &G, &Gs);
-                    Fy = scaled(G, Gs, c, gld(rg_rb(B), r) * gld(a.th.cq, j));
+                    Fy = scaled(G, Gs, c, aqr);
                 }
                 ldsF[tid] = Fy.d;
                 ldsF[NT + tid] = Fy.sr;
@@ -504,16 +517,18 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
         const int64_t np = (int64_t)B.nr * nq;
         const double* U = slot_of(B, a.in_slot) + (int64_t)i * nq;
         double* P = B.prim + (int64_t)i * nq;
-        const double rvolA = gld(rg_rvolA(B), i);
+        const double volA = gld(rg_volA(B), i);
+        const double* rdvrow = zg_rdv(B) + (int64_t)i * nq;
         for (int j = it.j0 + tid; j < it.j1; j += NT) {
-            const double rdv = rvolA * gld(a.th.rdcost, j);
+            const double rdv = gld(rdvrow, j);
+            const double dv = volA * gld(a.th.dcost, j);
             const double u0 = gld(U, j), u1 = gld(U, np + j), u2 = gld(U, 2 * np + j), u3 = gld(U, 3 * np + j);
             const double u4 = gld(U, 4 * np + j);
             const double guess =
                 GUESS ? mlp_guess_pressure(u0 * rdv, gldf(guess_plane(B, a.guess_in), (int64_t)i * nq + j)) : 0.0;
             {
-                const PS q = zone_primitive<SYS, STATS, GUESS>(a, u0, u1, u2, u3, u4, rdv, (B.order << 16) | i, j,
-                                                               lds_hist, guess);
+                const PS q = zone_primitive<SYS, STATS, GUESS>(a, u0, u1, u2, u3, u4, dv, rdv, (B.order << 16) | i,
+                                                               j, lds_hist, guess);
                 gst(P, j, q.P.rho);
                 gst(P, np + j, q.P.ur);
                 gst(P, 2 * np + j, q.P.uq);
@@ -588,7 +603,7 @@ __device__ inline GuessIn guess_load(const StageArgs& a, const GuessStep& p)
     GuessIn x;
 #pragma unroll
     for (int f = 0; f < 4; ++f) x.u[f] = gld(U, f * np);
-    x.rdv = gld(rg_rvolA(B), p.i) * gld(a.th.rdcost, jj);
+    x.rdv = gld(zg_rdv(B), (int64_t)p.i * B.nq + jj);
     return x;
 }
 
@@ -650,15 +665,16 @@ __global__ void __launch_bounds__(256) k_dt(StageArgs a, double* partial)
         const BlockDesc B = a.blocks[it.block];
         const int nq = B.nq;
         const int64_t np = (int64_t)B.nr * nq;
-        const double rvolA = gld(rg_rvolA(B), i);
+        const double volA = gld(rg_volA(B), i);
         const double dr = gld(rg_dr(B), i);
         const double r0 = gld(rg_rv(B), i);
         for (int j = it.j0 + (int)threadIdx.x; j < it.j1; j += blockDim.x) {
-            // the four conserved planes only: the scalar plays no part in dt
+            // the four conserved planes only: the scalar plays no part in dt.
+            // 1/dV by division here: the pass needs no per-zone table.
             const double* s = slot_of(B, a.in_slot) + (int64_t)i * nq + j;
-            const double rdv = rvolA * gld(a.th.rdcost, j);
+            const double dv = volA * gld(a.th.dcost, j);
             const PS q = zone_primitive<SYS, false, false>(a, gld(s, 0), gld(s, np), gld(s, 2 * np), gld(s, 3 * np),
-                                                           0.0, rdv, (B.order << 16) | i, j, nullptr, 0.0);
+                                                           0.0, dv, 1.0 / dv, (B.order << 16) | i, j, nullptr, 0.0);
             const double dl = dmin(dr, gld(a.th.dq, j) * r0);
             m = dmin(m, dl / (max_signal_speed<SYS>(q.P, a.zp) * a.zp.c));
         }
@@ -776,15 +792,18 @@ __global__ void k_plm(double th, int64_t n, const double* a, const double* b, co
 }
 
 // Exact FP64 arithmetic probe: a/b, sqrt(|a|) as the stage computes it
-// (sqrt_rn_ge1 from 1 up, sqrt_rn below), a*b+c evaluated as two roundings.
+// (sqrt_rn_ge1 from 1 up, sqrt_rn below), a*b+c evaluated as two roundings,
+// and a/|b| as the stage's U/dV computes it (div_rcp from 1/|b|).
 __global__ void k_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div,
-                             double* sq, double* mad)
+                             double* sq, double* mad, double* quot)
 {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         div[k] = a[k] / b[k];
         const double x = fabs(a[k]);
         sq[k] = x >= 1.0 ? sqrt_rn_ge1(x) : sqrt_rn(x);
         mad[k] = a[k] * b[k] + c[k];
+        const double d = fabs(b[k]);
+        quot[k] = d >= DIV_RCP_DMIN && d <= DIV_RCP_DMAX ? div_rcp(a[k], d, 1.0 / d) : a[k] / d;
     }
 }
 
@@ -1025,9 +1044,9 @@ hipError_t launch_plm(double th, int64_t n, const double* a, const double* b, co
 }
 
 hipError_t launch_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
-                             double* mad, hipStream_t s)
+                             double* mad, double* quot, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fp64_probe, dim3(grid_for(n, 256)), dim3(256), 0, s, n, a, b, c, div, sq, mad);
+    hipLaunchKernelGGL(k_fp64_probe, dim3(grid_for(n, 256)), dim3(256), 0, s, n, a, b, c, div, sq, mad, quot);
     return hipGetLastError();
 }
 

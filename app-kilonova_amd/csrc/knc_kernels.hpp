@@ -60,11 +60,10 @@ constexpr int GHOST_INNER_ORDER = 0x7ffe;
 constexpr int GHOST_OUTER_ORDER = 0x7fff;
 
 // Geometry of the two ghost rows of a model boundary: centre radius
-// sqrt(r0 r1), the radial volume factor 2 pi (r1^3 - r0^3)/3 and its inverse.
+// sqrt(r0 r1) and the radial volume factor 2 pi (r1^3 - r0^3)/3.
 struct GhostRows {
     double rc[2];
     double volA[2];
-    double rvolA[2];
 };
 
 // One stage: fused (c2p inside the stage kernel) or, with prims, the flux
@@ -110,10 +109,10 @@ hipError_t launch_block_from_model(int system, const ModelDev& m, const ZonePara
                                    const double* rc, const double* volA, const double* qc, const double* dcost,
                                    double t, double* planes, int* fail, hipStream_t s);
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
-                             const double* qc, const double* dcost, const double* rdcost, double t, int step,
+                             const double* qc, const double* dcost, double t, int step,
                              int tag0, double* out, ErrLog* err, hipStream_t s);
 hipError_t launch_fma_probe(int chains, int iters, int grid, double* out, hipStream_t s);
 hipError_t launch_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
-                             double* mad, hipStream_t s);
+                             double* mad, double* quot, hipStream_t s);
 
 }  // namespace knc

@@ -58,13 +58,13 @@ __global__ void k_block_from_model(ModelDev m, ZoneParams zp, int nr, int nq, co
 }
 
 // Ghost rows of one side at one stage time: from_model on the two rows next
-// to the mesh edge, then try_to_primitive of U/dV (as U * rvolA * rdcost,
-// the stage's own convention) and s = M/dV/Gamma, written as the boundary
+// to the mesh edge, then try_to_primitive of U/dV and s = M/dV/Gamma, each
+// quotient divided as the reference divides, written as the boundary
 // row set (5 planes x 2nq, radial order).  A c2p failure is logged like a
 // stage failure, under the ghost block's tag.
 template <int SYS>
 __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, const double* qc, const double* dcost,
-                             const double* rdcost, double t, int step, int tag0, double* out, ErrLog* err)
+                             double t, int step, int tag0, double* out, ErrLog* err)
 {
     const int n2 = 2 * nq;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
@@ -76,8 +76,7 @@ __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, con
         const double dv = g.volA[i] * dcost[j];
         const double ud = c.d * dv, usr = c.sr * dv, usq = c.sq * dv, utau = c.tau * dv;
         const double um = ud * X;
-        const double rdv = g.rvolA[i] * rdcost[j];
-        const Cons u{ud * rdv, usr * rdv, usq * rdv, utau * rdv};
+        const Cons u{ud / dv, usr / dv, usq / dv, utau / dv};
         Prim Q{0.0, 0.0, 0.0, 0.0};
         double rW = 1.0, value = u.d;
         int code = ok ? E_OK : KNC_ERR_MODEL;
@@ -107,7 +106,8 @@ __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, con
             record_error(err, step, tag0 | i, j, code, value, u);
             Q = Prim{0.0, 0.0, 0.0, 0.0};
         } else {
-            s = (um * rdv) * rW;
+            s = um / dv;
+            if (SYS == SYS_SRHD) s = s / sqrt_rn_ge1(1.0 + Q.ur * Q.ur + Q.uq * Q.uq);
         }
         out[k] = Q.rho;
         out[n2 + k] = Q.ur;
@@ -148,15 +148,15 @@ hipError_t launch_block_from_model(int system, const ModelDev& m, const ZonePara
 }
 
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
-                             const double* qc, const double* dcost, const double* rdcost, double t, int step,
+                             const double* qc, const double* dcost, double t, int step,
                              int tag0, double* out, ErrLog* err, hipStream_t s)
 {
     const dim3 g(grid_of(2 * (int64_t)nq, 256)), b(256);
     if (system == SYS_SRHD)
-        hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, rdcost, t, step, tag0, out,
+        hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, step, tag0, out,
                            err);
     else
-        hipLaunchKernelGGL(k_model_rows<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, rdcost, t, step, tag0,
+        hipLaunchKernelGGL(k_model_rows<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, step, tag0,
                            out, err);
     return hipGetLastError();
 }

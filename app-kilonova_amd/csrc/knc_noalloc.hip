@@ -98,6 +98,7 @@ extern "C" int knc_block_time_step_noalloc(const knc_hydro* hydro, int32_t index
     if (nq < 1 || nq > KNC_NOALLOC_MAX_NQ)
         return fail(KNC_ERR_INVALID_ARGUMENT, "nq = %d outside [1, %d], the limit of the no-allocation scratch array",
                     nq, KNC_NOALLOC_MAX_NQ);
+    if ((rc = check_cell_volumes(nr, nq, radial_vertices, polar_vertices))) return rc;
     // State::time_step with a global signal speed (src/state.rs:146-149)
     if (hydro->system == KNC_SRHD && !hydro->adaptive_time_step) {
         *dt = hydro->cfl_number * (radial_vertices[1] - radial_vertices[0]) / hydro->light_speed;

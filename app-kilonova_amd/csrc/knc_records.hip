@@ -35,15 +35,15 @@ namespace {
 // scalar concentration scalar_mass / lab_frame_mass = M / D as a plane of
 // nr * nq doubles.  A failure is logged under (order << 16 | row, column).
 template <int SYS>
-__global__ void k_products(const double* U, int nr, int nq, const double* rvolA, const double* rdcost, ZoneParams zp,
+__global__ void k_products(const double* U, int nr, int nq, const double* volA, const double* dcost, ZoneParams zp,
                            int order, ErrLog* err, double* out)
 {
     const int64_t n = (int64_t)nr * nq;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(k / nq), j = (int)(k - (int64_t)i * nq);
         const double ud = U[k];
-        const double rdv = rvolA[i] * rdcost[j];
-        const Cons u{ud * rdv, U[n + k] * rdv, U[2 * n + k] * rdv, U[3 * n + k] * rdv};
+        const double dv = volA[i] * dcost[j];
+        const Cons u{ud / dv, U[n + k] / dv, U[2 * n + k] / dv, U[3 * n + k] / dv};
         Prim P{0.0, 0.0, 0.0, 0.0};
         int code;
         double value = u.d;
@@ -511,14 +511,14 @@ int knc_write_products(knc_context* c, const char* path, const knc_record_meta* 
         BlockDesc d{};
         d.rgeo = b.rgeo;
         d.nr = b.nr;
-        const double* rvolA = rg_rvolA(d);
+        const double* volA = rg_volA(d);
         double* o = st.dev + offset[order];
         if (c->h.system == KNC_SRHD)
             hipLaunchKernelGGL(k_products<SYS_SRHD>, dim3(products_grid(n)), dim3(256), 0, c->stream, U, b.nr, b.nq,
-                               rvolA, c->th.rdcost, c->zp, order, c->d_err, o);
+                               volA, c->th.dcost, c->zp, order, c->d_err, o);
         else
             hipLaunchKernelGGL(k_products<SYS_EULER>, dim3(products_grid(n)), dim3(256), 0, c->stream, U, b.nr, b.nq,
-                               rvolA, c->th.rdcost, c->zp, order, c->d_err, o);
+                               volA, c->th.dcost, c->zp, order, c->d_err, o);
         HIP_TRY(hipGetLastError());
         ++order;
     }

@@ -90,6 +90,29 @@ __device__ inline double sqrt_rn_ge1(double x)
     return g;
 }
 
+// x / d, correctly rounded, from y = 1/d correctly rounded: q = x y is within
+// one ulp of x/d, the residual x - q d is exact under fma, and q + (x - q d) y
+// rounds to x/d (Markstein, IBM J. Res. Dev. 34 (1990) 111, theorem 1).  The
+// residual is formed as -(q d - x), which keeps the sign of a zero quotient
+// for d > 0.  Three operations instead of a division when y is shared or
+// tabulated.  The theorem needs every intermediate in the normal range:
+// for d in [2^-400, 2^400] (cell volumes, checked where their tables are
+// built) that holds when |q| is in [2^-500, 2^500], and lanes outside it
+// (tiny scalar tails, zeros excepted, infinities, NaN) divide.  Equal to the
+// host's x / d bit for bit (test_fp64_arithmetic_rounds_like_host).
+__host__ __device__ inline double div_rcp(double x, double d, double y)
+{
+    const double q = x * y;
+    const double t = fma(q, d, -x);
+    double r = fma(-t, y, q);
+    if (__builtin_expect(!(fabs(r) >= 0x1p-500 && fabs(r) <= 0x1p500) && x != 0.0, 0)) r = x / d;
+    return r;
+}
+
+// The range of d for which div_rcp's three-operation path is exact.
+constexpr double DIV_RCP_DMIN = 0x1p-400;
+constexpr double DIV_RCP_DMAX = 0x1p400;
+
 // ---------------------------------------------------------------------------
 // c2p, SRHD: Newton-Raphson on p (Marti & Mueller, Living Rev. Relativ. 6
 // (2003) 7, sec. 2.3).  Q = tau + D + p, den = Q^2 - S^2:

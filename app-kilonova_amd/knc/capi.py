@@ -223,7 +223,7 @@ def lib():
     L.knc_hydro_geometrical_source_terms.argtypes = [ctypes.POINTER(Hydro), I64, P, P, P, P]
     L.knc_hydro_max_signal_speed.argtypes = [ctypes.POINTER(Hydro), I64, P, P]
     L.knc_hydro_plm_gradient.argtypes = [D, I64, P, P, P, P]
-    L.knc_fp64_probe.argtypes = [I64, P, P, P, P, P, P]
+    L.knc_fp64_probe.argtypes = [I64, P, P, P, P, P, P, P]
     L.knc_fp64_fma_rate.argtypes = [I32, I32, I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]
     L.knc_mesh_block_vertices.argtypes = [ctypes.POINTER(Mesh), I32, P, P]
     L.knc_model_eval.argtypes = [P, ctypes.POINTER(Model), I64, P, P, D, P, P]
@@ -694,7 +694,9 @@ def fp64_fma_rate(chains=8, waves_per_simd=8, iters=4096):
 
 
 def fp64_probe(a, b, c):
+    """Device a/b, sqrt|a|, a*b+c (two roundings) and a/|b| as the stage
+    divides U by dV."""
     a, b, c = (_f64(x).reshape(-1) for x in (a, b, c))
-    div, sq, mad = np.zeros_like(a), np.zeros_like(a), np.zeros_like(a)
-    _check(lib().knc_fp64_probe(a.size, _p(a), _p(b), _p(c), _p(div), _p(sq), _p(mad)))
-    return div, sq, mad
+    div, sq, mad, quot = np.zeros_like(a), np.zeros_like(a), np.zeros_like(a), np.zeros_like(a)
+    _check(lib().knc_fp64_probe(a.size, _p(a), _p(b), _p(c), _p(div), _p(sq), _p(mad), _p(quot)))
+    return div, sq, mad, quot

@@ -497,9 +497,11 @@ int knc_context_set_mlp(knc_context* ctx, const knc_mlp* mlp);
 int knc_fp64_fma_rate(int chains, int waves_per_simd, int iters, double* fma_per_s, float* ms);
 
 /* FP64 arithmetic probe of the device (a/b, sqrt|a|, a*b+c with two
- * roundings), used to check that device arithmetic rounds like the host's. */
+ * roundings, and a/|b| as the stage divides U by dV: from the correctly
+ * rounded 1/|b| with one correction), used to check that device arithmetic
+ * rounds like the host's. */
 int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sqrt_abs_a,
-                   double* mul_add);
+                   double* mul_add, double* quot_abs_b);
 
 #ifdef __cplusplus
 }

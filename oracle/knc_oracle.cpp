@@ -391,6 +391,163 @@ Cons sources(const Hydro& h, const Prim& P, double rinv, double cot)
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// Literal forms of the crates' arithmetic.  The functions above restate the
+// un-vendored crates with quotients rewritten for speed (one division per
+// Newton step, per face and per flux; reciprocal radii; speeds kept as
+// fractions), identically on the device.  The forms below write the same
+// quantities as the published methods state them, every quotient divided
+// out, so that a Sim built with `literal` set measures how far two
+// restatements of the crates' arithmetic move the state:
+//   sources divided by the centre radius;
+//   wave speeds of Marti & Mueller (2003) eq. (72) in velocity form, the
+//     HLLE flux (a+ F_L - a- F_R + a+ a- (U_R - U_L)) / (a+ - a-), and the
+//     HLLC star states (Mignone & Bodo 2005, Toro 2009 sec. 10.4), each
+//     quotient divided.
+// The expressions of the reference's own code (face areas, volumes, U/dV,
+// M/dV/Gamma) are written as the reference writes them in both modes.
+// ---------------------------------------------------------------------------
+// Parts of the literal mode (a Sim's `literal` is a mask of them), both
+// restating the un-vendored crates' arithmetic in its textbook form.
+enum {
+    LIT_SOURCES = 4,    // sources divided by r
+    LIT_FLUX = 8,       // wave speeds and HLLE/HLLC fluxes with every quotient divided
+};
+
+struct FluxLit {
+    Cons u, f;
+    double us, fs, vn, lm, lp, p;
+};
+
+FluxLit flux_state_literal(const Hydro& h, const Prim& P, double s, int axis)
+{
+    const double g = h.gamma_law_index;
+    FluxLit st;
+    st.u = p2c(h, P);
+    st.p = P.p;
+    const double un = axis == 0 ? P.ur : P.uq;
+    if (h.system == SYS_EULER) {
+        st.vn = un;
+        st.us = s;
+        const double cs = std::sqrt(g * P.p / P.rho);
+        st.lm = st.vn - cs;
+        st.lp = st.vn + cs;
+    } else {
+        const double W = std::sqrt(1.0 + P.ur * P.ur + P.uq * P.uq);
+        const double v2 = (P.ur * P.ur + P.uq * P.uq) / (W * W);
+        const double enthalpy = 1.0 + g * P.p / ((g - 1.0) * P.rho);
+        const double cs2 = g * P.p / (P.rho * enthalpy);
+        const double cs = std::sqrt(cs2);
+        st.vn = un / W;
+        const double root = std::sqrt((1.0 - v2) * (1.0 - v2 * cs2 - st.vn * st.vn * (1.0 - cs2)));
+        const double den = 1.0 - v2 * cs2;
+        st.lm = (st.vn * (1.0 - cs2) - cs * root) / den;
+        st.lp = (st.vn * (1.0 - cs2) + cs * root) / den;
+        st.us = s * W;
+    }
+    // F = U v_n + p n, F_tau = (tau + p) v_n (F_E = E v_n + p v_n for Euler), F_s = U_s v_n
+    st.f.d = st.u.d * st.vn;
+    st.f.sr = st.u.sr * st.vn;
+    st.f.sq = st.u.sq * st.vn;
+    if (axis == 0) st.f.sr = st.f.sr + P.p; else st.f.sq = st.f.sq + P.p;
+    st.f.tau = (st.u.tau + P.p) * st.vn;
+    st.fs = st.us * st.vn;
+    return st;
+}
+
+void riemann_literal(const Hydro& h, const Prim& PL, const Prim& PR, double sL, double sR, int axis, Cons* F,
+                     double* Fs)
+{
+    const FluxLit L = flux_state_literal(h, PL, sL, axis);
+    const FluxLit R = flux_state_literal(h, PR, sR, axis);
+    if (h.riemann_solver == HLLE) {
+        const double am = std::min(std::min(L.lm, R.lm), 0.0);
+        const double ap = std::max(std::max(L.lp, R.lp), 0.0);
+        const double w = ap - am;
+        F->d   = (ap * L.f.d   - am * R.f.d   + ap * am * (R.u.d   - L.u.d))   / w;
+        F->sr  = (ap * L.f.sr  - am * R.f.sr  + ap * am * (R.u.sr  - L.u.sr))  / w;
+        F->sq  = (ap * L.f.sq  - am * R.f.sq  + ap * am * (R.u.sq  - L.u.sq))  / w;
+        F->tau = (ap * L.f.tau - am * R.f.tau + ap * am * (R.u.tau - L.u.tau)) / w;
+        *Fs    = (ap * L.fs    - am * R.fs    + ap * am * (R.us    - L.us))    / w;
+        return;
+    }
+    const double am = std::min(L.lm, R.lm);
+    const double ap = std::max(L.lp, R.lp);
+    if (am >= 0.0) { *F = L.f; *Fs = L.fs; return; }
+    if (ap <= 0.0) { *F = R.f; *Fs = R.fs; return; }
+    const double w = ap - am;
+    double lstar, pstar = 0.0;
+    if (h.system == SYS_SRHD) {
+        // HLL state and flux of E = tau + D and of the normal momentum
+        const double EL = L.u.tau + L.u.d, ER = R.u.tau + R.u.d;
+        const double FEL = L.f.tau + L.f.d, FER = R.f.tau + R.f.d;
+        const double E = (ap * ER - am * EL + FEL - FER) / w;
+        const double m = (ap * cn(R.u, axis) - am * cn(L.u, axis) + cn(L.f, axis) - cn(R.f, axis)) / w;
+        const double FE = (ap * FEL - am * FER + ap * am * (ER - EL)) / w;
+        const double Fm = (ap * cn(L.f, axis) - am * cn(R.f, axis) + ap * am * (cn(R.u, axis) - cn(L.u, axis))) / w;
+        // root of FE l^2 - (E + Fm) l + m = 0 with the minus sign (Mignone &
+        // Bodo eq. 18), written 2m / (b + sqrt(b^2 - 4 FE m)), defined at FE = 0
+        const double b = E + Fm;
+        lstar = 2.0 * m / (b + std::sqrt(b * b - 4.0 * FE * m));
+        pstar = Fm - FE * lstar;
+    } else {
+        lstar = (R.p - L.p + L.u.d * L.vn * (am - L.vn) - R.u.d * R.vn * (ap - R.vn)) /
+                (L.u.d * (am - L.vn) - R.u.d * (ap - R.vn));
+    }
+    const bool left = lstar >= 0.0;
+    const FluxLit& K = left ? L : R;
+    const double lam = left ? am : ap;
+    Cons us;
+    double uss;
+    if (h.system == SYS_SRHD) {
+        const double EK = K.u.tau + K.u.d;
+        us.d = K.u.d * (lam - K.vn) / (lam - lstar);
+        const double mn = (cn(K.u, axis) * (lam - K.vn) + pstar - K.p) / (lam - lstar);
+        const double mt = (axis == 0 ? K.u.sq : K.u.sr) * (lam - K.vn) / (lam - lstar);
+        const double Est = (EK * (lam - K.vn) + pstar * lstar - K.p * K.vn) / (lam - lstar);
+        if (axis == 0) { us.sr = mn; us.sq = mt; } else { us.sq = mn; us.sr = mt; }
+        us.tau = Est - us.d;
+        uss = K.us * (lam - K.vn) / (lam - lstar);
+    } else {
+        const double rhoK = K.u.d;
+        const double fac = rhoK * (lam - K.vn) / (lam - lstar);
+        const double vt = (axis == 0 ? K.u.sq : K.u.sr) / rhoK;
+        const double en = K.u.tau / rhoK + (lstar - K.vn) * (lstar + K.p / (rhoK * (lam - K.vn)));
+        us.d = fac;
+        if (axis == 0) { us.sr = fac * lstar; us.sq = fac * vt; }
+        else           { us.sq = fac * lstar; us.sr = fac * vt; }
+        us.tau = fac * en;
+        uss = K.us * (lam - K.vn) / (lam - lstar);
+    }
+    F->d   = K.f.d   + lam * (us.d   - K.u.d);
+    F->sr  = K.f.sr  + lam * (us.sr  - K.u.sr);
+    F->sq  = K.f.sq  + lam * (us.sq  - K.u.sq);
+    F->tau = K.f.tau + lam * (us.tau - K.u.tau);
+    *Fs    = K.fs    + lam * (uss    - K.us);
+}
+
+// Sources at the centre (r, theta), divided by r.
+Cons sources_literal(const Hydro& h, const Prim& P, double r, double theta)
+{
+    const double g = h.gamma_law_index;
+    const double rhoh = h.system == SYS_EULER ? P.rho : P.rho * (1.0 + g * P.p / ((g - 1.0) * P.rho));
+    const double cot = std::cos(theta) / std::sin(theta);
+    Cons s;
+    s.d = 0.0;
+    s.sr = (2.0 * P.p + rhoh * P.uq * P.uq) / r;
+    s.sq = (P.p * cot - rhoh * P.ur * P.uq) / r;
+    s.tau = 0.0;
+    return s;
+}
+
+// Lorentz factor of a recovered primitive for s = M/dV/Gamma:
+// lorentz_factor_squared().sqrt() (relativistic_hydro.rs:162-164); 1 for
+// Euler (newtonian_hydro.rs:130-132).
+double lorentz_factor(const Hydro& h, const Prim& P)
+{
+    return h.system == SYS_EULER ? 1.0 : std::sqrt(1.0 + P.ur * P.ur + P.uq * P.uq);
+}
+
 // Maximum signal speed (unscaled).  SRHD: relativistic sum of |v| and c_s,
 // an upper bound of every characteristic speed; Euler: |v| + c_s.
 double max_signal_speed(const Hydro& h, const Prim& P)
@@ -444,25 +601,19 @@ inline PS sub_half(const PS& a, const PS& g)
 }
 
 // ---------------------------------------------------------------------------
-// Geometry of one block: the 2-D arrays of GridGeometry (mesh.rs:21-29) from
-// the formulas of mesh.rs:119-134,157-161.  The frustum area
-// pi (s0+s1) sqrt(ds^2+dz^2) (mesh.rs:125-134) is evaluated in its exact
-// factored forms, r^2 * pi (sin q0 + sin q1) sqrt((sin q1-sin q0)^2 +
-// (cos q1-cos q0)^2) for a radial face and pi (r0+r1)(r1-r0) *
-// sin q sqrt(sin^2 q + cos^2 q) for a polar face, and each zone also keeps
-// the reciprocal volume factors 1/volA and 1/dcost used for U/dV.
+// Geometry of one block: the 2-D arrays of GridGeometry (mesh.rs:21-29),
+// zone by zone with the expressions of mesh.rs:119-134,157-161 as written
+// there (the frustum area pi (s0+s1) sqrt(ds^2+dz^2) from the corners, the
+// volume 2 pi (r1^3 - r0^3) / 3 * dcost), plus 1/r and cot theta at the
+// centre for the sources.
 // ---------------------------------------------------------------------------
 struct RadialRow {            // per radial index, from the vertices
     double volA;              // ((2 pi)(r1^3 - r0^3))/3
-    double rvolA;             // 1/volA
     double rc, rinv;          // sqrt(r0 r1), 1/rc
-    double rb;                // (pi (r0 + r1)) (r1 - r0)
 };
 
 struct PolarCol {             // per polar index, from the vertices
-    double dcost, rdcost;     // -(cos q1 - cos q0), 1/dcost
-    double aq;                // (pi (sin q0 + sin q1)) sqrt(dsin^2 + dcos^2)
-    double cq;                // sin q0 sqrt(sin^2 q0 + cos^2 q0)
+    double dcost;             // -(cos q1 - cos q0)
     double cot;               // cos(qc)/sin(qc), qc = (q0+q1)/2
 };
 
@@ -470,23 +621,15 @@ RadialRow radial_row(double r0, double r1)
 {
     RadialRow t;
     t.volA = 2.0 * PI * ((r1 * r1) * r1 - (r0 * r0) * r0) / 3.0;
-    t.rvolA = 1.0 / t.volA;
     t.rc = std::sqrt(r0 * r1);
     t.rinv = 1.0 / t.rc;
-    t.rb = (PI * (r0 + r1)) * (r1 - r0);
     return t;
 }
 
 PolarCol polar_col(double q0, double q1)
 {
     PolarCol t;
-    const double s0 = std::sin(q0), s1 = std::sin(q1);
-    const double c0 = std::cos(q0), c1 = std::cos(q1);
-    t.dcost = -(c1 - c0);
-    t.rdcost = 1.0 / t.dcost;
-    const double ds = s1 - s0, dc = c1 - c0;
-    t.aq = (PI * (s0 + s1)) * std::sqrt(ds * ds + dc * dc);
-    t.cq = s0 * std::sqrt(s0 * s0 + c0 * c0);
+    t.dcost = -(std::cos(q1) - std::cos(q0));
     const double qc = 0.5 * (q0 + q1);
     t.cot = std::cos(qc) / std::sin(qc);
     return t;
@@ -494,13 +637,33 @@ PolarCol polar_col(double q0, double q1)
 
 struct Block {
     int nr = 0, nq = 0;
+    int literal = 0;                 // LIT_* parts of the crates' arithmetic in their literal forms
     std::vector<double> rv, qv;      // vertices (nr+1), (nq+1)
     std::vector<double> Ar, Aq, dV;  // (nr+1,nq), (nr,nq+1), (nr,nq)
-    std::vector<double> rdV, rinv, cot, dl; // 1/dV factors, 1/rc, cot(qc), linear dim per zone
-    std::vector<double> rc;          // zone centre radius (error positions)
+    std::vector<double> rinv, cot, dl; // 1/rc, cot(qc), linear dim per zone
+    std::vector<double> rc, qc;      // zone centre (src/mesh.rs:157-161)
     std::vector<Cons> U;             // conserved * dV, (nr,nq) row-major, theta fastest
     std::vector<double> M;           // scalar mass
 };
+
+// face_area and cell_volume of src/mesh.rs:119-134, as written there:
+// c = (r, theta) corners, powi(3) as r * r * r.
+double face_area_literal(double r0, double q0, double r1, double q1)
+{
+    const double s0 = r0 * std::sin(q0);
+    const double s1 = r1 * std::sin(q1);
+    const double z0 = r0 * std::cos(q0);
+    const double z1 = r1 * std::cos(q1);
+    const double ds = s1 - s0;
+    const double dz = z1 - z0;
+    return PI * (s0 + s1) * std::sqrt(ds * ds + dz * dz);
+}
+
+double cell_volume_literal(double r0, double q0, double r1, double q1)
+{
+    const double dcost = -(std::cos(q1) - std::cos(q0));
+    return 2.0 * PI * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0 * dcost;
+}
 
 void build_geometry(Block& b)
 {
@@ -508,28 +671,27 @@ void build_geometry(Block& b)
     b.Ar.assign((size_t)(nr + 1) * nq, 0.0);
     b.Aq.assign((size_t)nr * (nq + 1), 0.0);
     b.dV.assign((size_t)nr * nq, 0.0);
-    b.rdV.assign((size_t)nr * nq, 0.0);
     b.rinv.assign((size_t)nr * nq, 0.0);
     b.rc.assign((size_t)nr * nq, 0.0);
+    b.qc.assign((size_t)nr * nq, 0.0);
     b.cot.assign((size_t)nr * nq, 0.0);
     b.dl.assign((size_t)nr * nq, 0.0);
     std::vector<RadialRow> R(nr);
-    std::vector<PolarCol> C(nq + 1);
+    std::vector<PolarCol> C(nq);
     for (int i = 0; i < nr; ++i) R[i] = radial_row(b.rv[i], b.rv[i + 1]);
     for (int j = 0; j < nq; ++j) C[j] = polar_col(b.qv[j], b.qv[j + 1]);
-    C[nq] = polar_col(b.qv[nq], b.qv[nq]);
     for (int i = 0; i <= nr; ++i)
         for (int j = 0; j < nq; ++j)   // zone(i,j).face_area_r()
-            b.Ar[(size_t)i * nq + j] = (b.rv[i] * b.rv[i]) * C[j].aq;
+            b.Ar[(size_t)i * nq + j] = face_area_literal(b.rv[i], b.qv[j], b.rv[i], b.qv[j + 1]);
     for (int i = 0; i < nr; ++i)
         for (int j = 0; j <= nq; ++j)  // zone(i,j).face_area_q()
-            b.Aq[(size_t)i * (nq + 1) + j] = R[i].rb * C[j].cq;
+            b.Aq[(size_t)i * (nq + 1) + j] = face_area_literal(b.rv[i], b.qv[j], b.rv[i + 1], b.qv[j]);
     for (int i = 0; i < nr; ++i)
         for (int j = 0; j < nq; ++j) {
             const size_t k = (size_t)i * nq + j;
-            b.dV[k] = R[i].volA * C[j].dcost;
-            b.rdV[k] = R[i].rvolA * C[j].rdcost;
+            b.dV[k] = cell_volume_literal(b.rv[i], b.qv[j], b.rv[i + 1], b.qv[j + 1]);   // zone(i,j).volume()
             b.rc[k] = R[i].rc;
+            b.qc[k] = 0.5 * (b.qv[j] + b.qv[j + 1]);
             b.rinv[k] = R[i].rinv;
             b.cot[k] = C[j].cot;
             const double dr = b.rv[i + 1] - b.rv[i];
@@ -574,6 +736,7 @@ struct Sim {
     double time = 0.0;
     long long iteration = 0;
     int nthreads = 1;
+    int literal = 0;              // LIT_* parts of blocks, births and model rows in literal form
     C2PStats stats;
 };
 
@@ -613,8 +776,8 @@ ErrRec block_primitives(Sim& sim, int bi, const std::vector<Cons>& U, const std:
     ErrRec err;
     long long iters_sum = 0, floors = 0;
     for (size_t k = 0; k < n; ++k) {
-        const double rdv = b.rdV[k];
-        const Cons u{U[k].d * rdv, U[k].sr * rdv, U[k].sq * rdv, U[k].tau * rdv};
+        const double dv = b.dV[k];
+        const Cons u{U[k].d / dv, U[k].sr / dv, U[k].sq / dv, U[k].tau / dv};   // state.rs:85
         int rc, it = 0;
         double value = 0.0;
         double rW = 1.0;
@@ -635,8 +798,8 @@ ErrRec block_primitives(Sim& sim, int bi, const std::vector<Cons>& U, const std:
             out[k] = PS{{0, 0, 0, 0}, 0};
             continue;
         }
-        // s = M/dV/Gamma (scheme.rs:35) with 1/dV and 1/Gamma from c2p
-        out[k].s = (M[k] * rdv) * rW;
+        // s = M/dV/Gamma (scheme.rs:35)
+        out[k].s = M[k] / dv / lorentz_factor(sim.h, out[k].P);
     }
     sim.stats.zones += (long long)n;
     sim.stats.iterations += iters_sum;
@@ -710,7 +873,8 @@ ErrRec stage(Sim& sim, const std::vector<std::vector<Cons>>& Uin, const std::vec
                 const PS L = add_half(pe(f + 1, j), gx[(size_t)(f - i0) * nq + j]);
                 const PS R = sub_half(pe(f + 2, j), gx[(size_t)(f + 1 - i0) * nq + j]);
                 Cons F; double g;
-                riemann(h, L.P, R.P, L.s, R.s, 0, &F, &g);
+                if (B.literal & LIT_FLUX) riemann_literal(h, L.P, R.P, L.s, R.s, 0, &F, &g);
+                else riemann(h, L.P, R.P, L.s, R.s, 0, &F, &g);
                 const double A = B.Ar[(size_t)f * nq + j];
                 Cons& o = Fx[(size_t)(f - i0) * nq + j];
                 o.d = F.d * c * A; o.sr = F.sr * c * A; o.sq = F.sq * c * A; o.tau = F.tau * c * A;
@@ -732,7 +896,8 @@ ErrRec stage(Sim& sim, const std::vector<std::vector<Cons>>& Uin, const std::vec
                     const PS L = add_half(pe(k, j - 1), gy[j - 1]);
                     const PS R = sub_half(pe(k, j), gy[j]);
                     Cons F; double g;
-                    riemann(h, L.P, R.P, L.s, R.s, 1, &F, &g);
+                    if (B.literal & LIT_FLUX) riemann_literal(h, L.P, R.P, L.s, R.s, 1, &F, &g);
+                    else riemann(h, L.P, R.P, L.s, R.s, 1, &F, &g);
                     const double A = B.Aq[(size_t)i * (nq + 1) + j];
                     Fy[j] = Cons{F.d * c * A, F.sr * c * A, F.sq * c * A, F.tau * c * A};
                     Gy[j] = g * c * A;
@@ -741,7 +906,7 @@ ErrRec stage(Sim& sim, const std::vector<std::vector<Cons>>& Uin, const std::vec
             for (int j = 0; j < nq; ++j) {
                 const size_t z = (size_t)i * nq + j;
                 const Prim& P = prim[b][z].P;
-                const Cons S = sources(h, P, B.rinv[z], B.cot[z]);
+                const Cons S = (B.literal & LIT_SOURCES) ? sources_literal(h, P, B.rc[z], B.qc[z]) : sources(h, P, B.rinv[z], B.cot[z]);
                 const double dv = B.dV[z];
                 const Cons sc{S.d * c * dv, S.sr * c * dv, S.sq * c * dv, S.tau * c * dv};
                 const Cons& a = Fx[(size_t)(i - i0) * nq + j];
@@ -821,7 +986,7 @@ ErrRec time_step(Sim& sim, double* dt)
 // Ghost rows of a model side at time t: BlockState::from_model on the two
 // rows of mesh block first-1 (side 0) or last+1 (side 1) next to the live
 // mesh, then try_to_primitive and s = M/dV/Gamma (src/scheme.rs:30-52),
-// with U/dV as U (1/volA)(1/dcost) like the blocks.
+// with U/dV and M/dV/Gamma divided as the reference divides.
 ErrRec model_rows(Sim& sim, int side, double t, std::vector<PS>& rows)
 {
     const om::Mesh& mesh = sim.mesh[side];
@@ -850,11 +1015,10 @@ ErrRec model_rows(Sim& sim, int side, double t, std::vector<PS>& rows)
                 continue;
             }
             const Cons c = p2c(sim.h, Prim{P[0], P[1], P[2], P[3]});
-            const double dv = R.volA * C.dcost;
+            const double dv = R.volA * C.dcost;   // cell_volume (mesh.rs:119-123)
             const Cons U{c.d * dv, c.sr * dv, c.sq * dv, c.tau * dv};
             const double M = U.d * X;
-            const double rdv = R.rvolA * C.rdcost;
-            const Cons u{U.d * rdv, U.sr * rdv, U.sq * rdv, U.tau * rdv};
+            const Cons u{U.d / dv, U.sr / dv, U.sq / dv, U.tau / dv};
             PS q{{0, 0, 0, 0}, 0};
             double rW = 1.0;
             int rc, it = 0;
@@ -872,7 +1036,7 @@ ErrRec model_rows(Sim& sim, int side, double t, std::vector<PS>& rows)
                 if (err.code == OK) err = e;
                 continue;
             }
-            q.s = (M * rdv) * rW;
+            q.s = M / dv / lorentz_factor(sim.h, q.P);
             rows[(size_t)i * nq + j] = q;
         }
     }
@@ -1083,8 +1247,8 @@ int ko_products(const Hydro* h, int nr, int nq, const double* rv, const double* 
         for (int j = 0; j < nq; ++j) {
             const PolarCol C = polar_col(qv[j], qv[j + 1]);
             const size_t k = (size_t)i * nq + j;
-            const double rdv = R.rvolA * C.rdcost;
-            const Cons u{cons[4 * k] * rdv, cons[4 * k + 1] * rdv, cons[4 * k + 2] * rdv, cons[4 * k + 3] * rdv};
+            const double dv = R.volA * C.dcost;   // cell_volume (mesh.rs:119-123)
+            const Cons u{cons[4 * k] / dv, cons[4 * k + 1] / dv, cons[4 * k + 2] / dv, cons[4 * k + 3] / dv};
             Prim P{0, 0, 0, 0};
             int rc, it = 0;
             double rW = 1.0;
@@ -1123,6 +1287,21 @@ void* ko_sim_create(const Hydro* h, int nthreads)
 
 void ko_sim_destroy(void* p) { delete (Sim*)p; }
 
+// Switches the LIT_* parts of a simulation (its blocks, births and model
+// rows) between the rewritten forms the device uses and the literal forms.
+int ko_sim_set_literal(void* p, int mask)
+{
+    Sim* s = (Sim*)p;
+    s->literal = mask & (LIT_SOURCES | LIT_FLUX);
+    for (Block& b : s->blocks) {
+        b.literal = s->literal;
+        build_geometry(b);
+    }
+    return 0;
+}
+
+
+
 // Appends a block at the outer end of the chain.  conserved: (nr,nq,4)
 // volume-integrated, as BlockState::conserved (state.rs:26-30).
 int ko_sim_push_block(void* p, int nr, int nq, const double* rv, const double* qv, const double* cons,
@@ -1131,6 +1310,7 @@ int ko_sim_push_block(void* p, int nr, int nq, const double* rv, const double* q
     Sim* s = (Sim*)p;
     Block b;
     b.nr = nr; b.nq = nq;
+    b.literal = s->literal;
     b.rv.assign(rv, rv + nr + 1);
     b.qv.assign(qv, qv + nq + 1);
     build_geometry(b);

@@ -41,6 +41,10 @@ LIGHT_SPEED = 3e10
 EULER, SRHD = 0, 1
 HLLE, HLLC = 0, 1
 FIXED, OUTFLOW = 0, 1
+# parts of Sim(literal=...): reference-code expressions, then crate arithmetic
+LIT_GEOMETRY, LIT_VOLUME, LIT_SOURCES, LIT_FLUX = 1, 2, 4, 8
+LIT_REFERENCE = LIT_GEOMETRY | LIT_VOLUME
+LIT_ALL = 15
 CODES = {0: "ok", 1: "NegativeDensity", 2: "NegativePressure", 3: "NegativeEnergyDensity",
          4: "RootFinderFailed", 14: "ModelPanics"}
 
@@ -116,6 +120,8 @@ def lib():
                      "ko_sim_num_blocks", "ko_sim_c2p_stats", "ko_sim_set_time"):
             getattr(L, name).restype = ctypes.c_int
         L.ko_sim_set_time.argtypes = [P, ctypes.c_double]
+        L.ko_sim_set_literal.argtypes = [P, ctypes.c_int]
+        L.ko_sim_set_literal.restype = ctypes.c_int
         L.ko_sim_time.restype = ctypes.c_double
         L.ko_sim_time.argtypes = [P]
         L.ko_sim_iteration.restype = ctypes.c_longlong
@@ -265,6 +271,18 @@ def products(h, rv, qv, cons, scalar_mass):
     return prim, scal
 
 
+def geometry_literal(rv, qv):
+    """Ar, Aq, dV as src/mesh.rs:119-134 writes them, zone by zone."""
+    rv = _f64(rv)
+    qv = _f64(qv)
+    nr, nq = rv.size - 1, qv.size - 1
+    Ar = np.zeros((nr + 1, nq))
+    Aq = np.zeros((nr, nq + 1))
+    dV = np.zeros((nr, nq))
+    lib().ko_geometry_literal(nr, nq, _p(rv), _p(qv), _p(Ar), _p(Aq), _p(dV))
+    return dict(Ar=Ar, Aq=Aq, dV=dV)
+
+
 def geometry(rv, qv):
     rv = _f64(rv)
     qv = _f64(qv)
@@ -292,9 +310,17 @@ class OracleError(RuntimeError):
 class Sim:
     """A radially ordered chain of blocks advanced by the oracle."""
 
-    def __init__(self, h, nthreads=1):
+    def __init__(self, h, nthreads=1, literal=False):
+        """literal=True (or a mask of LIT_*) evaluates the reference's own
+        expressions as written (LIT_GEOMETRY: per-zone frustum areas and
+        volumes; LIT_VOLUME: U/dV and M/dV/Gamma) and the textbook forms of
+        the crates' arithmetic with every quotient divided out (LIT_SOURCES,
+        LIT_FLUX), instead of the rewritten forms the device shares with the
+        default oracle."""
         self.h = h
         self._p = ctypes.c_void_p(lib().ko_sim_create(ctypes.byref(h), int(nthreads)))
+        if literal:
+            lib().ko_sim_set_literal(self._p, LIT_ALL if literal is True else int(literal))
         self.nq = None
         self.qv = None
         self.radial_vertices = []

@@ -45,11 +45,13 @@ def _near_midpoint_squares(rng, k):
 
 
 def test_fp64_arithmetic_rounds_like_host():
-    """Device division, the stage's square root and unfused multiply-add
-    round like the host's: over the whole exponent range, at the edges of the
-    square root's fast range (2^-767 and DBL_MAX) and of the compiler's range
-    scaling (subnormals, 0, inf, NaN), on 2^20 random normal inputs and on
-    inputs whose root is next to a rounding midpoint."""
+    """Device division, the stage's square root, unfused multiply-add and the
+    stage's U/dV division (from the correctly rounded 1/dV with one
+    correction) round like the host's: over the whole exponent range, at the
+    edges of the square root's fast range (2^-767 and DBL_MAX) and of the
+    compiler's range scaling (subnormals, 0, inf, NaN), on 2^20 random normal
+    inputs, on inputs whose root is next to a rounding midpoint, and on
+    divisors and dividends with significands near 1 and near 2."""
     rng = np.random.default_rng(0)
     n = 1 << 20
     a = rng.standard_normal(n) * np.exp(rng.uniform(-300, 300, n) / 10)
@@ -62,10 +64,22 @@ def test_fp64_arithmetic_rounds_like_host():
     hard = _near_midpoint_squares(rng, 1000)
     a[n // 2 + 64: n // 2 + 64 + hard.size] = hard
     b = rng.standard_normal(n) * np.exp(rng.uniform(-300, 300, n) / 10)
+    # significands of the divisor next to 2 and to 1, and of the dividend next to 2
+    m, k0 = 1 << 16, 3 * n // 4
+    e = rng.integers(-60, 60, (3, m))
+    b[k0: k0 + m] = np.ldexp(2.0 - rng.integers(1, 256, m) * 2.0 ** -52, e[0])
+    b[k0 + m: k0 + 2 * m] = np.ldexp(1.0 + rng.integers(0, 256, m) * 2.0 ** -52, e[1])
+    a[k0 + 2 * m: k0 + 3 * m] = np.ldexp(2.0 - rng.integers(1, 256, m) * 2.0 ** -52, e[2])
     c = rng.standard_normal(n)
     with np.errstate(all="ignore"):
-        div, sq, mad = knc.fp64_probe(a, b, c)
+        div, sq, mad, quot = knc.fp64_probe(a, b, c)
         assert np.array_equal(div, a / b, equal_nan=True)
+        ref = a / np.abs(b)
+        # every dividend, the divisors in the range of cell volumes, signed zeros included
+        q = (np.abs(b) >= 2.0 ** -400) & (np.abs(b) <= 2.0 ** 400)
+        assert q.sum() > 3 * n // 4
+        assert np.array_equal(quot[q], ref[q], equal_nan=True)
+        assert np.array_equal(np.signbit(quot[q]), np.signbit(ref[q]))
         assert np.array_equal(sq, np.sqrt(np.abs(a)), equal_nan=True)
         m = np.isfinite(a * b + c)
         assert np.array_equal(mad[m], (a * b + c)[m])

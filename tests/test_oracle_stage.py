@@ -29,12 +29,14 @@ def blast_setup(system=0, nr=64, nq=32, nblocks=2, solver=1, rk=2, ur=0.0):
 
 
 def totals(state):
-    return np.array([sum(c[..., 0].sum() for c, m in state), sum(m.sum() for c, m in state)])
+    """Sums of D dV, tau dV and the scalar mass over the mesh."""
+    return np.array([sum(c[..., 0].sum() for c, m in state), sum(c[..., 3].sum() for c, m in state),
+                     sum(m.sum() for c, m in state)])
 
 
 def edge_flux(h, block, row, c, m):
-    """Mass and scalar flux through the boundary face next to `row`: with
-    zero-gradient ghosts both PLM face states equal the edge row."""
+    """Mass, energy and scalar flux through the boundary face next to `row`:
+    with zero-gradient ghosts both PLM face states equal the edge row."""
     g = oracle.geometry(block.rv, block.qv)
     dv = g["dV"][row]
     prim, code, _ = oracle.try_to_primitive(h, c[row] / dv[:, None])
@@ -43,14 +45,15 @@ def edge_flux(h, block, row, c, m):
     sc = m[row] / dv / W
     F, gs = oracle.intercell_flux(h, prim, prim, sc, sc, 0)
     area = g["Ar"][0 if row == 0 else -1]
-    return np.array([(F[:, 0] * area).sum(), (gs * area).sum()])
+    return np.array([(F[:, 0] * area).sum(), (F[:, 3] * area).sum(), (gs * area).sum()])
 
 
 @pytest.mark.parametrize("system", [0, 1])
-def test_mass_and_scalar_conserved_up_to_boundary_flux(system):
-    """Sum of D dV and of M over the mesh changes only by dt times the net
-    flux through the two radial boundary faces (mass and scalar sources are
-    zero; interior and pole fluxes telescope)."""
+def test_mass_energy_and_scalar_conserved_up_to_boundary_flux(system):
+    """Sums of D dV, tau dV (E dV for Euler) and M over the mesh change only
+    by dt times the net flux through the two radial boundary faces (mass,
+    energy and scalar sources are zero; interior and pole fluxes telescope).
+    A wrong energy flux or a nonzero energy source breaks the second."""
     s = blast_setup(system, rk=1, ur=0.1)
     h = oracle_hydro(s.hydro)
     sim = oracle_from_setup(s, 4)

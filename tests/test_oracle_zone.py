@@ -4,6 +4,8 @@ These pin the restated crate arithmetic (c2p, p2c, wave speeds, sources,
 PLM, geometry) independently of the oracle's own code: every check below is
 a mathematical identity or special case, not a re-typing of its formula.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -172,6 +174,34 @@ def test_plm_brute_force(theta):
     assert ((b[mono] + 0.5 * g[mono] <= c[mono] + 1e-12) & (b[mono] - 0.5 * g[mono] >= a[mono] - 1e-12)).all()
 
 
+def _mesh_rs_face_area(r0, q0, r1, q1):
+    """face_area of src/mesh.rs:125-134, operation for operation."""
+    s0, s1 = r0 * math.sin(q0), r1 * math.sin(q1)
+    z0, z1 = r0 * math.cos(q0), r1 * math.cos(q1)
+    ds, dz = s1 - s0, z1 - z0
+    return math.pi * (s0 + s1) * math.sqrt(ds * ds + dz * dz)
+
+
+def test_geometry_is_the_reference_expression():
+    """Face areas and volumes are src/mesh.rs:119-134 evaluated zone by zone
+    as written there, bit for bit."""
+    nr, nq = 12, 9
+    rv = 10.0 ** np.linspace(0.3, 1.1, nr + 1)
+    qv = 0.1 + (2.9 / nq) * np.arange(nq + 1)
+    g = oracle.geometry(rv, qv)
+    for i in range(nr + 1):
+        for j in range(nq):
+            assert g["Ar"][i, j] == _mesh_rs_face_area(rv[i], qv[j], rv[i], qv[j + 1])
+    for i in range(nr):
+        for j in range(nq + 1):
+            assert g["Aq"][i, j] == _mesh_rs_face_area(rv[i], qv[j], rv[i + 1], qv[j])
+        for j in range(nq):
+            dcost = -(math.cos(qv[j + 1]) - math.cos(qv[j]))
+            # powi(3) is r * r * r (repeated multiplication)
+            r0, r1 = rv[i], rv[i + 1]
+            assert g["dV"][i, j] == 2.0 * math.pi * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0 * dcost
+
+
 def test_geometry_telescopes_and_closed_forms():
     nr, nq = 40, 24
     rv = 10.0 ** np.linspace(0.0, 1.0, nr + 1)
@@ -179,9 +209,14 @@ def test_geometry_telescopes_and_closed_forms():
     g = oracle.geometry(rv, qv)
     total = 2 * np.pi / 3 * (rv[-1] ** 3 - rv[0] ** 3) * (np.cos(qv[0]) - np.cos(qv[-1]))
     assert np.isclose(g["dV"].sum(), total, rtol=1e-13)
-    # polar face area = pi sin(q) (r1^2 - r0^2)
+    # every radial face is the frustum of its chord: pi r^2 (sin q0 + sin q1) 2 sin(dq/2)
+    dq = np.diff(qv)
+    ref_r = np.pi * rv[:, None] ** 2 * (np.sin(qv[:-1]) + np.sin(qv[1:]))[None, :] * (2 * np.sin(dq / 2))[None, :]
+    assert np.max(np.abs(g["Ar"] / ref_r - 1)) <= 1e-13
+    # every polar face: pi sin(q) (r1^2 - r0^2) (zero at the pole vertex q = 0)
     ref = np.pi * np.sin(qv)[None, :] * (rv[1:, None] ** 2 - rv[:-1, None] ** 2)
-    assert np.allclose(g["Aq"], ref, rtol=1e-12, atol=1e-12 * ref.max())
+    assert g["Aq"][:, 0].max() == 0.0
+    assert np.max(np.abs(g["Aq"][:, 1:] / ref[:, 1:] - 1)) <= 1e-13
     # radial faces over the full polar range sum to 4 pi r^2 (chord areas: O(dq^2))
     assert np.allclose(g["Ar"].sum(1), 4 * np.pi * rv ** 2, rtol=5e-3)
     assert np.allclose(g["rc"], np.sqrt(rv[:-1] * rv[1:])[:, None])

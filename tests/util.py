@@ -12,8 +12,8 @@ def oracle_hydro(spec):
                           spec.gamma_law_index, spec.plm_theta, spec.cfl_number, spec.light_speed)
 
 
-def oracle_from_setup(setup, nthreads=8):
-    sim = oracle.Sim(oracle_hydro(setup.hydro), nthreads)
+def oracle_from_setup(setup, nthreads=8, literal=False):
+    sim = oracle.Sim(oracle_hydro(setup.hydro), nthreads, literal=literal)
     for b in setup.blocks:
         sim.push_block(b.rv, b.qv, b.conserved, b.scalar_mass)
     for side, bd in ((0, setup.inner), (1, setup.outer)):
