@@ -756,18 +756,36 @@ double excision_surface(double r0, double speed, double delay, double t)
     return r0 + std::max(t - delay, 0.0) * speed;
 }
 
+// The outer radius add_remove_blocks compares with the excision surfaces:
+// mesh.subgrid_extent(index).outer_radius (src/scheme.rs:190,195,
+// src/mesh.rs:358-372) when the context knows the reference mesh (device
+// births or model boundaries), else the block's last radial vertex.
+double block_outer_radius(const knc_context* c, const HostBlock& b)
+{
+    const DevModel* dm = c->birth.on ? &c->birth : c->bmodel[0].on ? &c->bmodel[0] : c->bmodel[1].on ? &c->bmodel[1]
+                                                                                                        : nullptr;
+    if (!dm) return b.rv[b.nr];
+    const knc_mesh& m = dm->mesh;
+    const double block = (double)m.block_size * mesh_zone_dlogr(m);
+    return m.reference_radius * std::pow(1.0 + block, (double)b.index + 1.0);
+}
+
 // add_remove_blocks (src/scheme.rs:179-206) at state time t.  With apply
-// false only reports whether the block set would change.
+// false only reports whether the block set would change.  The innermost
+// block is never removed when it is the only one (the reference would leave
+// an empty state).
 int excise(knc_context* c, double t, bool apply, bool* change)
 {
     *change = false;
     const HostBlock& bi = c->blocks.begin()->second;
     const HostBlock& bo = c->blocks.rbegin()->second;
     const int32_t inner_index = bi.index, outer_index = bo.index;
-    const bool remove = c->blocks.size() > 1 &&
-                        bi.rv[bi.nr] < excision_surface(c->exc.inner_radius, c->exc.inner_speed, c->exc.delay, t);
+    const bool remove = c->blocks.size() > 1 && block_outer_radius(c, bi) < excision_surface(c->exc.inner_radius,
+                                                                                            c->exc.inner_speed,
+                                                                                            c->exc.delay, t);
     const bool add = (c->exc.source != nullptr || c->birth.on) &&
-                     bo.rv[bo.nr] < excision_surface(c->exc.outer_radius, c->exc.outer_speed, c->exc.delay, t);
+                     block_outer_radius(c, bo) < excision_surface(c->exc.outer_radius, c->exc.outer_speed,
+                                                                  c->exc.delay, t);
     *change = remove || add;
     if (!apply) return KNC_OK;
     int rc;

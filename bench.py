@@ -21,14 +21,14 @@ trained weights exist); the other rates, the rel-L1 of every final state
 against the plain path, and Newton-iteration statistics are reported beside
 it.
 
-Roofline of the stage kernel k_stage: algorithmic HBM bytes per zone-update
+Roofline of the headline path's stage: algorithmic HBM bytes per zone-update
 are 40 B (5 FP64 fields) read + 40 B written on stage 1 and 80 B read + 40 B
-written on stage 2 (fused RK average), i.e. 100 B per zone per stage launch
-on average; achieved = 100 B x zones / average k_stage duration in the
-plain-guess batched leg, measured with HIP events on the context's stream.
-The headline path's stage (k_stage plus its share of the guess pass) gets
-the same figure as headline_stage_roofline.  The FP64 VALU roofline
-(valu_roofline) is the kernel's actual bound.
+written on stage 2 (fused RK average), i.e. 100 B per zone per stage on
+average; achieved = 100 B x zones / the stage's average device time (k_stage
+plus its share of the learned-guess pass), measured with HIP events on the
+context's stream over the timed region.  The plain-guess batched leg's
+k_stage alone gets the same figure as plain_stage_roofline.  The FP64 VALU
+roofline (valu_roofline) is the kernel's actual bound.
 
 The default run is 2000 timed steps per GPU path (the benchmark's length).
 
@@ -93,6 +93,17 @@ def run_gpu(setup, launch_mode, steps, warmup, fold, net=None, tune=False, guess
     wall = time.perf_counter() - t0
     avg_stage_ms = sum(m * f for m, f in stage_ms) / sum(f for _, f in stage_ms)
     return ctx, wall, avg_stage_ms
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(setup, nsteps, nthreads):
@@ -357,20 +368,26 @@ def main():
     # headline: the faster learned-guess form (batched plain guess without weights)
     head = max(learned, key=lambda k: learned[k]["value"]) if learned else None
     head_rate, head_stage = (learned[head]["value"], learned[head]["stage_ms"]) if head else (rate_b, stage_ms_b)
-    # Roofline of the dominant kernel, k_stage, from the plain-guess batched
-    # leg, whose timed launches are k_stage alone (HIP events on the
-    # context's stream around each fold's graph); the headline path's stage
-    # (k_stage plus its share of the guess pass) is reported beside it.
+    # Roofline of the headline path's stage: k_stage and, for the learned
+    # paths, its share of the guess pass (k_guess once per RK step for
+    # learned_step, every stage for learned), timed together with HIP events
+    # on the context's stream around each fold's graph.  The plain-guess
+    # batched leg's stage, k_stage alone, is reported beside it.
     achieved = BYTES_PER_ZONE_STAGE * zones / (stage_ms_b * 1e-3) / 1e9
     achieved_head = BYTES_PER_ZONE_STAGE * zones / (head_stage * 1e-3) / 1e9
+    head_kernel = {"learned": "k_stage (guess plane) + k_guess every stage, batched",
+                   "learned_step": "k_stage (guess plane) + half a k_guess pass (once per RK2 step), batched",
+                   None: "k_stage (plain guess), batched"}[head]
     cpu = None
     if not args.no_cpu:
-        ncores = min(16, os.cpu_count() or 1)
+        # the host threads this process may run on, as the reference sizes its
+        # runtime from the machine's cores (src/app.rs:148-153)
+        ncores = len(os.sched_getaffinity(0))
         rate_c, wall_c = cpu_baseline(setup, args.cpu_steps, ncores)
         # the reference's default worker count is twice the core count (src/app.rs:148-153)
         steps2 = max(1, args.cpu_steps // 2)
         rate_2x, wall_2x = cpu_baseline(setup, steps2, 2 * ncores)
-        cpu = {"value": rate_c, "unit": "zone-updates/s", "cores": ncores, "kind": "port",
+        cpu = {"value": rate_c, "unit": "zone-updates/s", "cores": ncores, "kind": "port", "cpu_model": cpu_model(),
                "sample": f"oracle (C++ restatement), bomb_jet {args.nr}x{args.nq}, {args.cpu_steps} RK2 steps "
                          f"after 1 warm-up step, {wall_c:.1f} s",
                "threads_2x_cores": {"value": rate_2x, "threads": 2 * ncores,
@@ -398,12 +415,12 @@ def main():
                             [head] + " + batched persistent stage launches, hipGraph per fold"
                             if head else "batched persistent stage launches, hipGraph per fold, plain c2p guess"),
                    "zones": zones},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(None),
-                     "kernel": "k_stage (plain guess, batched)", "kernel_ms": stage_ms_b},
-        "headline_stage_roofline": {"bound": "hbm", "achieved": achieved_head, "peak": HBM_PEAK_GBS,
-                                    "unit": "GB/s", "frac": achieved_head / HBM_PEAK_GBS,
-                                    "traffic": read_traffic(head), "stage_ms": head_stage},
+        "roofline": {"bound": "hbm", "achieved": achieved_head, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_head / HBM_PEAK_GBS, "traffic": read_traffic(head),
+                     "kernel": head_kernel, "kernel_ms": head_stage},
+        "plain_stage_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(None),
+                                 "kernel": "k_stage (plain guess), batched", "kernel_ms": stage_ms_b},
         "valu_roofline": valu_roofline(zones, stage_ms_b, probe),
         "fma_probe": probe,
         "c2p_fp64_roofline": (c2p_roofline(zones, head_stage, newton[head or "plain"]["mean_iterations"])

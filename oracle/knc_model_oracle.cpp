@@ -341,4 +341,13 @@ void ko_mesh_block_vertices(const om::Mesh* mesh, int index, double* rv, double*
     om::block_vertices(*mesh, index, rv, qv);
 }
 
+// mesh.subgrid_extent(index).outer_radius (src/mesh.rs:358-372), the radius
+// add_remove_blocks compares with the excision surfaces (src/scheme.rs:190,195).
+double ko_mesh_outer_radius(const om::Mesh* mesh, int index)
+{
+    const double zone = mesh->num_radial_zones > 0 ? 1.0 / mesh->num_radial_zones : om::PI / mesh->num_polar_zones;
+    const double block = mesh->block_size * zone;
+    return mesh->reference_radius * std::pow(1.0 + block, (double)index + 1.0);
+}
+
 }  // extern "C"

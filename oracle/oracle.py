@@ -132,6 +132,8 @@ def lib():
         L.ko_sim_set_boundary_source.argtypes = [P, ctypes.c_int, BOUNDARY_SOURCE, P]
         L.ko_model_eval.restype = ctypes.c_longlong
         L.ko_model_eval.argtypes = [ctypes.POINTER(OModel), ctypes.c_longlong, P, P, ctypes.c_double, P, P]
+        L.ko_mesh_outer_radius.restype = ctypes.c_double
+        L.ko_mesh_outer_radius.argtypes = [ctypes.POINTER(OMesh), ctypes.c_int]
         L.ko_mesh_block_vertices.restype = None
         L.ko_mesh_block_vertices.argtypes = [ctypes.POINTER(OMesh), ctypes.c_int, P, P]
         L.ko_block_from_model.restype = ctypes.c_longlong
@@ -349,19 +351,25 @@ class Sim:
         """(cons, scalar_mass) of every block, inner to outer."""
         return [self.get_block(b, rv.size - 1, self.nq) for b, rv in enumerate(self.radial_vertices)]
 
-    def excise(self, ex, source, first_index):
+    def excise(self, ex, source, first_index, mesh=None):
         """add_remove_blocks (src/scheme.rs:179-206) at the current time:
         remove the inner block when its outer radius is inside the IES
         (src/scheme.rs:191-194), add block outer+1 from `source` when the
         outer block's outer radius is inside the OES (src/scheme.rs:196-205);
-        surfaces r0 + max(t - delay, 0) v (src/mesh.rs:318-336).  The only
-        block is never removed.  Returns the index of the inner block."""
+        surfaces r0 + max(t - delay, 0) v (src/mesh.rs:318-336).  A block's
+        outer radius is the mesh's subgrid_extent when `mesh` (an OMesh) is
+        given, else its last radial vertex.  The only block is never removed.
+        Returns the index of the inner block."""
         t = self.time
         ies = ex.inner_radius + max(t - ex.delay, 0.0) * ex.inner_speed
         oes = ex.outer_radius + max(t - ex.delay, 0.0) * ex.outer_speed
         outer_index = first_index + len(self.radial_vertices) - 1
-        remove = len(self.radial_vertices) > 1 and self.radial_vertices[0][-1] < ies
-        add = source is not None and self.radial_vertices[-1][-1] < oes
+
+        def outer_radius(k, index):
+            return lib().ko_mesh_outer_radius(ctypes.byref(mesh), int(index)) if mesh is not None else \
+                self.radial_vertices[k][-1]
+        remove = len(self.radial_vertices) > 1 and outer_radius(0, first_index) < ies
+        add = source is not None and outer_radius(-1, outer_index) < oes
         if remove:
             self.pop_inner_block()
             first_index += 1
@@ -371,7 +379,7 @@ class Sim:
         self.set_first_index(first_index)
         return first_index
 
-    def scheme_advance(self, fold, ex=None, source=None, first_index=0):
+    def scheme_advance(self, fold, ex=None, source=None, first_index=0, mesh=None):
         """scheme::advance (src/scheme.rs:212-240): dt from time_step once,
         then `fold` RK steps, each preceded by the excision check when the
         surfaces move (src/scheme.rs:228-232; every ex.check_every
@@ -380,7 +388,7 @@ class Sim:
         moving = ex is not None and (ex.inner_speed > 0.0 or ex.outer_speed > 0.0)
         for _ in range(fold):
             if moving and self.iteration % ex.check_every == 0:
-                first_index = self.excise(ex, source, first_index)
+                first_index = self.excise(ex, source, first_index, mesh)
             self.advance(dt, 1)
         return dt, first_index
 

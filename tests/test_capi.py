@@ -144,3 +144,39 @@ def test_compiled_driver_writes_and_reads_a_checkpoint(tmp_path):
     assert list(doc) == ["state", "tasks", "config", "version"] and doc["version"] == "knc_drive"
     st = doc["state"]["Relativistic"]
     assert st["iteration"] == [30, 1] and sorted(st["solution"]) == [(k, 0) for k in range(4)]
+
+
+def _rust_impl(notes, name):
+    m = re.search(r"impl GpuHydro for " + name + r"\b.*?\n\}\n", notes, re.S)
+    assert m, name
+    return m.group(0)
+
+
+@pytest.mark.parametrize("name", ["RelativisticHydro", "NewtonianHydro"])
+def test_integration_hydro_mapping_matches_the_reference_fields(name):
+    """The KncHydro built in INTEGRATION.md reads every field of the
+    reference's struct with an expression its type allows: f64 fields
+    directly, bool fields cast, enum fields through a match on their
+    variants (field types from tests/golden/serde_layout.json, read from the
+    reference's declarations)."""
+    import json
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "serde_layout.json")))
+    fields = {f["name"]: f["type"] for f in golden["structs"][name]["fields"]}
+    notes = open(os.path.join(os.path.dirname(HEADER), "..", "INTEGRATION.md")).read()
+    body = _rust_impl(notes, name)
+    used = set(re.findall(r"self\.([a-z_]+)", body))
+    assert used == set(fields), (used, fields)
+    for f, t in fields.items():
+        uses = [(body[max(0, m.start() - 12):m.start()], m.group(1))
+                for m in re.finditer(r"self\." + f + r"\b([^,\n]*?)(?=,|\n| \{)", body)]
+        assert uses, f
+        for pre, post in uses:
+            if t == "f64":
+                assert post.strip() == "" and pre.endswith(": "), (f, pre, post)
+            elif t == "bool":
+                assert post.strip() == "as i32", (f, post)
+            else:   # an enum: matched on its variants, never cast
+                assert "as " not in post and (pre.endswith("match ") or pre.endswith("rk_order(&")), (f, pre, post)
+    if "runge_kutta_order" in fields:
+        fn = re.search(r"fn rk_order\(.*?\n\}", notes, re.S).group(0)
+        assert all(f"RungeKuttaOrder::RK{k} => {k}" in fn for k in (1, 2, 3))

@@ -142,9 +142,10 @@ def _run_pair(preset, folds, fold, excise, one_step=1e-12, conditioned=("S_r", "
     noise1 = _self_noise(preset, ctx, [1]) if one_step > 1e-11 else None
     ex = preset_excision(preset) if excise else None
     src = oracle_birth_source(sim, preset) if excise else None
+    mesh = oracle_mesh(preset) if excise else None
     first = 0
     start = rel_l1(gpu_state(ctx), sim.blocks())
-    dt, first = sim.scheme_advance(1, ex, src, first)
+    dt, first = sim.scheme_advance(1, ex, src, first, mesh)
     assert ctx.advance(1) == pytest.approx(dt, rel=1e-14)
     assert ctx.block_indexes()[0] == first
     err = rel_l1(gpu_state(ctx), sim.blocks())
@@ -152,7 +153,7 @@ def _run_pair(preset, folds, fold, excise, one_step=1e-12, conditioned=("S_r", "
         bound = max(one_step, 10 * noise1[f]) if noise1 is not None and f in conditioned else min(one_step, 1e-11)
         assert e <= bound, (f, start, err, noise1)
     for _ in range(folds):
-        dt, first = sim.scheme_advance(fold, ex, src, first)
+        dt, first = sim.scheme_advance(fold, ex, src, first, mesh)
         assert ctx.advance(fold) == pytest.approx(dt, rel=1e-12)
     assert ctx.time == pytest.approx(sim.time, rel=1e-13)
     assert list(ctx.block_indexes()) == list(range(first, first + sim.num_blocks()))
