@@ -193,7 +193,8 @@ namespace {
 // The network supplies guesses (k_guess passes, a network must be attached).
 bool learned(const knc_context* c)
 {
-    return (c->o.c2p_guess == KNC_GUESS_LEARNED || c->o.c2p_guess == KNC_GUESS_LEARNED_STEP) &&
+    return (c->o.c2p_guess == KNC_GUESS_LEARNED || c->o.c2p_guess == KNC_GUESS_LEARNED_STEP ||
+            c->o.c2p_guess == KNC_GUESS_LEARNED_FOLD) &&
            c->h.system == KNC_SRHD;
 }
 // The stage's c2p starts Newton from the guess planes (learned or previous).
@@ -489,15 +490,20 @@ int row_pass_grid(const knc_context* c, int nitems)
 // one.  The passes before the stage kernels (the learned-guess pass, the c2p
 // pass of the two-pass stage) cover every block first, since a block's stage
 // also reads its neighbours' edge rows.
-int launch_one_stage(knc_context* c, StageArgs a, bool first_stage)
+int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment_start)
 {
     const bool stats = c->o.collect_stats != 0;
     const bool prims = two_pass(c);
     const bool lg = guess_read(c);
     // KNC_GUESS_LEARNED_STEP evaluates the network on the first stage's input
     // only; the later stages of the step start Newton from the same guesses.
+    // KNC_GUESS_LEARNED_FOLD evaluates it on the first stage of a run of
+    // steps with an unchanged block set (a fold, or the part of one between
+    // excision events), whose stages all start from those guesses.
     // KNC_GUESS_PREVIOUS has no pass: each stage stores its own pressures.
-    const bool guess_pass = learned(c) && (first_stage || c->o.c2p_guess == KNC_GUESS_LEARNED);
+    const int mode = c->o.c2p_guess;
+    const bool guess_pass = learned(c) && (mode == KNC_GUESS_LEARNED || (mode == KNC_GUESS_LEARNED_STEP && first_stage) ||
+                                           (mode == KNC_GUESS_LEARNED_FOLD && first_stage && segment_start));
     auto pre_pass = [&](const StageArgs& b, int nitems) -> int {
         const int grid = row_pass_grid(c, nitems);
         if (guess_pass) {
@@ -557,7 +563,8 @@ void stage_boundaries(knc_context* c, StageArgs& a, int fold_step, int stage)
 
 // Guess planes of a stage: the c2p starts from the plane of the stage's
 // input slot (with KNC_GUESS_LEARNED_STEP, the plane the network wrote for
-// the step's first stage) and KNC_GUESS_PREVIOUS stores the recovered
+// the step's first stage; with KNC_GUESS_LEARNED_FOLD, the one it wrote for
+// the first stage of the run of steps) and KNC_GUESS_PREVIOUS stores the recovered
 // pressures into the plane of the output slot, which no workgroup of the
 // launch reads.
 void stage_slots(knc_context* c, StageArgs& a, int in, int out, int avg, double bweight)
@@ -566,11 +573,15 @@ void stage_slots(knc_context* c, StageArgs& a, int in, int out, int avg, double 
     a.out_slot = out;
     a.avg_slot = avg;
     a.bweight = bweight;
-    a.guess_in = c->o.c2p_guess == KNC_GUESS_LEARNED_STEP ? c->cur : in;
+    a.guess_in = c->o.c2p_guess == KNC_GUESS_LEARNED_STEP ? c->cur
+                 : c->o.c2p_guess == KNC_GUESS_LEARNED_FOLD ? c->segment_slot
+                                                            : in;
     a.guess_out = out;
 }
 
-int enqueue_rk_step(knc_context* c, int step, int fold_step)
+// `segment_start`: the first step of a run of steps over one block set
+// (c->segment_slot holds its input slot).
+int enqueue_rk_step(knc_context* c, int step, int fold_step, bool segment_start)
 {
     const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
     StageArgs a = base_args(c);
@@ -580,28 +591,28 @@ int enqueue_rk_step(knc_context* c, int step, int fold_step)
     case 1:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
-        if ((rc = launch_one_stage(c, a, true))) return rc;
+        if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
         c->cur = f1;
         break;
     case 2:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
-        if ((rc = launch_one_stage(c, a, true))) return rc;
+        if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
         stage_slots(c, a, f1, f2, c->cur, 0.5);
         stage_boundaries(c, a, fold_step, 1);
-        if ((rc = launch_one_stage(c, a, false))) return rc;
+        if ((rc = launch_one_stage(c, a, false, segment_start))) return rc;
         c->cur = f2;
         break;
     default:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
-        if ((rc = launch_one_stage(c, a, true))) return rc;
+        if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
         stage_slots(c, a, f1, f2, c->cur, 0.75);
         stage_boundaries(c, a, fold_step, 1);
-        if ((rc = launch_one_stage(c, a, false))) return rc;
+        if ((rc = launch_one_stage(c, a, false, segment_start))) return rc;
         stage_slots(c, a, f2, f1, c->cur, 1.0 / 3.0);
         stage_boundaries(c, a, fold_step, 2);
-        if ((rc = launch_one_stage(c, a, false))) return rc;
+        if ((rc = launch_one_stage(c, a, false, segment_start))) return rc;
         c->cur = f1;
         break;
     }
@@ -708,11 +719,12 @@ int enqueue_steps(knc_context* c, int n, int first)
 {
     if (n <= 0) return KNC_OK;
     const int start_slot = c->cur;
+    c->segment_slot = start_slot;
     const bool graph = c->o.launch_mode == KNC_LAUNCH_BATCHED && c->o.use_graph && n > 1;
     if (!graph) {
         int rc;
         for (int s = 0; s < n; ++s)
-            if ((rc = enqueue_rk_step(c, s + 1, first + s))) return rc;
+            if ((rc = enqueue_rk_step(c, s + 1, first + s, s == 0))) return rc;
         return KNC_OK;
     }
     // the ghost-row pointers of model boundaries depend on the fold step
@@ -724,7 +736,7 @@ int enqueue_steps(knc_context* c, int n, int first)
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int crc = KNC_OK;
         const int launches = c->last_launches;
-        for (int s = 0; s < n && crc == KNC_OK; ++s) crc = enqueue_rk_step(c, s + 1, first + s);
+        for (int s = 0; s < n && crc == KNC_OK; ++s) crc = enqueue_rk_step(c, s + 1, first + s, s == 0);
         hipError_t ce = hipStreamEndCapture(c->stream, &g);
         c->cur = start_slot;
         c->last_launches = launches;
@@ -998,7 +1010,8 @@ int time_shape(knc_context* c, int threads, int rows, int passes, int trials, do
     c->trial = true;
     for (int w = 0; w < trials + 1; ++w) {
         if (w == 1) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-        rc = enqueue_rk_step(c, 1, 0);
+        c->segment_slot = c->cur;
+        rc = enqueue_rk_step(c, 1, 0, true);
         c->cur = saved;
         if (rc) {
             c->trial = false;
@@ -1132,7 +1145,8 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
         c->o.use_graph = 0;
     }
     if (c->o.c2p_guess != KNC_GUESS_PLAIN && c->o.c2p_guess != KNC_GUESS_LEARNED &&
-        c->o.c2p_guess != KNC_GUESS_LEARNED_STEP && c->o.c2p_guess != KNC_GUESS_PREVIOUS) {
+        c->o.c2p_guess != KNC_GUESS_LEARNED_STEP && c->o.c2p_guess != KNC_GUESS_PREVIOUS &&
+        c->o.c2p_guess != KNC_GUESS_LEARNED_FOLD) {
         delete c;
         return fail(KNC_ERR_INVALID_ARGUMENT, "unknown c2p guess mode");
     }

@@ -62,6 +62,7 @@ struct knc_context {
 
     std::map<int32_t, knc::HostBlock> blocks;  // ordered by radial index
     int cur = 0;                          // slot holding U^n
+    int segment_slot = 0;                 // input slot of the current run of steps (KNC_GUESS_LEARNED_FOLD)
     double time = 0.0;
     int64_t iteration = 0;
 

@@ -3,7 +3,8 @@
 ``capi`` binds the C ABI of ``libknc_gpu.so`` (include/knc_gpu.h);
 ``setups`` builds the seeded benchmark workloads.
 """
-from .capi import (BATCHED, EULER, FIXED, GUESS_LEARNED, GUESS_LEARNED_STEP, GUESS_PLAIN, GUESS_PREVIOUS, HLLC, HLLE,
+from .capi import (BATCHED, EULER, FIXED, GUESS_LEARNED, GUESS_LEARNED_FOLD, GUESS_LEARNED_STEP, GUESS_PLAIN,
+                   GUESS_PREVIOUS, HLLC, HLLE,
                    INNER, OUTER, OUTFLOW, PER_BLOCK, POLAR, RADIAL, SRHD, Context, Hydro, HydroError, fp64_fma_rate,
                    fp64_probe, geometrical_source_terms, intercell_flux, lib, max_signal_speed, newtonian_hydro,
                    plm_gradient, relativistic_hydro, to_conserved, try_to_primitive)

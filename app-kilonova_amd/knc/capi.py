@@ -28,7 +28,7 @@ RADIAL, POLAR = 0, 1
 FIXED, OUTFLOW = 0, 1
 INNER, OUTER = 0, 1
 PER_BLOCK, BATCHED = 0, 1
-GUESS_PLAIN, GUESS_LEARNED, GUESS_LEARNED_STEP, GUESS_PREVIOUS = 0, 1, 2, 3
+GUESS_PLAIN, GUESS_LEARNED, GUESS_LEARNED_STEP, GUESS_PREVIOUS, GUESS_LEARNED_FOLD = 0, 1, 2, 3, 4
 
 ERROR_KINDS = {1: "NegativeDensity", 2: "NegativePressure", 3: "NegativeEnergyDensity",
                4: "RootFinderFailed"}

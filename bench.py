@@ -126,7 +126,8 @@ def newton_stats(setup, fold, net, steps=20):
     import knc
     out = {}
     for name, n, guess in (("plain", None, knc.GUESS_PLAIN), ("learned", net, knc.GUESS_LEARNED),
-                           ("learned_step", net, knc.GUESS_LEARNED_STEP), ("previous", None, knc.GUESS_PREVIOUS)):
+                           ("learned_step", net, knc.GUESS_LEARNED_STEP), ("learned_fold", net, knc.GUESS_LEARNED_FOLD),
+                           ("previous", None, knc.GUESS_PREVIOUS)):
         if name.startswith("learned") and n is None:
             continue
         ctx = knc.context_from_setup(setup, launch_mode=knc.BATCHED, c2p_guess=guess, collect_stats=True,
@@ -324,7 +325,8 @@ def main():
     learned = {}
     ctx_l = {}
     if net is not None:
-        for name, guess in (("learned", knc.GUESS_LEARNED), ("learned_step", knc.GUESS_LEARNED_STEP)):
+        for name, guess in (("learned", knc.GUESS_LEARNED), ("learned_step", knc.GUESS_LEARNED_STEP),
+                            ("learned_fold", knc.GUESS_LEARNED_FOLD)):
             ctx, wall, stage_ms = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, net=net,
                                           tune=tune, guess=guess, **shape)
             rate = zones * args.steps / wall
@@ -377,6 +379,8 @@ def main():
     achieved_head = BYTES_PER_ZONE_STAGE * zones / (head_stage * 1e-3) / 1e9
     head_kernel = {"learned": "k_stage (guess plane) + k_guess every stage, batched",
                    "learned_step": "k_stage (guess plane) + half a k_guess pass (once per RK2 step), batched",
+                   "learned_fold": f"k_stage (guess plane) + one k_guess pass per fold ({2 * args.fold} stages), "
+                                   "batched",
                    None: "k_stage (plain guess), batched"}[head]
     cpu = None
     if not args.no_cpu:
@@ -411,7 +415,8 @@ def main():
         "config": {"workload": f"bomb_jet SRHD kinetic bomb + jet, {args.nr}x{args.nq} zones, "
                                f"{len(setup.blocks)} uneven blocks, HLLE, RK2, fold {args.fold}",
                    "path": ({"learned": "learned c2p guess evaluated for every stage",
-                             "learned_step": "learned c2p guess evaluated once per RK step (first stage's input)"}
+                             "learned_step": "learned c2p guess evaluated once per RK step (first stage's input)",
+                             "learned_fold": "learned c2p guess evaluated once per fold (first stage's input)"}
                             [head] + " + batched persistent stage launches, hipGraph per fold"
                             if head else "batched persistent stage launches, hipGraph per fold, plain c2p guess"),
                    "zones": zones},

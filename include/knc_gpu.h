@@ -45,10 +45,19 @@ enum { KNC_LAUNCH_PER_BLOCK = 0, KNC_LAUNCH_BATCHED = 1 };
 /* c2p initial pressure: PLAIN p = 0; LEARNED the network's guess for every
  * stage's input; LEARNED_STEP the network evaluated on the first stage's
  * input of each RK step, its guesses reused by the step's later stages;
+ * LEARNED_FOLD the network evaluated on the first stage's input of each run
+ * of steps over one block set (the steps of a knc_advance fold, split at
+ * excision events), its guesses reused by every later stage of the run;
  * PREVIOUS the pressure each zone's previous stage recovered (p = 0 for a
  * zone with none yet: a new block's first stage).  Newton ends in a
  * guess-independent polish, so every mode gives the plain mode's state. */
-enum { KNC_GUESS_PLAIN = 0, KNC_GUESS_LEARNED = 1, KNC_GUESS_LEARNED_STEP = 2, KNC_GUESS_PREVIOUS = 3 };
+enum {
+    KNC_GUESS_PLAIN = 0,
+    KNC_GUESS_LEARNED = 1,
+    KNC_GUESS_LEARNED_STEP = 2,
+    KNC_GUESS_PREVIOUS = 3,
+    KNC_GUESS_LEARNED_FOLD = 4
+};
 
 /* return codes: hydro errors mirror HydroErrorType (src/physics/mod.rs:82-95) */
 enum {
@@ -93,7 +102,7 @@ typedef struct knc_options {
     int32_t launch_mode;     /* KNC_LAUNCH_PER_BLOCK: one stage launch per block;
                                 KNC_LAUNCH_BATCHED: one persistent launch per stage
                                 over every block, fold steps replayed as a hipGraph */
-    int32_t c2p_guess;       /* KNC_GUESS_PLAIN | KNC_GUESS_LEARNED | KNC_GUESS_LEARNED_STEP */
+    int32_t c2p_guess;       /* KNC_GUESS_PLAIN | _LEARNED | _LEARNED_STEP | _PREVIOUS | _LEARNED_FOLD */
     int32_t collect_stats;   /* 1: Newton iteration histogram and fallback counts */
     int32_t rows_per_item;   /* radial rows per work item (<= 64); 0 = chosen from the mesh */
     int32_t device;          /* HIP device ordinal */

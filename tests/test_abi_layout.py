@@ -36,7 +36,8 @@ ENUMS = {"KNC_EULER": capi.EULER, "KNC_SRHD": capi.SRHD, "KNC_HLLE": capi.HLLE, 
          "KNC_BOUNDARY_FIXED": capi.FIXED, "KNC_BOUNDARY_OUTFLOW": capi.OUTFLOW, "KNC_SIDE_INNER": capi.INNER,
          "KNC_SIDE_OUTER": capi.OUTER, "KNC_LAUNCH_PER_BLOCK": capi.PER_BLOCK, "KNC_LAUNCH_BATCHED": capi.BATCHED,
          "KNC_GUESS_PLAIN": capi.GUESS_PLAIN, "KNC_GUESS_LEARNED": capi.GUESS_LEARNED,
-         "KNC_GUESS_LEARNED_STEP": capi.GUESS_LEARNED_STEP, "KNC_GUESS_PREVIOUS": capi.GUESS_PREVIOUS, "KNC_ERR_MODEL": capi.ERR_MODEL,
+         "KNC_GUESS_LEARNED_STEP": capi.GUESS_LEARNED_STEP, "KNC_GUESS_PREVIOUS": capi.GUESS_PREVIOUS,
+         "KNC_GUESS_LEARNED_FOLD": capi.GUESS_LEARNED_FOLD, "KNC_ERR_MODEL": capi.ERR_MODEL,
          "KNC_MODEL_KINETIC_BOMB": capi.KINETIC_BOMB, "KNC_MODEL_JET_IN_CLOUD": capi.JET_IN_CLOUD,
          "KNC_MODEL_JET_IN_STAR": capi.JET_IN_STAR, "KNC_MODEL_HALO_KILONOVA": capi.HALO_KILONOVA,
          "KNC_MODEL_WIND_SHOCK": capi.WIND_SHOCK, "KNC_MODEL_NPARAM": capi.MODEL_NPARAM,

@@ -88,7 +88,7 @@ def test_context_errors_are_reported_not_raised_in_c():
 def test_invalid_options_are_rejected_before_any_device_call():
     h = knc.relativistic_hydro()
     out = ctypes.c_void_p()
-    for bad in (capi.Options(1, 4, 0, 0, 0, 1, 0, 0),       # unknown c2p guess mode
+    for bad in (capi.Options(1, 5, 0, 0, 0, 1, 0, 0),       # unknown c2p guess mode
                 capi.Options(1, 0, 0, 0, 0, 1, 0, 3)):      # stage_passes must be 0, 1 or 2
         rc = knc.lib().knc_context_create(ctypes.byref(h), ctypes.byref(bad), ctypes.byref(out))
         assert rc == 10 and out.value is None

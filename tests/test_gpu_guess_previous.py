@@ -57,6 +57,26 @@ def test_previous_guess_across_births_and_removals():
     assert prev.c2p_stats()["fallbacks"] == 0
 
 
+def test_fold_learned_guess_across_births_and_removals():
+    """KNC_GUESS_LEARNED_FOLD evaluates the network once per run of steps
+    over one block set: with blocks born and removed every 5 steps each run
+    is re-guessed, born blocks included, and the state is the plain one."""
+    from knc import train_guess
+    print("learned-per-fold guess vs plain: wind_shock blocks cycling every 5 steps", flush=True)
+    s = knc.setups.wind_shock(nq=64, nblocks=16, choices=(8, 16, 32, 64), check_every=5)
+    plain = knc.context_from_setup(s, excision=True, launch_mode=knc.BATCHED)
+    fold = knc.context_from_setup(s, excision=True, launch_mode=knc.BATCHED, c2p_guess=knc.GUESS_LEARNED_FOLD,
+                                  collect_stats=True)
+    train_guess.load().attach(fold)
+    for _ in range(3):
+        assert plain.advance(10) == fold.advance(10)
+    assert fold.excision_counts() == (5, 5)
+    assert plain.block_indexes() == fold.block_indexes()
+    assert _same(plain, fold)
+    st = fold.c2p_stats()
+    assert st["fallbacks"] <= 1e-3 * st["zones"]
+
+
 def test_previous_guess_is_plain_for_euler():
     """Euler's c2p has no Newton solve: the mode changes nothing."""
     s = knc.setups.euler_blast(nr=128, nq=32)

@@ -44,17 +44,19 @@ def test_harvested_pairs_match_the_oracle_c2p():
     assert np.median(err) < 1.0
 
 
-def test_committed_network_on_the_benchmark_mesh():
-    """KNC_GUESS_LEARNED_STEP + batched with the committed network against the
-    plain batched path on bomb_jet 4096 x 1024, 20 steps in folds of 10: the
-    same state bit for bit (bar rel-L1 <= 1e-9), no fallback, fewer Newton
-    evaluations per zone."""
-    print("learned_step vs plain: bomb_jet 4096x1024, 20 steps", flush=True)
+@pytest.mark.parametrize("mode", [knc.GUESS_LEARNED_STEP, knc.GUESS_LEARNED_FOLD])
+def test_committed_network_on_the_benchmark_mesh(mode):
+    """The learned guess, evaluated once per RK step or once per fold, with
+    the committed network against the plain batched path on bomb_jet
+    4096 x 1024, 20 steps in folds of 10: the same state bit for bit (bar
+    rel-L1 <= 1e-9), fewer Newton evaluations per zone, and no fallback to
+    the plain guess when the guesses are the step's own (fewer than one zone
+    in 10^4 when they are up to a fold old)."""
+    print(f"learned (mode {mode}) vs plain: bomb_jet 4096x1024, 20 steps", flush=True)
     s = knc.setups.bomb_jet()
     net = train_guess.load()
     plain = knc.context_from_setup(s, launch_mode=knc.BATCHED, collect_stats=True)
-    learned = knc.context_from_setup(s, launch_mode=knc.BATCHED, collect_stats=True,
-                                     c2p_guess=knc.GUESS_LEARNED_STEP)
+    learned = knc.context_from_setup(s, launch_mode=knc.BATCHED, collect_stats=True, c2p_guess=mode)
     net.attach(learned)
     for _ in range(2):
         assert plain.advance(10) == learned.advance(10)
@@ -64,5 +66,5 @@ def test_committed_network_on_the_benchmark_mesh():
           f"learned {_mean_iterations(sl):.3f}; fallbacks {sl['fallbacks']}", flush=True)
     assert max(err.values()) <= 1e-9, err
     assert sl["zones"] == sp["zones"] > 0
-    assert sl["fallbacks"] == 0
+    assert sl["fallbacks"] <= (0 if mode == knc.GUESS_LEARNED_STEP else 1e-4 * sl["zones"])
     assert _mean_iterations(sl) < _mean_iterations(sp) - 0.5

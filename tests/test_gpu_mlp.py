@@ -79,7 +79,7 @@ def test_trained_network_reduces_newton_iterations(trained):
     assert learned_it.mean() < plain_it.mean() - 0.3
 
 
-@pytest.mark.parametrize("mode", [knc.GUESS_LEARNED, knc.GUESS_LEARNED_STEP])
+@pytest.mark.parametrize("mode", [knc.GUESS_LEARNED, knc.GUESS_LEARNED_STEP, knc.GUESS_LEARNED_FOLD])
 def test_learned_batched_path_matches_plain(trained, mode):
     net, _ = trained
     s = knc.setups.jet_in_cloud(nr=256, nq=64, nblocks=4, corr=4.0)
