@@ -202,7 +202,9 @@ bool guess_read(const knc_context* c)
 {
     return learned(c) || (c->o.c2p_guess == KNC_GUESS_PREVIOUS && c->h.system == KNC_SRHD);
 }
-bool two_pass(const knc_context* c) { return c->o.stage_passes == 2; }
+// The stage form (STAGE_FUSED, STAGE_TWO_PASS, STAGE_SPECIALISED).
+int stage_form(const knc_context* c) { return c->o.stage_passes == 0 ? STAGE_FUSED : c->o.stage_passes; }
+bool two_pass(const knc_context* c) { return stage_form(c) == STAGE_TWO_PASS; }
 // A side whose ghost rows are refreshed at every stage time (host callback
 // or device model).
 bool model_side(const knc_context* c, int side) { return c->bsrc[side] || c->bmodel[side].on; }
@@ -318,8 +320,9 @@ int choose_stage_shape(knc_context* c)
 {
     const int nq = c->nq;
     int t = c->o.stage_threads;
-    if (t <= 0) t = nq <= 64 ? 64 : nq <= 128 ? 128 : 256;
-    if (!stage_variant_supported(t)) return fail(KNC_ERR_INVALID_ARGUMENT, "unsupported stage workgroup size %d", t);
+    if (t <= 0) t = nq <= 64 && stage_form(c) != STAGE_SPECIALISED ? 64 : nq <= 128 ? 128 : 256;
+    if (!stage_variant_supported(t, stage_form(c)))
+        return fail(KNC_ERR_INVALID_ARGUMENT, "unsupported stage shape: %d columns in form %d", t, stage_form(c));
     c->threads = t;
     return KNC_OK;
 }
@@ -347,14 +350,14 @@ int upload_theta(knc_context* c, int nq, const double* qv)
 
 int choose_rows(knc_context* c)
 {
-    if (c->o.rows_per_item > 0) return std::min(c->o.rows_per_item, stage_max_rows());
+    if (c->o.rows_per_item > 0) return std::min(c->o.rows_per_item, stage_max_rows(stage_form(c)));
     // Enough work items for one full round of resident workgroups, while
     // keeping the 4 recomputed halo rows per item a small fraction.
     int64_t rows = 0;
     for (auto& kv : c->blocks) rows += kv.second.nr;
     const int tiles = (c->nq + c->threads - 1) / c->threads;
     const int occ = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0, guess_read(c),
-                                                two_pass(c), c->threads));
+                                                stage_form(c), c->threads));
     const int64_t want = (int64_t)c->num_cu * occ;
     int64_t r = rows * tiles / std::max<int64_t>(1, want);
     return (int)std::max<int64_t>(4, std::min<int64_t>(32, r));
@@ -429,7 +432,7 @@ int rebuild_tables(knc_context* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     // Persistent grid: every CU holds as many stage workgroups as fit.
     const int per_cu = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
-                                                   guess_read(c), two_pass(c), c->threads));
+                                                   guess_read(c), stage_form(c), c->threads));
     c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
     c->dirty = false;
     return KNC_OK;
@@ -521,8 +524,8 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment
         a.items = c->d_items;
         a.nitems_ptr = c->d_nitems;
         if ((rc = pre_pass(a, c->nitems))) return rc;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, prims, a, c->persistent_grid, c->threads,
-                             c->stream));
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, stage_form(c), a, c->persistent_grid,
+                             c->threads, c->stream));
         c->last_launches += 1;
         return KNC_OK;
     }
@@ -536,7 +539,8 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment
         a.items = c->d_items + bi.first;
         a.nitems_ptr = nullptr;
         a.nitems = bi.second;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, prims, a, bi.second, c->threads, c->stream));
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, stage_form(c), a, bi.second, c->threads,
+                             c->stream));
         c->last_launches += 1;
     }
     return KNC_OK;
@@ -567,6 +571,16 @@ void stage_boundaries(knc_context* c, StageArgs& a, int fold_step, int stage)
 // the first stage of the run of steps) and KNC_GUESS_PREVIOUS stores the recovered
 // pressures into the plane of the output slot, which no workgroup of the
 // launch reads.
+// Error-key sequence number of stage `stage` of step `step` (1-based) of a
+// run of steps: 2k for the k-th stage of the run, 2k - 1 for the model ghost
+// rows it reads (evaluated before the stage, as scheme::advance refreshes
+// the boundary before each stage); the time-step pass before the run is 0.
+int stage_seq(const knc_context* c, int step, int stage, bool ghost = false)
+{
+    const int64_t k = (int64_t)(step - 1) * c->h.runge_kutta_order + stage + 1;
+    return (int)std::min<int64_t>(2 * k - (ghost ? 1 : 0), (int64_t)ERR_SEQ_MAX);
+}
+
 void stage_slots(knc_context* c, StageArgs& a, int in, int out, int avg, double bweight)
 {
     a.in_slot = in;
@@ -579,39 +593,45 @@ void stage_slots(knc_context* c, StageArgs& a, int in, int out, int avg, double 
     a.guess_out = out;
 }
 
-// `segment_start`: the first step of a run of steps over one block set
-// (c->segment_slot holds its input slot).
+// `step`: 1 for the first step of the run of steps; `segment_start`: the
+// first step of a run of steps over one block set (c->segment_slot holds its
+// input slot).  Each stage logs its failures under its own sequence number.
 int enqueue_rk_step(knc_context* c, int step, int fold_step, bool segment_start)
 {
     const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
     StageArgs a = base_args(c);
-    a.step = step;
     int rc;
     switch (c->h.runge_kutta_order) {
     case 1:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
+        a.step = stage_seq(c, step, 0);
         if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
         c->cur = f1;
         break;
     case 2:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
+        a.step = stage_seq(c, step, 0);
         if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
         stage_slots(c, a, f1, f2, c->cur, 0.5);
         stage_boundaries(c, a, fold_step, 1);
+        a.step = stage_seq(c, step, 1);
         if ((rc = launch_one_stage(c, a, false, segment_start))) return rc;
         c->cur = f2;
         break;
     default:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
+        a.step = stage_seq(c, step, 0);
         if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
         stage_slots(c, a, f1, f2, c->cur, 0.75);
         stage_boundaries(c, a, fold_step, 1);
+        a.step = stage_seq(c, step, 1);
         if ((rc = launch_one_stage(c, a, false, segment_start))) return rc;
         stage_slots(c, a, f2, f1, c->cur, 1.0 / 3.0);
         stage_boundaries(c, a, fold_step, 2);
+        a.step = stage_seq(c, step, 2);
         if ((rc = launch_one_stage(c, a, false, segment_start))) return rc;
         c->cur = f1;
         break;
@@ -664,11 +684,25 @@ int knc::collect_error(knc_context* c, knc_error* err)
     if (key == ~0ull) return KNC_OK;
     ErrLog log;
     HIP_TRY(hipMemcpy(&log, c->d_err, sizeof log, hipMemcpyDeviceToHost));
-    const int order = (int)((key >> 32) & 0xffff), row = (int)((key >> 16) & 0xffff), col = (int)(key & 0xffff);
+    const int order = err_key_order(key), row = err_key_row(key), col = err_key_col(key);
     int code = KNC_HYDRO_ROOT_FINDER_FAILED;
     knc_error e{};
     e.i = row;
     e.j = col;
+    if ((order == GHOST_INNER_ORDER || order == GHOST_OUTER_ORDER) && !c->blocks.empty()) {
+        // a model ghost row: reported as the zone of the neighbour block the
+        // reference's mesh puts there (rows block_size - 2, block_size - 1
+        // inside the mesh, rows 0, 1 outside it)
+        const int side = order == GHOST_OUTER_ORDER;
+        const knc_mesh& m = c->bmodel[side].mesh;
+        const int32_t inner = c->blocks.begin()->first, outer = c->blocks.rbegin()->first;
+        e.block = side ? outer + 1 : inner - 1;
+        e.i = side ? row : m.block_size - 2 + row;
+        if (c->bmodel[side].on && row < 2 && col < c->nq) {
+            e.r = ghost_rows(m, side, inner, outer).rc[row];
+            e.theta = 0.5 * (c->qv[col] + c->qv[col + 1]);
+        }
+    }
     int o = 0;
     for (auto& kv : c->blocks) {
         if (o++ == order) {
@@ -895,7 +929,7 @@ int flush_steps(knc_context* c, int first, int n)
                     const int tag0 = (side ? GHOST_OUTER_ORDER : GHOST_INNER_ORDER) << 16;
                     HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g[side], c->d_qc,
                                               c->th.dcost, c->stage_times[(size_t)s * order + st],
-                                              s - first + 1, tag0, out, c->d_err, c->stream));
+                                              stage_seq(c, s - first + 1, st, true), tag0, out, c->d_err, c->stream));
                 }
     }
     return enqueue_steps(c, n, first);
@@ -1066,13 +1100,13 @@ int autotune(knc_context* c, int trials, bool force, knc_tune_result* out)
     int ncand = 0;
     const int widths[] = {64, 128, 256, 512};
     const int rows[] = {4, 8, 16, 32, 64};
-    for (int passes = 1; passes <= 2 && rc == KNC_OK; ++passes)
+    for (int passes = 1; passes <= 3 && rc == KNC_OK; ++passes)
         for (int t : widths) {
             if (rc) break;
-            if (!stage_variant_supported(t)) continue;
+            if (!stage_variant_supported(t, passes)) continue;
             if (t > 64 && t / 2 >= c->nq) continue;   // wider than the mesh
             for (int nr : rows) {
-                if (nr > stage_max_rows()) continue;
+                if (nr > stage_max_rows(passes)) continue;
                 double ms = 0.0;
                 if ((rc = time_shape(c, t, nr, passes, trials, &ms))) break;
                 ++ncand;
@@ -1150,9 +1184,9 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
         delete c;
         return fail(KNC_ERR_INVALID_ARGUMENT, "unknown c2p guess mode");
     }
-    if (c->o.stage_passes < 0 || c->o.stage_passes > 2) {
+    if (c->o.stage_passes < 0 || c->o.stage_passes > 3) {
         delete c;
-        return fail(KNC_ERR_INVALID_ARGUMENT, "stage_passes must be 0, 1 or 2");
+        return fail(KNC_ERR_INVALID_ARGUMENT, "stage_passes must be 0, 1, 2 or 3");
     }
     c->zp = zone_params(c->h);
     int ndev = 0;
@@ -1230,7 +1264,7 @@ extern "C++" int knc::insert_block(knc_context* c, int32_t index, int32_t nr, in
     if (nq < 1 || nq > MAX_NQ) return fail(KNC_ERR_INVALID_ARGUMENT, "nq must be in [1, %d]", MAX_NQ);
     if (c->blocks.count(index)) return fail(KNC_ERR_INVALID_ARGUMENT, "block %d already present", index);
     if (int rc = check_cell_volumes(nr, nq, rv, qv)) return rc;
-    if (c->blocks.size() >= 65535) return fail(KNC_ERR_INVALID_ARGUMENT, "too many blocks");
+    if (c->blocks.size() >= (size_t)GHOST_INNER_ORDER) return fail(KNC_ERR_INVALID_ARGUMENT, "too many blocks");
     if (c->blocks.empty()) {
         int rc = upload_theta(c, nq, qv);
         if (rc) return rc;

@@ -114,9 +114,27 @@ struct WorkItem {
     int32_t pad[3];
 };
 
-// Error log of a launch sequence.  key = (step << 48) | (order << 32) |
-// (i << 16) | j; the smallest key is the reported error.
+// Error log of a launch sequence; the smallest key is the reported error.
+// A key orders failures by (sequence, block order, row, column): sequence 0
+// is the time-step pass, 2k - 1 the model ghost rows of the k-th RK stage of
+// the run of steps and 2k that stage (saturating at ERR_SEQ_MAX), so the
+// first stage that fails is the one reported, as the reference returns from
+// the first failing stage.
+// The tag is (block order << 16) | row, block order < 2^15, column < 2^11.
 constexpr int ERR_LOG_CAP = 256;
+constexpr int ERR_COL_BITS = 11;
+constexpr int ERR_TAG_BITS = 31;
+constexpr unsigned ERR_SEQ_MAX = (1u << (64 - ERR_COL_BITS - ERR_TAG_BITS)) - 1u;
+static_assert(MAX_NQ <= (1 << ERR_COL_BITS), "error keys hold the column in ERR_COL_BITS bits");
+__host__ __device__ inline unsigned long long err_key(unsigned seq, int tag, int col)
+{
+    const unsigned long long s = seq < ERR_SEQ_MAX ? seq : ERR_SEQ_MAX;
+    return (s << (ERR_TAG_BITS + ERR_COL_BITS)) | ((unsigned long long)(unsigned)tag << ERR_COL_BITS) |
+           (unsigned long long)(unsigned)col;
+}
+__host__ __device__ inline int err_key_order(unsigned long long key) { return (int)((key >> (ERR_COL_BITS + 16)) & 0x7fff); }
+__host__ __device__ inline int err_key_row(unsigned long long key) { return (int)((key >> ERR_COL_BITS) & 0xffff); }
+__host__ __device__ inline int err_key_col(unsigned long long key) { return (int)(key & ((1u << ERR_COL_BITS) - 1u)); }
 struct ErrEntry {
     unsigned long long key;
     int32_t code;

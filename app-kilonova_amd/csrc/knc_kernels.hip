@@ -492,6 +492,284 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Warp-specialised stage (stage form 3): the same stage as k_stage, with the
+// work of a work item's column split between two kinds of wavefront.
+//   * flux waves (threads [0, NT), one polar column each) march like k_stage
+//     without the c2p: radial PLM slope and radial Riemann flux, the polar
+//     slopes and Riemann fluxes of the row two behind, sources and update;
+//   * c2p waves (threads [NT, 3NT/2)) recover the primitives of the next row,
+//     two columns per lane (col and col + 64 of their 128-column half), so
+//     each lane runs two independent Newton chains.
+// Primitive rows pass from the c2p waves to the flux waves through a ring
+// of four LDS rows, and the polar fluxes through two alternating LDS rows,
+// with one workgroup barrier per row: in step k the c2p waves write row k
+// while the flux waves use row k-1 (radial) and row k-3 (polar and update).
+// Every per-zone operation is k_stage's, in the same order, so the results
+// are the same bits; only the schedule differs.  The radial halo rows of an
+// item and its halo columns are recovered as in k_stage.
+constexpr int WS_RING = 4;
+constexpr int WS_MAX_ROWS = 32;
+
+template <int SYS, int RS, bool STATS, int NT, bool GUESS>
+__global__ void __launch_bounds__(NT + NT / 2) __attribute__((amdgpu_waves_per_eu(3)))
+k_stage_ws(StageArgs a)
+{
+    constexpr int NTH = NT + NT / 2;
+    constexpr int WP = NT + 3;                 // primitive row: 2 left halo, NT own, 1 right halo
+    __shared__ double ldsP[WS_RING][5 * WP];   // primitive rows, row k in slot k % WS_RING
+    __shared__ double ldsF[2][5 * NT];         // polar face fluxes, alternating by step parity
+    __shared__ double ldsH[NHALO * WS_MAX_ROWS * 5];
+    __shared__ double ldsR[WS_MAX_ROWS * 5];   // flux through face j1 per row
+    __shared__ unsigned int lds_hist[HIST_BINS];
+    const int tid = threadIdx.x;
+    // the role is uniform over a wavefront (NT is a multiple of 64); reading it
+    // from the first lane tells the compiler so, and each wave branches to one role
+    const bool flux_role = __builtin_amdgcn_readfirstlane(tid < NT ? 1 : 0) != 0;
+    if (STATS) {
+        for (int b = tid; b < HIST_BINS; b += NTH) lds_hist[b] = 0u;
+        __syncthreads();
+    }
+    const int nitems = a.nitems_ptr ? *a.nitems_ptr : a.nitems;
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double th = a.zp.plm_theta;
+    const double c = a.zp.c;
+    const bool avg = a.avg_slot >= 0;
+    const double bw = a.bweight;
+    const double w1 = -bw + 1.0;
+    const PS zero{{0.0, 0.0, 0.0, 0.0}, 0.0};
+    // c2p lanes: columns cl and cl + 64 of their wave's 128-column half
+    const int ct = tid - NT;
+    const int cl = (ct >> 6) * 128 + (ct & 63);
+
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const WorkItem it = a.items[w];
+        const BlockDesc B = a.blocks[it.block];
+        const int nq = B.nq;
+        const int j0 = it.j0, j1 = it.j1;
+        const int nrow = it.i1 - it.i0;
+        const int64_t np = (int64_t)B.nr * nq;
+        const bool left_halo = j0 > 0;
+        const bool right_halo = j1 < nq;
+
+        __syncthreads();   // the previous item's last reads of the LDS rows are done
+        if (left_halo || right_halo) {
+            // Halo primitives of this item's rows (every thread), then the right-face fluxes.
+            for (int z0 = 0; z0 < NHALO * nrow; z0 += NTH) {
+                const int z = z0 + tid;
+                const int zz = z < NHALO * nrow ? z : 0;
+                const int h = zz % NHALO, rr = zz / NHALO;
+                const int col = h < 2 ? j0 - 2 + h : j1 - 4 + h;
+                const bool need = z < NHALO * nrow && (h < 2 ? left_halo : right_halo) && col >= 0 && col < nq;
+                const PS q0 = column_primitive<SYS, STATS, GUESS, false>(a, B, it.i0 + rr, need ? col : j1 - 1,
+                                                                         lds_hist);
+                if (z < NHALO * nrow) {
+                    const PS q = need ? q0 : zero;
+                    double* d = ldsH + (h * WS_MAX_ROWS + rr) * 5;
+                    d[0] = q.P.rho; d[1] = q.P.ur; d[2] = q.P.uq; d[3] = q.P.p; d[4] = q.s;
+                }
+            }
+            __syncthreads();
+            if (right_halo) {
+                for (int rr = tid; rr < nrow; rr += NTH) {
+                    PS q[4];
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const double* d = ldsH + ((2 + h) * WS_MAX_ROWS + rr) * 5;
+                        q[h] = PS{{d[0], d[1], d[2], d[3]}, d[4]};
+                    }
+                    PS gl = zero, gr = zero;
+                    if (j1 - 1 >= 1) gl = plm5(th, q[0], q[1], q[2]);
+                    if (j1 + 1 < nq) gr = plm5(th, q[1], q[2], q[3]);
+                    Cons G;
+                    double Gs;
+                    riemann<SYS, RS, 1>(add_half(q[1], gl), sub_half(q[2], gr), a.zp, &G, &Gs);
+                    const Flux5 f = scaled(G, Gs, c, gld(zg_aq(B), (int64_t)(it.i0 + rr) * (nq + 1) + j1));
+                    double* d = ldsR + rr * 5;
+                    d[0] = f.d; d[1] = f.sr; d[2] = f.sq; d[3] = f.tau; d[4] = f.s;
+                }
+            }
+            // the march's first barrier publishes ldsR
+        }
+
+        if (flux_role) {
+            // ---- flux waves -------------------------------------------------
+            const double* area_r = zg_ar(B);
+            const double* area_q = zg_aq(B);
+            const double* Uin = slot_of(B, a.in_slot);
+            double* Uout = slot_of(B, a.out_slot);
+            const double* U0 = slot_of(B, avg ? a.avg_slot : a.in_slot);
+            const bool act = j0 + tid < j1;
+            const int j = act ? j0 + tid : j1 - 1;
+            PS Pm2, Pm1, gm2;
+            Flux5 Fm, Fk;
+            Raw5 ui, u0;
+            for (int k = it.i0 - 2; k <= it.i1 + 2; ++k) {
+                const int kk = k - 1;   // the row whose primitives the last barrier published
+                const int r = kk - 2;   // the zone row updated in this step
+                const bool live = kk >= it.i0 - 2;
+                const bool update = kk >= it.i0 + 2;
+                PS Pk = zero;
+                if (live) {
+                    // Halo columns of row kk-1 (polar two steps from now) into its ring row.
+                    const int rh = kk - 1 - it.i0;
+                    if (rh >= 0 && rh < nrow) {
+                        double* L = ldsP[(kk - 1) & (WS_RING - 1)];
+                        if (tid < 2 && left_halo) {
+                            const double* d = ldsH + (tid * WS_MAX_ROWS + rh) * 5;
+                            lds_put(L, WP, tid, PS{{d[0], d[1], d[2], d[3]}, d[4]});
+                        }
+                        if (tid == NT - 1 && right_halo) {
+                            const double* d = ldsH + (4 * WS_MAX_ROWS + rh) * 5;
+                            lds_put(L, WP, NT + 2, PS{{d[0], d[1], d[2], d[3]}, d[4]});
+                        }
+                    }
+                    if (update) {
+                        // Zone row r: polar slopes and fluxes (scheme.rs:107-128).
+                        const double aqr = gld(area_q, (int64_t)r * (nq + 1) + j);
+                        const double* L = ldsP[r & (WS_RING - 1)];
+                        const int lp = tid + 2;
+                        const PS& P = Pm2;
+                        PS gy = zero;
+                        if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(L, WP, lp - 1), P, lds_get(L, WP, lp + 1));
+                        Flux5 Fy{0.0, 0.0, 0.0, 0.0, 0.0};
+                        if (act && j >= 1) {
+                            const PS Pl = lds_get(L, WP, lp - 1);
+                            PS gl = zero;
+                            if (j >= 2) gl = plm5(th, lds_get(L, WP, lp - 2), Pl, P);
+                            Cons G;
+                            double Gs;
+                            riemann<SYS, RS, 1>(add_half(Pl, gl), sub_half(P, gy), a.zp, &G, &Gs);
+                            Fy = scaled(G, Gs, c, aqr);
+                        }
+                        double* F = ldsF[k & 1];
+                        F[tid] = Fy.d;
+                        F[NT + tid] = Fy.sr;
+                        F[2 * NT + tid] = Fy.sq;
+                        F[3 * NT + tid] = Fy.tau;
+                        F[4 * NT + tid] = Fy.s;
+                    }
+                    Pk = lds_get(ldsP[kk & (WS_RING - 1)], WP, tid + 2);
+                    if (kk >= it.i0) {
+                        const PS g = plm5(th, Pm2, Pm1, Pk);                 // gx[kk-1]
+                        if (kk >= it.i0 + 1) {
+                            // Radial face kk-1, between rows kk-2 and kk-1 (scheme.rs:81-97).
+                            const double ar = gld(area_r, (int64_t)(kk - 1) * nq + j);
+                            Cons F;
+                            double Fs;
+                            riemann<SYS, RS, 0>(add_half(Pm2, gm2), sub_half(Pm1, g), a.zp, &F, &Fs);
+                            Fk = scaled(F, Fs, c, ar);
+                        }
+                        gm2 = g;
+                    }
+                    if (update) {
+                        // the update operands, in flight across the barrier
+                        const int64_t o = (int64_t)r * nq + j;
+#pragma unroll
+                        for (int f = 0; f < 5; ++f) ui.v[f] = gld(Uin, o + f * np);
+                        if (avg) {
+#pragma unroll
+                            for (int f = 0; f < 5; ++f) u0.v[f] = gld(U0, o + f * np);
+                        }
+                    }
+                }
+                __syncthreads();   // step k: ring row k (c2p waves) and the polar fluxes of row r
+                if (live) {
+                    if (update) {
+                        const int rr = r - it.i0;
+                        const double* F = ldsF[k & 1];
+                        const Flux5 Fy{F[tid], F[NT + tid], F[2 * NT + tid], F[3 * NT + tid], F[4 * NT + tid]};
+                        Flux5 Fn{0.0, 0.0, 0.0, 0.0, 0.0};
+                        if (j + 1 < nq) {
+                            if (j + 1 < j1) {
+                                Fn = Flux5{F[tid + 1], F[NT + tid + 1], F[2 * NT + tid + 1], F[3 * NT + tid + 1],
+                                           F[4 * NT + tid + 1]};
+                            } else {
+                                const double* d = ldsR + rr * 5;
+                                Fn = Flux5{d[0], d[1], d[2], d[3], d[4]};
+                            }
+                        }
+                        if (act) {
+                            // Sources x dV and the update (scheme.rs:130-158).
+                            const PS& P = Pm2;
+                            const double dv = gld(rg_volA(B), r) * gld(a.th.dcost, j);
+                            double ssr, ssq;
+                            sources<SYS>(P.P, gld(rg_rinv(B), r), gld(a.th.cot, j), a.zp, &ssr, &ssq);
+                            const double sc0 = 0.0 * c * dv;
+                            const double sc1 = ssr * c * dv;
+                            const double sc2 = ssq * c * dv;
+                            const double sc3 = 0.0 * c * dv;
+                            const double du0 = (sc0 - (Fk.d - Fm.d) - (Fn.d - Fy.d)) * dt;
+                            const double du1 = (sc1 - (Fk.sr - Fm.sr) - (Fn.sr - Fy.sr)) * dt;
+                            const double du2 = (sc2 - (Fk.sq - Fm.sq) - (Fn.sq - Fy.sq)) * dt;
+                            const double du3 = (sc3 - (Fk.tau - Fm.tau) - (Fn.tau - Fy.tau)) * dt;
+                            const double ds = ((Fk.s - Fm.s) + (Fn.s - Fy.s)) * -dt;
+                            double v0 = ui.v[0] + du0;
+                            double v1 = ui.v[1] + du1;
+                            double v2 = ui.v[2] + du2;
+                            double v3 = ui.v[3] + du3;
+                            double v4 = ui.v[4] + ds;
+                            if (avg) {
+                                v0 = v0 * w1 + u0.v[0] * bw;
+                                v1 = v1 * w1 + u0.v[1] * bw;
+                                v2 = v2 * w1 + u0.v[2] * bw;
+                                v3 = v3 * w1 + u0.v[3] * bw;
+                                v4 = v4 * w1 + u0.v[4] * bw;
+                            }
+                            double* o = Uout + (int64_t)r * nq + j;
+                            gst(o, 0, v0);
+                            gst(o, np, v1);
+                            gst(o, 2 * np, v2);
+                            gst(o, 3 * np, v3);
+                            gst(o, 4 * np, v4);
+                        }
+                    }
+                    Fm = Fk;
+                    Pm2 = Pm1;
+                    Pm1 = Pk;
+                }
+            }
+        } else {
+            // ---- c2p waves: rows i0-2 .. i1+1, two columns per lane --------
+            const int c0 = j0 + cl, c1 = c0 + 64;
+            const bool act0 = c0 < j1, act1 = c1 < j1;
+            const int jc0 = act0 ? c0 : j1 - 1, jc1 = act1 ? c1 : j1 - 1;
+            RowSrc src = resolve_row<false>(a, B, it.i0 - 2);
+            Raw5 cur0 = load_raw<GUESS, false>(src, jc0), cur1 = load_raw<GUESS, false>(src, jc1);
+            for (int k = it.i0 - 2; k <= it.i1 + 2; ++k) {
+                if (k <= it.i1 + 1) {
+                    RowSrc nsrc = src;
+                    Raw5 nxt0, nxt1;
+                    if (k + 1 <= it.i1 + 1) {
+                        nsrc = resolve_row<false>(a, B, k + 1);
+                        nxt0 = load_raw<GUESS, false>(nsrc, jc0);
+                        nxt1 = load_raw<GUESS, false>(nsrc, jc1);
+                    }
+                    const PS P0 = raw_primitive<SYS, STATS, GUESS>(a, src, cur0, jc0, lds_hist);
+                    const PS P1 = raw_primitive<SYS, STATS, GUESS>(a, src, cur1, jc1, lds_hist);
+                    if (GUESS && a.write_guess && k >= it.i0 && k < it.i1) {
+                        float* gp = guess_plane(B, a.guess_out) + (int64_t)k * nq;
+                        if (act0) gstf(gp, jc0, guess_log2(P0.P.p, cur0.v[0] * cur0.rdv));
+                        if (act1) gstf(gp, jc1, guess_log2(P1.P.p, cur1.v[0] * cur1.rdv));
+                    }
+                    double* L = ldsP[k & (WS_RING - 1)];
+                    if (act0) lds_put(L, WP, cl + 2, P0);
+                    if (act1) lds_put(L, WP, cl + 66, P1);
+                    cur0 = nxt0;
+                    cur1 = nxt1;
+                    src = nsrc;
+                }
+                __syncthreads();   // step k, as the flux waves
+            }
+        }
+    }
+    if (STATS) {
+        __syncthreads();
+        for (int b = tid; b < HIST_BINS; b += NTH)
+            if (lds_hist[b]) atomicAdd(&a.stats->hist[b], (unsigned long long)lds_hist[b]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // c2p pass of the two-pass stage: the primitives and scalar concentration of
 // every zone of the stage's input into the blocks' primitive planes, each
 // zone recovered once.  One (work item, row) unit per workgroup iteration,
@@ -870,17 +1148,48 @@ static int occupancy_p(int threads)
     return e == hipSuccess ? n : 0;
 }
 
+// The warp-specialised stage by polar columns per work item (NT flux
+// threads plus NT/2 c2p threads per workgroup).
 template <int SYS, int RS, bool ST, bool G>
-static hipError_t launch_stage_v(const StageArgs& a, int grid, int threads, bool prims, hipStream_t s)
+static hipError_t launch_stage_ws(const StageArgs& a, int grid, int threads, hipStream_t s)
 {
-    return prims ? launch_stage_p<SYS, RS, false, false, true>(a, grid, threads, s)
-                 : launch_stage_p<SYS, RS, ST, G, false>(a, grid, threads, s);
+    switch (threads) {
+    case 128: hipLaunchKernelGGL((k_stage_ws<SYS, RS, ST, 128, G>), dim3(grid), dim3(192), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_stage_ws<SYS, RS, ST, 256, G>), dim3(grid), dim3(384), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 template <int SYS, int RS, bool ST, bool G>
-static int occupancy_v(int threads, bool prims)
+static int occupancy_ws(int threads)
 {
-    return prims ? occupancy_p<SYS, RS, false, false, true>(threads) : occupancy_p<SYS, RS, ST, G, false>(threads);
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (threads) {
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_ws<SYS, RS, ST, 128, G>, 192, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_ws<SYS, RS, ST, 256, G>, 384, 0); break;
+    default: break;
+    }
+    return e == hipSuccess ? n : 0;
+}
+
+// Stage forms: 1 fused, 2 the flux pass of the two-pass stage, 3 the
+// warp-specialised fused stage.
+template <int SYS, int RS, bool ST, bool G>
+static hipError_t launch_stage_v(const StageArgs& a, int grid, int threads, int form, hipStream_t s)
+{
+    if (form == STAGE_TWO_PASS) return launch_stage_p<SYS, RS, false, false, true>(a, grid, threads, s);
+    if (form == STAGE_SPECIALISED) return launch_stage_ws<SYS, RS, ST, G>(a, grid, threads, s);
+    return launch_stage_p<SYS, RS, ST, G, false>(a, grid, threads, s);
+}
+
+template <int SYS, int RS, bool ST, bool G>
+static int occupancy_v(int threads, int form)
+{
+    if (form == STAGE_TWO_PASS) return occupancy_p<SYS, RS, false, false, true>(threads);
+    if (form == STAGE_SPECIALISED) return occupancy_ws<SYS, RS, ST, G>(threads);
+    return occupancy_p<SYS, RS, ST, G, false>(threads);
 }
 
 // c2p pass variants by workgroup size.
@@ -898,12 +1207,15 @@ static hipError_t launch_prim_v(const StageArgs& a, int grid, int threads, hipSt
     return hipGetLastError();
 }
 
-bool stage_variant_supported(int threads)
+bool stage_variant_supported(int threads, int form)
 {
+    if (form == STAGE_SPECIALISED) return threads == 128 || threads == 256;
     return threads == 64 || threads == 128 || threads == 256 || threads == 512;
 }
 
-int stage_max_rows() { return MAX_ROWS; }
+int stage_max_rows(int form) { return form == STAGE_SPECIALISED ? WS_MAX_ROWS : MAX_ROWS; }
+
+int stage_block_threads(int threads, int form) { return form == STAGE_SPECIALISED ? threads + threads / 2 : threads; }
 
 // Dispatch on (system, solver, stats, learned); the learned guess exists for
 // SRHD only (the Euler c2p is closed form).
@@ -922,15 +1234,15 @@ int stage_max_rows() { return MAX_ROWS; }
                           : (stats ? FN<SYS_EULER, RS_HLLE, true, false>(__VA_ARGS__)                      \
                                    : FN<SYS_EULER, RS_HLLE, false, false>(__VA_ARGS__))))
 
-int stage_occupancy(int system, int rs, bool stats, bool learned, bool prims, int threads)
+int stage_occupancy(int system, int rs, bool stats, bool learned, int form, int threads)
 {
-    return KNC_DISPATCH(occupancy_v, threads, prims);
+    return KNC_DISPATCH(occupancy_v, threads, form);
 }
 
-hipError_t launch_stage(int system, int rs, bool stats, bool learned, bool prims, const StageArgs& a, int grid,
+hipError_t launch_stage(int system, int rs, bool stats, bool learned, int form, const StageArgs& a, int grid,
                         int threads, hipStream_t s)
 {
-    return KNC_DISPATCH(launch_stage_v, a, grid, threads, prims, s);
+    return KNC_DISPATCH(launch_stage_v, a, grid, threads, form, s);
 }
 
 hipError_t launch_prim(int system, bool stats, bool learned, const StageArgs& a, int grid, int threads,

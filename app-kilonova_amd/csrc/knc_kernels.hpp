@@ -29,7 +29,7 @@ struct StageArgs {
     BoundaryDesc bnd;
     ErrLog* err;
     C2PStats* stats;
-    int32_t step;                // error-key step number (0 = dt pass)
+    int32_t step;                // error-key sequence (0 = dt pass, 2k = k-th RK stage of the run)
     int32_t rows;                // rows per work item (row-parallel passes: k_prim, k_guess, k_dt)
     MlpDev mlp;                  // learned c2p guess (learned-guess kernels only)
     int32_t write_guess;         // guess kernels also store each owned zone's recovered log2(p/D)
@@ -37,14 +37,15 @@ struct StageArgs {
     int32_t guess_out;           // guess plane write_guess stores into (never guess_in)
 };
 
-// One failed zone into the error log.  tag = (block order << 16) | row, or
+// One failed zone into the error log.  seq = 0 for the time-step pass, 2k
+// for the k-th RK stage of the run of steps and 2k - 1 for the model ghost
+// rows that stage reads; tag = (block order << 16) | row, or
 // (GHOST_INNER_ORDER / GHOST_OUTER_ORDER << 16) | row for the model ghost
-// rows; the key orders failures by (step, block, row, column) and the host
-// maps the block order back to the block index.
-__device__ inline void record_error(ErrLog* log, int step, int tag, int col, int code, double value, const Cons& u)
+// rows of that stage.  The key orders failures by (seq, block, row, column)
+// and the host maps the block order back to the block index.
+__device__ inline void record_error(ErrLog* log, int seq, int tag, int col, int code, double value, const Cons& u)
 {
-    const unsigned long long key = ((unsigned long long)step << 48) | ((unsigned long long)(unsigned)tag << 16) |
-                                   (unsigned long long)col;
+    const unsigned long long key = err_key((unsigned)seq, tag, col);
     atomicMin(&log->min_key, key);
     const unsigned int slot = atomicAdd(&log->count, 1u);
     if (slot < (unsigned int)ERR_LOG_CAP) {
@@ -66,16 +67,21 @@ struct GhostRows {
     double volA[2];
 };
 
-// One stage: fused (c2p inside the stage kernel) or, with prims, the flux
-// pass of the two-pass stage, which reads the primitive planes k_prim wrote.
-hipError_t launch_stage(int system, int rs, bool stats, bool learned, bool prims, const StageArgs& a, int grid,
+// One stage in one of its forms (knc_options::stage_passes): fused (c2p
+// inside the stage kernel), the flux pass of the two-pass stage (which reads
+// the primitive planes k_prim wrote), or the warp-specialised fused stage.
+enum { STAGE_FUSED = 1, STAGE_TWO_PASS = 2, STAGE_SPECIALISED = 3 };
+hipError_t launch_stage(int system, int rs, bool stats, bool learned, int form, const StageArgs& a, int grid,
                         int threads, hipStream_t s);
 // The c2p pass of the two-pass stage (grid x over work items, y over rows).
 hipError_t launch_prim(int system, bool stats, bool learned, const StageArgs& a, int grid, int threads,
                        hipStream_t s);
-bool stage_variant_supported(int threads);
-int stage_occupancy(int system, int rs, bool stats, bool learned, bool prims, int threads);
-int stage_max_rows();
+// Polar columns per work item (`threads`) a form supports, its largest
+// radial rows per item, and the workgroup size it launches with.
+bool stage_variant_supported(int threads, int form);
+int stage_occupancy(int system, int rs, bool stats, bool learned, int form, int threads);
+int stage_max_rows(int form);
+int stage_block_threads(int threads, int form);
 // The learned c2p guess of every zone of a.in_slot into the blocks' guess
 // planes.  Row-parallel passes (k_guess, k_prim, k_dt) take one (work item,
 // row) unit per workgroup iteration, a.rows rows per item, with a 1-D grid
@@ -109,7 +115,7 @@ hipError_t launch_block_from_model(int system, const ModelDev& m, const ZonePara
                                    const double* rc, const double* volA, const double* qc, const double* dcost,
                                    double t, double* planes, int* fail, hipStream_t s);
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
-                             const double* qc, const double* dcost, double t, int step,
+                             const double* qc, const double* dcost, double t, int seq,
                              int tag0, double* out, ErrLog* err, hipStream_t s);
 hipError_t launch_fma_probe(int chains, int iters, int grid, double* out, hipStream_t s);
 hipError_t launch_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,

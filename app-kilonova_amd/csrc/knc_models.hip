@@ -64,7 +64,7 @@ __global__ void k_block_from_model(ModelDev m, ZoneParams zp, int nr, int nq, co
 // stage failure, under the ghost block's tag.
 template <int SYS>
 __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, const double* qc, const double* dcost,
-                             double t, int step, int tag0, double* out, ErrLog* err)
+                             double t, int seq, int tag0, double* out, ErrLog* err)
 {
     const int n2 = 2 * nq;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
@@ -103,7 +103,7 @@ __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, con
         }
         double s = 0.0;
         if (code != E_OK) {
-            record_error(err, step, tag0 | i, j, code, value, u);
+            record_error(err, seq, tag0 | i, j, code, value, u);
             Q = Prim{0.0, 0.0, 0.0, 0.0};
         } else {
             s = um / dv;
@@ -148,15 +148,15 @@ hipError_t launch_block_from_model(int system, const ModelDev& m, const ZonePara
 }
 
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
-                             const double* qc, const double* dcost, double t, int step,
+                             const double* qc, const double* dcost, double t, int seq,
                              int tag0, double* out, ErrLog* err, hipStream_t s)
 {
     const dim3 g(grid_of(2 * (int64_t)nq, 256)), b(256);
     if (system == SYS_SRHD)
-        hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, step, tag0, out,
+        hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq, tag0, out,
                            err);
     else
-        hipLaunchKernelGGL(k_model_rows<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, step, tag0,
+        hipLaunchKernelGGL(k_model_rows<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq, tag0,
                            out, err);
     return hipGetLastError();
 }

@@ -181,8 +181,8 @@ extern "C" int knc_block_time_step_noalloc(const knc_hydro* hydro, int32_t index
     knc_error e{};
     e.code = KNC_HYDRO_ROOT_FINDER_FAILED;
     e.block = index;
-    e.i = (int)((key >> 16) & 0xffff);
-    e.j = (int)(key & 0xffff);
+    e.i = err_key_row(key);
+    e.j = err_key_col(key);
     if (e.i < nr && e.j < nq) {
         e.r = std::sqrt(radial_vertices[e.i] * radial_vertices[e.i + 1]);
         e.theta = 0.5 * (polar_vertices[e.j] + polar_vertices[e.j + 1]);

@@ -393,6 +393,8 @@ class Context:
     def _raise(self, rc, err):
         if rc in ERROR_KINDS:
             raise HydroError(rc, err)
+        if rc == ERR_MODEL and err.code == ERR_MODEL:
+            raise ModelError(err=err)
         _check(rc)
 
     def time_step(self):
@@ -674,7 +676,20 @@ def model_eval(kind, params, r, theta, time, table=None, ctx=None):
 
 
 class ModelError(RuntimeError):
-    """An initial model evaluated where the reference panics."""
+    """An initial model evaluated where the reference panics; raised by a
+    run whose model ghost rows fail, with the zone of the mesh block they
+    stand for (block, i, j) and its (r, theta) centre."""
+
+    def __init__(self, message="", err=None):
+        self.code = ERR_MODEL
+        self.block = self.i = self.j = None
+        self.r = self.theta = None
+        if err is not None:
+            self.block, self.i, self.j = err.block, err.i, err.j
+            self.r, self.theta = err.r, err.theta
+            message = (f"ModelPanics at position (r, theta) = ({err.r:.4e}, {err.theta:.4f}) "
+                       f"block {err.block} zone ({err.i}, {err.j})")
+        super().__init__(message)
 
 
 def plm_gradient(plm_theta, a, b, c):

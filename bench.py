@@ -71,7 +71,8 @@ def run_gpu(setup, launch_mode, steps, warmup, fold, net=None, tune=False, guess
         net.attach(ctx)
     ctx.tuned = ctx.autotune() if tune else None
     if ctx.tuned:
-        log(f"bench: autotuned shape {ctx.tuned['stage_threads']} columns x {ctx.tuned['rows_per_item']} rows "
+        log(f"bench: autotuned shape {ctx.tuned['stage_threads']} columns x {ctx.tuned['rows_per_item']} rows, "
+            f"stage form {ctx.tuned['stage_passes']} of {ctx.tuned['candidates']} candidates "
             f"({ctx.tuned['ms_per_stage']:.4f} ms vs default {ctx.tuned['default_ms_per_stage']:.4f} ms)")
     ctx.set_timing(True)
     done = 0
@@ -286,6 +287,8 @@ def main():
     ap.add_argument("--nq", type=int, default=1024)
     ap.add_argument("--threads", type=int, default=0, help="stage workgroup size (0: library choice)")
     ap.add_argument("--rows", type=int, default=0, help="radial rows per work item (0: library choice)")
+    ap.add_argument("--passes", type=int, default=0,
+                    help="stage form: 1 fused, 2 two-pass, 3 warp-specialised (0: library choice)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-previous", action="store_true", help="skip the previous-pressure guess path")
     ap.add_argument("--no-stats", action="store_true")
@@ -309,8 +312,8 @@ def main():
     log(f"bench: bomb_jet {args.nr}x{args.nq} zones={zones} blocks={len(setup.blocks)} "
         f"steps={args.steps} warmup={args.warmup} fold={args.fold}")
 
-    shape = dict(stage_threads=args.threads, rows_per_item=args.rows)
-    tune = not args.no_tune and not (args.threads or args.rows)
+    shape = dict(stage_threads=args.threads, rows_per_item=args.rows, stage_passes=args.passes)
+    tune = not args.no_tune and not (args.threads or args.rows or args.passes)
     ctx_b, wall_b, stage_ms_b = run_gpu(setup, knc.BATCHED, args.steps, args.warmup, args.fold, tune=tune, **shape)
     rate_b = zones * args.steps / wall_b
     log(f"bench: batched {rate_b / 1e9:.3f} Gzu/s, stage {stage_ms_b:.4f} ms")

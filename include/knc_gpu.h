@@ -112,7 +112,10 @@ typedef struct knc_options {
     int32_t stage_passes;    /* 1: fused stage (c2p inside the stage kernel);
                                 2: a c2p pass writing primitive planes, then the
                                 fused reconstruction -> flux -> update pass;
-                                0 = 1 (knc_autotune may choose either) */
+                                3: fused stage with the c2p and the flux work on
+                                separate wavefronts of each workgroup (128 or 256
+                                polar columns per work item, at most 32 rows);
+                                0 = 1 (knc_autotune may choose any) */
 } knc_options;
 
 /* HydroError (src/physics/mod.rs:109-117): code, the failing zone and its

@@ -197,6 +197,23 @@ def test_halo_kilonova_setup_matches_oracle():
     _run_pair(p, 1, 5, False, one_step=1e-11)
 
 
+def test_model_ghost_row_failure_is_reported_at_the_oracles_zone():
+    """halo_kilonova with the outer radius at 9e19: the blocks lie inside
+    the disk's domain but the outer ghost rows (mesh block 8, r = 1.11e20)
+    reach below the midplane near theta = pi, where the model panics in the
+    first stage's boundary refresh; reported at the oracle's zone."""
+    p = presets.parse(presets.apply_overrides(SETUP_HALO, ["mesh.num_polar_zones=16", "mesh.outer_radius=9e19"]))
+    ctx = presets.context_from_preset(p, launch_mode=knc.BATCHED)
+    sim = oracle_from_preset(p)
+    with pytest.raises(capi.ModelError) as e:
+        ctx.advance(1)
+    with pytest.raises(oracle.OracleError) as eo:
+        sim.scheme_advance(1)
+    assert eo.value.code == capi.ERR_MODEL
+    assert (e.value.block, e.value.i, e.value.j) == (eo.value.block, eo.value.i, eo.value.j) == (8, 0, 14)
+    assert e.value.r == eo.value.r and e.value.theta == eo.value.theta
+
+
 def test_wind_shock_setup_with_table_matches_oracle(tmp_path):
     """setups/wind_shock.yaml with an initial data table (an ASCII file, read
     like LookupTable::from_ascii_file)."""

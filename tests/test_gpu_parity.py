@@ -227,6 +227,40 @@ def test_negative_density_is_reported_with_position():
     assert e.value.r == eo.value.r and e.value.theta == eo.value.theta
 
 
+@pytest.mark.parametrize("mode,passes", [("per_block", 1), ("per_block", 2), ("batched", 1), ("batched", 2),
+                                         ("batched", 3)])
+def test_first_failing_stage_is_reported(mode, passes):
+    """dt = 1e-12 is beyond the CFL limit next to the inner jet (c = 3e10):
+    the second RK stage fails at block 0, but the planted zone fails in the
+    first stage, which the reference returns from (scheme.rs:162-166)."""
+    s = knc.setups.bomb_jet(nr=256, nq=128, nblocks=2, nshell=4)
+    s.blocks[1].conserved[17, 40, 0] = -1.0
+    launch = knc.BATCHED if mode == "batched" else knc.PER_BLOCK
+    ctx = knc.context_from_setup(s, launch_mode=launch, stage_passes=passes)
+    with pytest.raises(knc.HydroError) as e:
+        ctx.advance_steps(1e-12, 1)
+    sim = oracle_from_setup(s)
+    with pytest.raises(oracle.OracleError) as eo:
+        sim.advance(1e-12, 1)
+    got = (e.value.code, e.value.block, e.value.i, e.value.j, e.value.value, e.value.r, e.value.theta)
+    want = (1, eo.value.block, eo.value.i, eo.value.j, eo.value.value, eo.value.r, eo.value.theta)
+    assert got == want and want[1:4] == (1, 17, 40)
+
+
+def test_later_stage_failure_is_reported_at_its_zone():
+    """Without the planted zone the same step fails in its second stage, at
+    the zone the oracle names."""
+    s = knc.setups.bomb_jet(nr=256, nq=128, nblocks=2, nshell=4)
+    ctx = knc.context_from_setup(s, launch_mode=knc.BATCHED)
+    with pytest.raises(knc.HydroError) as e:
+        ctx.advance_steps(1e-12, 1)
+    sim = oracle_from_setup(s)
+    with pytest.raises(oracle.OracleError) as eo:
+        sim.advance(1e-12, 1)
+    assert (e.value.code, e.value.block, e.value.i, e.value.j, e.value.value) == \
+        (eo.value.code, eo.value.block, eo.value.i, eo.value.j, eo.value.value)
+
+
 @pytest.mark.parametrize("nq,threads,passes", [(256, 64, 1), (100, 64, 1), (65, 64, 1), (130, 128, 1),
                                                 (100, 64, 2), (130, 128, 2)])
 def test_polar_tiles_match_untiled_and_oracle(nq, threads, passes):
