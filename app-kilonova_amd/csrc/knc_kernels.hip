@@ -60,7 +60,12 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
                                     double um, double dv, double rdv, int tag, int col, unsigned int* lds_hist,
                                     double guess)
 {
-    const Cons u{div_rcp(ud, dv, rdv), div_rcp(usr, dv, rdv), div_rcp(usq, dv, rdv), div_rcp(utau, dv, rdv)};
+    // U/dV on the three-operation path, one branch for lanes outside its range
+    bool slow = false;
+    Cons u{div_rcp_fast(ud, dv, rdv, &slow), div_rcp_fast(usr, dv, rdv, &slow), div_rcp_fast(usq, dv, rdv, &slow),
+           div_rcp_fast(utau, dv, rdv, &slow)};
+    if (__builtin_expect(slow, 0))
+        u = Cons{div_rcp(ud, dv, rdv), div_rcp(usr, dv, rdv), div_rcp(usq, dv, rdv), div_rcp(utau, dv, rdv)};
     PS out;
     int code;
     double value = u.d;
@@ -112,8 +117,14 @@ __device__ inline PS zone_primitive(const StageArgs& a, double ud, double usr, d
         return out;
     }
     // lorentz_factor = lorentz_factor_squared().sqrt() (relativistic_hydro.rs:162-164), 1 for Euler
-    const double m = div_rcp(um, dv, rdv);
-    out.s = SYS == SYS_EULER ? m : m / sqrt_rn_ge1(1.0 + out.P.ur * out.P.ur + out.P.uq * out.P.uq);
+    bool slow_s = false;
+    double m = div_rcp_fast(um, dv, rdv, &slow_s);
+    double W = SYS == SYS_EULER ? 1.0 : sqrt_rn_ge1_fast(1.0 + out.P.ur * out.P.ur + out.P.uq * out.P.uq, &slow_s);
+    if (__builtin_expect(slow_s, 0)) {
+        m = div_rcp(um, dv, rdv);
+        if (SYS != SYS_EULER) W = sqrt_rn_ge1(1.0 + out.P.ur * out.P.ur + out.P.uq * out.P.uq);
+    }
+    out.s = SYS == SYS_EULER ? m : m / W;
     return out;
 }
 

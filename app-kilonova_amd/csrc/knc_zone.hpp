@@ -73,10 +73,13 @@ __device__ inline double sqrt_rn_core(double x)
     return fma(d, h, g);
 }
 
+// The range in which sqrt_rn_core is the correctly rounded root.
+__device__ inline bool sqrt_rn_core_exact(double x) { return x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023; }
+
 __device__ inline double sqrt_rn(double x)
 {
     double g = sqrt_rn_core(x);
-    if (__builtin_expect(!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023), 0)) g = __builtin_sqrt(x);
+    if (__builtin_expect(!sqrt_rn_core_exact(x), 0)) g = __builtin_sqrt(x);
     return g;
 }
 
@@ -113,6 +116,31 @@ __host__ __device__ inline double div_rcp(double x, double d, double y)
 constexpr double DIV_RCP_DMIN = 0x1p-400;
 constexpr double DIV_RCP_DMAX = 0x1p400;
 
+// Unguarded forms for a group of operations that takes one branch: each
+// returns the fast path's value and sets *slow where that lane needs the
+// guarded form (sqrt_rn, sqrt_rn_ge1, div_rcp), which the caller then
+// recomputes for the whole group.  Rare lanes only; no lane's result changes.
+__device__ inline double sqrt_rn_fast(double x, bool* slow)
+{
+    *slow = *slow || !sqrt_rn_core_exact(x);
+    return sqrt_rn_core(x);
+}
+
+__device__ inline double sqrt_rn_ge1_fast(double x, bool* slow)
+{
+    *slow = *slow || !(x <= 0x1.fffffffffffffp+1023);
+    return sqrt_rn_core(x);
+}
+
+__device__ inline double div_rcp_fast(double x, double d, double y, bool* slow)
+{
+    const double q = x * y;
+    const double t = fma(q, d, -x);
+    const double r = fma(-t, y, q);
+    *slow = *slow || (!(fabs(r) >= 0x1p-500 && fabs(r) <= 0x1p500) && x != 0.0);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // c2p, SRHD: Newton-Raphson on p (Marti & Mueller, Living Rev. Relativ. 6
 // (2003) 7, sec. 2.3).  Q = tau + D + p, den = Q^2 - S^2:
@@ -137,8 +165,13 @@ __device__ inline double c2p_srhd_newton_step(double D, double S2, double E, con
     p = ldexp(rint(ldexp(p, -e)), e);
     const double Q = E + p;
     const double den = Q * Q - S2;
-    if (!(den > 0.0)) return p;
-    const double sq = sqrt_rn(den);
+    double sq;
+    if (__builtin_expect(!sqrt_rn_core_exact(den), 0)) {
+        if (!(den > 0.0)) return p;
+        sq = __builtin_sqrt(den);
+    } else {
+        sq = sqrt_rn_core(den);
+    }
     const double pQ = p * Q;
     const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
     return p - F * den / (zp.gamma * p * S2 - den * Q);
@@ -163,15 +196,21 @@ __device__ inline int c2p_srhd_newton(double D, double S2, double tau, const Zon
     for (; n <= C2P_MAX_ITER; ++n) {
         const double Q = E + p;
         const double den = Q * Q - S2;
-        if (!(den > 0.0)) break;
-        const double sq = sqrt_rn(den);
+        // one branch for the exits and the square root's slow lanes
+        double sq;
+        if (__builtin_expect(!sqrt_rn_core_exact(den), 0)) {
+            if (!(den > 0.0)) break;
+            sq = __builtin_sqrt(den);
+        } else {
+            sq = sqrt_rn_core(den);
+        }
         const double pQ = p * Q;
         const double F = zp.gm1 * (den - D * sq - pQ) - pQ;
         const double dp = F * den / (zp.gamma * p * S2 - den * Q);
         p = p - dp;
-        if (!(fabs(p) < 1e300)) break;
-        if (fabs(dp) <= C2P_TOL * (E + p)) {
-            code = E_OK;
+        const bool finite = fabs(p) < 1e300;
+        if (!finite || fabs(dp) <= C2P_TOL * (E + p)) {
+            if (finite) code = E_OK;
             break;
         }
     }
@@ -256,8 +295,10 @@ struct FluxState {
     double nm, np, den;
 };
 
-template <int SYS, int RS, int AXIS>
-__device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams& zp)
+// FAST: the square roots take their fast path and *slow records the lanes
+// that need the guarded form (the caller recomputes those with FAST false).
+template <int SYS, int RS, int AXIS, bool FAST = false>
+__device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams& zp, bool* slow = nullptr)
 {
     FluxState st;
     const double un = AXIS == 0 ? P.ur : P.uq;
@@ -266,7 +307,7 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
         st.u = p2c<SYS>(P, zp);
         st.vn = un;
         st.us = s;
-        const double cs = sqrt_rn(zp.gamma * P.p / P.rho);
+        const double cs = FAST ? sqrt_rn_fast(zp.gamma * P.p / P.rho, slow) : sqrt_rn(zp.gamma * P.p / P.rho);
         st.lm = st.vn - cs;
         st.lp = st.vn + cs;
         st.nm = st.lm;
@@ -281,7 +322,7 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
         return st;
     }
     const double uu = P.ur * P.ur + P.uq * P.uq;
-    const double W = sqrt_rn_ge1(1.0 + uu);
+    const double W = FAST ? sqrt_rn_ge1_fast(1.0 + uu, slow) : sqrt_rn_ge1(1.0 + uu);
     st.u = p2c_srhd(P, W, zp);
     const double rhoh = P.rho + P.p * zp.gfac;
     const double ut = AXIS == 0 ? P.uq : P.ur;
@@ -293,7 +334,7 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
     const double gp = zp.gamma * P.p;
     const double e = rhoh - gp;
     const double Y = rhoh + uu * e;
-    const double k0 = sqrt_rn(gp * (rhoh + (ut * ut) * e));
+    const double k0 = FAST ? sqrt_rn_fast(gp * (rhoh + (ut * ut) * e), slow) : sqrt_rn(gp * (rhoh + (ut * ut) * e));
     const double b = (un * W) * e;
     st.nm = b - k0;
     st.np = b + k0;
@@ -326,8 +367,15 @@ template <int AXIS> __device__ inline double ctang(const Cons& u) { return AXIS 
 template <int SYS, int RS, int AXIS>
 __device__ inline void riemann(const PS& Lq, const PS& Rq, const ZoneParams& zp, Cons* F, double* Fs)
 {
-    const FluxState L = flux_state<SYS, RS, AXIS>(Lq.P, Lq.s, zp);
-    const FluxState R = flux_state<SYS, RS, AXIS>(Rq.P, Rq.s, zp);
+    // both states' square roots on their fast path, one branch for the rare
+    // lanes that need the guarded forms
+    bool slow = false;
+    FluxState L = flux_state<SYS, RS, AXIS, true>(Lq.P, Lq.s, zp, &slow);
+    FluxState R = flux_state<SYS, RS, AXIS, true>(Rq.P, Rq.s, zp, &slow);
+    if (__builtin_expect(slow, 0)) {
+        L = flux_state<SYS, RS, AXIS>(Lq.P, Lq.s, zp);
+        R = flux_state<SYS, RS, AXIS>(Rq.P, Rq.s, zp);
+    }
 
     if (RS == RS_HLLE) {
         // a0 = min(lm_L, lm_R, 0) = n0/d0 and a1 = max(lp_L, lp_R, 0) = n1/d1,
