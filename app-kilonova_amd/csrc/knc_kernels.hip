@@ -50,6 +50,16 @@ __device__ inline void gstf(float* p, int64_t k, float v)
     ((__attribute__((address_space(1))) float*)p)[k] = v;
 }
 
+// The value in vector registers: the compiler treats it as per-lane from
+// here on.  Uniform doubles that the stage reads in every row then occupy
+// VGPRs, of which it has spare, instead of scalar registers, of which it
+// has none.
+__device__ inline double in_vgpr(double x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // c2p of one zone (BlockState::try_to_primitive of U/dV, src/state.rs:76-94,
 // and the scalar concentration s = M/dV/Gamma of src/scheme.rs:35, each
 // quotient correctly rounded as the reference divides).  u is the
@@ -287,11 +297,17 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
         __syncthreads();
     }
     const int nitems = a.nitems_ptr ? *a.nitems_ptr : a.nitems;
-    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    // uniform doubles read in every row: vector registers (see in_vgpr)
+    a.zp.gamma = in_vgpr(a.zp.gamma);
+    a.zp.gm1 = in_vgpr(a.zp.gm1);
+    a.zp.gfac = in_vgpr(a.zp.gfac);
+    a.zp.plm_theta = in_vgpr(a.zp.plm_theta);
+    a.zp.c = in_vgpr(a.zp.c);
+    const double dt = in_vgpr(a.dt_dev ? *a.dt_dev : a.dt);
+    const double bw = in_vgpr(a.bweight);
     const double th = a.zp.plm_theta;
     const double c = a.zp.c;
     const bool avg = a.avg_slot >= 0;
-    const double bw = a.bweight;
     const double w1 = -bw + 1.0;
     const PS zero{{0.0, 0.0, 0.0, 0.0}, 0.0};
 

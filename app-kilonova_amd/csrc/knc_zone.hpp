@@ -360,6 +360,73 @@ __device__ inline FluxState flux_state(const Prim& P, double s, const ZoneParams
 template <int AXIS> __device__ inline double cnorm(const Cons& u) { return AXIS == 0 ? u.sr : u.sq; }
 template <int AXIS> __device__ inline double ctang(const Cons& u) { return AXIS == 0 ? u.sq : u.sr; }
 
+// The two square roots of an SRHD face state and the outer speeds they give
+// (flux_state's W, rho h and nm/den, np/den): all the HLLE flux needs beyond
+// the primitive itself.
+struct SrhdSpeeds {
+    double W, rhoh, nm, np, den;
+};
+
+template <int AXIS, bool FAST>
+__device__ inline SrhdSpeeds srhd_speeds(const Prim& P, const ZoneParams& zp, bool* slow = nullptr)
+{
+    SrhdSpeeds v;
+    const double un = AXIS == 0 ? P.ur : P.uq;
+    const double ut = AXIS == 0 ? P.uq : P.ur;
+    const double uu = P.ur * P.ur + P.uq * P.uq;
+    v.W = FAST ? sqrt_rn_ge1_fast(1.0 + uu, slow) : sqrt_rn_ge1(1.0 + uu);
+    v.rhoh = P.rho + P.p * zp.gfac;
+    const double gp = zp.gamma * P.p;
+    const double e = v.rhoh - gp;
+    const double k0 = FAST ? sqrt_rn_fast(gp * (v.rhoh + (ut * ut) * e), slow) : sqrt_rn(gp * (v.rhoh + (ut * ut) * e));
+    const double b = (un * v.W) * e;
+    v.nm = b - k0;
+    v.np = b + k0;
+    v.den = v.rhoh + uu * e;
+    return v;
+}
+
+// SRHD HLLE flux from the primitives and their speeds, each conserved and
+// flux component formed where it is consumed (the expressions and their
+// order are flux_state's and the HLLE branch of riemann's, so the bits are
+// the same); only the speeds stay live across the whole face.
+template <int AXIS>
+__device__ inline void riemann_hlle_srhd(const PS& Lq, const PS& Rq, const ZoneParams& zp, Cons* F, double* Fs)
+{
+    bool slow = false;
+    SrhdSpeeds L = srhd_speeds<AXIS, true>(Lq.P, zp, &slow);
+    SrhdSpeeds R = srhd_speeds<AXIS, true>(Rq.P, zp, &slow);
+    if (__builtin_expect(slow, 0)) {
+        L = srhd_speeds<AXIS, false>(Lq.P, zp);
+        R = srhd_speeds<AXIS, false>(Rq.P, zp);
+    }
+    const bool left_m = L.nm * R.den <= R.nm * L.den;
+    const double nm = left_m ? L.nm : R.nm, dm = left_m ? L.den : R.den;
+    const bool left_p = L.np * R.den >= R.np * L.den;
+    const double np = left_p ? L.np : R.np, dp = left_p ? L.den : R.den;
+    const double n0 = nm < 0.0 ? nm : 0.0, d0 = nm < 0.0 ? dm : 1.0;
+    const double n1 = np > 0.0 ? np : 0.0, d1 = np > 0.0 ? dp : 1.0;
+    const double c1 = n1 * d0, c0 = n0 * d1, ab = n1 * n0;
+    const double rd = 1.0 / (c1 - c0);
+    const Prim& a = Lq.P;
+    const Prim& b = Rq.P;
+    const double una = AXIS == 0 ? a.ur : a.uq, unb = AXIS == 0 ? b.ur : b.uq;
+    // p2c_srhd: d = rho W, s = (rho h W) u, tau = ((rho h W) W - p) - d
+    const double da = a.rho * L.W, db = b.rho * R.W;
+    F->d = ((a.rho * una) * c1 - (b.rho * unb) * c0 + (db - da) * ab) * rd;
+    const double hwa = L.rhoh * L.W, hwb = R.rhoh * R.W;
+    double fa = (L.rhoh * a.ur) * una, fb = (R.rhoh * b.ur) * unb;
+    if (AXIS == 0) { fa = fa + a.p; fb = fb + b.p; }
+    F->sr = (fa * c1 - fb * c0 + (hwb * b.ur - hwa * a.ur) * ab) * rd;
+    fa = (L.rhoh * a.uq) * una;
+    fb = (R.rhoh * b.uq) * unb;
+    if (AXIS == 1) { fa = fa + a.p; fb = fb + b.p; }
+    F->sq = (fa * c1 - fb * c0 + (hwb * b.uq - hwa * a.uq) * ab) * rd;
+    F->tau = (((L.rhoh * L.W - a.rho) * una) * c1 - ((R.rhoh * R.W - b.rho) * unb) * c0 +
+              (((hwb * R.W - b.p) - db) - ((hwa * L.W - a.p) - da)) * ab) * rd;
+    *Fs = ((Lq.s * una) * c1 - (Rq.s * unb) * c0 + (Rq.s * R.W - Lq.s * L.W) * ab) * rd;
+}
+
 // HLLE / HLLC Godunov flux with passive scalar (intercell_flux,
 // relativistic_hydro.rs:123-134, newtonian_hydro.rs:96-102), unscaled.
 // HLLE: Harten-Lax-van Leer-Einfeldt.  HLLC SRHD: Mignone & Bodo, MNRAS 364
@@ -367,6 +434,10 @@ template <int AXIS> __device__ inline double ctang(const Cons& u) { return AXIS 
 template <int SYS, int RS, int AXIS>
 __device__ inline void riemann(const PS& Lq, const PS& Rq, const ZoneParams& zp, Cons* F, double* Fs)
 {
+    if (SYS == SYS_SRHD && RS == RS_HLLE) {
+        riemann_hlle_srhd<AXIS>(Lq, Rq, zp, F, Fs);
+        return;
+    }
     // both states' square roots on their fast path, one branch for the rare
     // lanes that need the guarded forms
     bool slow = false;
