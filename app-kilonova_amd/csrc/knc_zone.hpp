@@ -565,7 +565,11 @@ __device__ inline double plm(double th, double a, double b, double c)
     const double z = (c - b) * th;
     const double lo = fmin(x, fmin(y, z));
     const double hi = fmax(x, fmax(y, z));
-    return lo > 0.0 ? lo : (hi < 0.0 ? hi : 0.0);
+    // lo > 0 ? lo : (hi < 0 ? hi : 0) as the median of lo, 0 and hi
+    // (lo <= hi), max(lo, min(hi, 0)), plus +0 so that a zero comes out +0
+    // as the select returns it: one min, one max and one add instead of two
+    // compares and four 32-bit selects, the same bits for every input.
+    return fmax(lo, fmin(hi, 0.0)) + 0.0;
 }
 
 __device__ inline PS plm5(double th, const PS& a, const PS& b, const PS& c)

@@ -157,6 +157,19 @@ def _run_pair(setup, nsteps, fold, **options):
     return ctx, sim
 
 
+def test_plm_gradient_bits_match_oracle_at_zeros_and_edges():
+    """The slope limiter bit for bit, zero signs included: every triple of
+    signed zeros, tiny and subnormal values, equal neighbours and
+    extrema, against the oracle's select form."""
+    vals = np.array([0.0, -0.0, 1.0, -1.0, 5e-324, -5e-324, 2.2e-308, -2.2e-308, 1e-300, -1e-300, 3.0, 1e300,
+                     -1e300, 0.5])
+    a, b, c = (x.ravel() for x in np.meshgrid(vals, vals, vals, indexing="ij"))
+    for theta in (1.0, 1.5, 2.0):
+        g = knc.plm_gradient(theta, a, b, c)
+        go = oracle.plm_gradient(theta, a, b, c)
+        assert np.array_equal(g.view(np.uint64), go.view(np.uint64)), theta
+
+
 def test_euler_blast_one_step():
     s = knc.setups.euler_blast()
     ctx, sim = _run_pair(s, 1, 1)
