@@ -434,6 +434,9 @@ int rebuild_tables(knc_context* c)
     const int per_cu = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
                                                    guess_read(c), stage_form(c), c->threads));
     c->persistent_grid = std::min(c->nitems, c->num_cu * per_cu);
+    const int per_cu_tp = std::max(1, stage_occupancy(c->h.system, c->h.riemann_solver, c->o.collect_stats != 0,
+                                                      guess_read(c), STAGE_TWO_PASS, c->threads));
+    c->persistent_grid_two_pass = std::min(c->nitems, c->num_cu * per_cu_tp);
     c->dirty = false;
     return KNC_OK;
 }
@@ -492,11 +495,16 @@ int row_pass_grid(const knc_context* c, int nitems)
 // Launches one stage over every block: per-block launches or one batched
 // one.  The passes before the stage kernels (the learned-guess pass, the c2p
 // pass of the two-pass stage) cover every block first, since a block's stage
-// also reads its neighbours' edge rows.
-int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment_start)
+// also reads its neighbours' edge rows.  With with_dt (the first stage of a
+// batched run whose dt is not yet computed) the stage runs in the two-pass
+// form and its c2p pass is also the CFL reduction of the run (launch_prim
+// with the partial minima, then k_dt_finish), in place of a separate dt pass
+// over the same state; its failures are logged as the time step's
+// (sequence 0), where State::time_step meets them first.
+int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment_start, bool with_dt = false)
 {
     const bool stats = c->o.collect_stats != 0;
-    const bool prims = two_pass(c);
+    const bool prims = two_pass(c) || with_dt;
     const bool lg = guess_read(c);
     // KNC_GUESS_LEARNED_STEP evaluates the network on the first stage's input
     // only; the later stages of the step start Newton from the same guesses.
@@ -513,7 +521,13 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment
             HIP_TRY(launch_guess(b, grid, c->threads, c->stream));
             c->last_launches += 1;
         }
-        if (prims) {
+        if (with_dt) {
+            StageArgs d = b;
+            d.step = 0;
+            HIP_TRY(launch_prim(c->h.system, stats, lg, d, grid, c->threads, c->stream, c->d_dt_partial));
+            HIP_TRY(launch_dt_finish(c->d_dt_partial, grid, c->h.cfl_number, c->d_dt, c->stream));
+            c->last_launches += 2;
+        } else if (prims) {
             HIP_TRY(launch_prim(c->h.system, stats, lg, b, grid, c->threads, c->stream));
             c->last_launches += 1;
         }
@@ -524,11 +538,12 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment
         a.items = c->d_items;
         a.nitems_ptr = c->d_nitems;
         if ((rc = pre_pass(a, c->nitems))) return rc;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, stage_form(c), a, c->persistent_grid,
-                             c->threads, c->stream));
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, with_dt ? (int)STAGE_TWO_PASS : stage_form(c),
+                             a, with_dt ? c->persistent_grid_two_pass : c->persistent_grid, c->threads, c->stream));
         c->last_launches += 1;
         return KNC_OK;
     }
+    if (with_dt) return fail(KNC_ERR_UNSUPPORTED, "the dt reduction rides on batched stages only");
     for (auto& bi : c->block_items) {
         a.items = c->d_items + bi.first;
         a.nitems_ptr = nullptr;
@@ -601,19 +616,21 @@ int enqueue_rk_step(knc_context* c, int step, int fold_step, bool segment_start)
     const int f1 = (c->cur + 1) % NSLOT, f2 = (c->cur + 2) % NSLOT;
     StageArgs a = base_args(c);
     int rc;
+    // the run's CFL reduction rides on its first stage's c2p pass
+    const bool dt_here = step == 1 && c->dt_in_first_stage && !c->trial;
     switch (c->h.runge_kutta_order) {
     case 1:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
         a.step = stage_seq(c, step, 0);
-        if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
+        if ((rc = launch_one_stage(c, a, true, segment_start, dt_here))) return rc;
         c->cur = f1;
         break;
     case 2:
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
         a.step = stage_seq(c, step, 0);
-        if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
+        if ((rc = launch_one_stage(c, a, true, segment_start, dt_here))) return rc;
         stage_slots(c, a, f1, f2, c->cur, 0.5);
         stage_boundaries(c, a, fold_step, 1);
         a.step = stage_seq(c, step, 1);
@@ -624,7 +641,7 @@ int enqueue_rk_step(knc_context* c, int step, int fold_step, bool segment_start)
         stage_slots(c, a, c->cur, f1, -1, 0.0);
         stage_boundaries(c, a, fold_step, 0);
         a.step = stage_seq(c, step, 0);
-        if ((rc = launch_one_stage(c, a, true, segment_start))) return rc;
+        if ((rc = launch_one_stage(c, a, true, segment_start, dt_here))) return rc;
         stage_slots(c, a, f1, f2, c->cur, 0.75);
         stage_boundaries(c, a, fold_step, 1);
         a.step = stage_seq(c, step, 1);
@@ -763,7 +780,7 @@ int enqueue_steps(knc_context* c, int n, int first)
     }
     // the ghost-row pointers of model boundaries depend on the fold step
     const bool model = any_model_side(c);
-    auto key = std::make_tuple(start_slot, n, model ? first : 0);
+    auto key = std::make_tuple(start_slot, n, model ? first : 0, (int)c->dt_in_first_stage);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         hipGraph_t g;
@@ -951,13 +968,22 @@ int run_steps(knc_context* c, bool with_dt, double dt_fixed, int fold, double* d
     if (model) c->stage_times.assign((size_t)fold * c->h.runge_kutta_order, 0.0);
     hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(64), 0, c->stream, c->d_err);
     HIP_TRY(hipGetLastError());
+    const bool host_dt = with_dt && (moving || model);
+    // Batched, with dt wanted only on the device and a CFL reduction to do:
+    // the first stage's c2p pass computes dt (launch_one_stage), and the
+    // state is read once for both instead of twice.
+    c->dt_in_first_stage = with_dt && !host_dt && c->o.launch_mode == KNC_LAUNCH_BATCHED &&
+                           !(c->h.system == KNC_SRHD && !c->h.adaptive_time_step);
+    struct ClearOnExit {
+        bool& flag;
+        ~ClearOnExit() { flag = false; }
+    } clear_dt_flag{c->dt_in_first_stage};
     if (with_dt) {
-        if ((rc = enqueue_time_step(c))) return rc;
+        if (!c->dt_in_first_stage && (rc = enqueue_time_step(c))) return rc;
     } else {
         hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, c->stream, c->d_dt, dt_fixed);
         HIP_TRY(hipGetLastError());
     }
-    const bool host_dt = with_dt && (moving || model);
     double dt = dt_fixed;
     if (host_dt) {
         HIP_TRY(hipStreamSynchronize(c->stream));

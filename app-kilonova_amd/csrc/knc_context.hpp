@@ -88,6 +88,8 @@ struct knc_context {
     std::vector<std::pair<int, int>> block_items;  // (offset, count) per table entry
     int threads = 64;
     int persistent_grid = 0;
+    int persistent_grid_two_pass = 0;     // the two-pass form's flux kernel (first stage with the dt reduction)
+    bool dt_in_first_stage = false;       // the next run's first stage computes dt (its c2p pass reduces the CFL minimum)
 
     // per-call device state
     knc::ErrLog* d_err = nullptr;
@@ -95,8 +97,9 @@ struct knc_context {
     double* d_dt_partial = nullptr;       // per-workgroup minima of the CFL pass
     double* d_dt = nullptr;
 
-    // graphs keyed by (starting slot, steps, first fold step for model boundaries)
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+    // graphs keyed by (starting slot, steps, first fold step for model
+    // boundaries, whether the run's first stage carries the CFL reduction)
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
 
     // learned c2p guess: device copy of the network's operand fragments
     void* d_mlp = nullptr;

@@ -796,22 +796,38 @@ k_stage_ws(StageArgs a)
     }
 }
 
+// Minimum over the wavefront by cross-lane exchanges.
+__device__ inline double wave_min(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = dmin(v, o);
+    }
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // c2p pass of the two-pass stage: the primitives and scalar concentration of
 // every zone of the stage's input into the blocks' primitive planes, each
 // zone recovered once.  One (work item, row) unit per workgroup iteration,
 // NT columns at a time.  With GUESS, Newton starts from the guess plane
-// written by k_guess.
+// written by k_guess.  With DT the pass is also the CFL reduction of the
+// run's first stage (State::time_step, src/state.rs:143-162, on the same
+// state): each zone's dl / (max signal speed x c), as k_dt forms it, is
+// folded into a workgroup minimum in partial[blockIdx.x] for k_dt_finish,
+// so the run needs no separate dt pass over the state.
 // ---------------------------------------------------------------------------
-template <int SYS, bool STATS, bool GUESS, int NT>
-__global__ void __launch_bounds__(NT) k_prim(StageArgs a)
+template <int SYS, bool STATS, bool GUESS, int NT, bool DT>
+__global__ void __launch_bounds__(NT) k_prim(StageArgs a, double* partial)
 {
     __shared__ unsigned int lds_hist[HIST_BINS];
+    __shared__ double wmin[NT / 64];
     const int tid = threadIdx.x;
     if (STATS) {
         for (int b = tid; b < HIST_BINS; b += NT) lds_hist[b] = 0u;
         __syncthreads();
     }
+    double m = DBL_MAX;
     const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const WorkItem it = a.items[u / a.rows];
@@ -841,7 +857,21 @@ __global__ void __launch_bounds__(NT) k_prim(StageArgs a)
                 gst(P, 4 * np + j, q.s);
                 if (GUESS && a.write_guess)
                     gstf(guess_plane(B, a.guess_out), (int64_t)i * nq + j, guess_log2(q.P.p, u0 * rdv));
+                if (DT) {
+                    const double dl = dmin(gld(rg_dr(B), i), gld(a.th.dq, j) * gld(rg_rv(B), i));
+                    m = dmin(m, dl / (max_signal_speed<SYS>(q.P, a.zp) * a.zp.c));
+                }
             }
+        }
+    }
+    if (DT) {
+        m = wave_min(m);
+        if ((tid & 63) == 0) wmin[tid >> 6] = m;
+        __syncthreads();
+        if (tid == 0) {
+            double r = wmin[0];
+            for (int k = 1; k < NT / 64; ++k) r = dmin(r, wmin[k]);
+            partial[blockIdx.x] = r;
         }
     }
     if (STATS) {
@@ -948,14 +978,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1
 // ---------------------------------------------------------------------------
 // CFL reduction.
 // ---------------------------------------------------------------------------
-__device__ inline double wave_min(double v)
-{
-    for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(v, off, 64);
-        v = dmin(v, o);
-    }
-    return v;
-}
 
 template <int SYS>
 __global__ void __launch_bounds__(256) k_dt(StageArgs a, double* partial)
@@ -1221,14 +1243,24 @@ static int occupancy_v(int threads, int form)
 
 // c2p pass variants by workgroup size.
 template <int SYS, int RS, bool ST, bool G>
-static hipError_t launch_prim_v(const StageArgs& a, int grid, int threads, hipStream_t s)
+static hipError_t launch_prim_v(const StageArgs& a, double* partial, int grid, int threads, hipStream_t s)
 {
     const dim3 g(grid);
+    if (partial) {
+        switch (threads) {
+        case 64: hipLaunchKernelGGL((k_prim<SYS, ST, G, 64, true>), g, dim3(64), 0, s, a, partial); break;
+        case 128: hipLaunchKernelGGL((k_prim<SYS, ST, G, 128, true>), g, dim3(128), 0, s, a, partial); break;
+        case 256: hipLaunchKernelGGL((k_prim<SYS, ST, G, 256, true>), g, dim3(256), 0, s, a, partial); break;
+        case 512: hipLaunchKernelGGL((k_prim<SYS, ST, G, 512, true>), g, dim3(512), 0, s, a, partial); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (threads) {
-    case 64: hipLaunchKernelGGL((k_prim<SYS, ST, G, 64>), g, dim3(64), 0, s, a); break;
-    case 128: hipLaunchKernelGGL((k_prim<SYS, ST, G, 128>), g, dim3(128), 0, s, a); break;
-    case 256: hipLaunchKernelGGL((k_prim<SYS, ST, G, 256>), g, dim3(256), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((k_prim<SYS, ST, G, 512>), g, dim3(512), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_prim<SYS, ST, G, 64, false>), g, dim3(64), 0, s, a, nullptr); break;
+    case 128: hipLaunchKernelGGL((k_prim<SYS, ST, G, 128, false>), g, dim3(128), 0, s, a, nullptr); break;
+    case 256: hipLaunchKernelGGL((k_prim<SYS, ST, G, 256, false>), g, dim3(256), 0, s, a, nullptr); break;
+    case 512: hipLaunchKernelGGL((k_prim<SYS, ST, G, 512, false>), g, dim3(512), 0, s, a, nullptr); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1273,10 +1305,11 @@ hipError_t launch_stage(int system, int rs, bool stats, bool learned, int form, 
 }
 
 hipError_t launch_prim(int system, bool stats, bool learned, const StageArgs& a, int grid, int threads,
-                       hipStream_t s)
+                       hipStream_t s, double* dt_partial)
 {
+    if (dt_partial && (grid < 1 || grid > DT_MAX_GRID)) return hipErrorInvalidValue;
     const int rs = RS_HLLE;
-    return KNC_DISPATCH(launch_prim_v, a, grid, threads, s);
+    return KNC_DISPATCH(launch_prim_v, a, dt_partial, grid, threads, s);
 }
 
 hipError_t launch_dt(int system, const StageArgs& a, double* partial, int grid, hipStream_t s)

@@ -73,9 +73,12 @@ struct GhostRows {
 enum { STAGE_FUSED = 1, STAGE_TWO_PASS = 2, STAGE_SPECIALISED = 3 };
 hipError_t launch_stage(int system, int rs, bool stats, bool learned, int form, const StageArgs& a, int grid,
                         int threads, hipStream_t s);
-// The c2p pass of the two-pass stage (grid x over work items, y over rows).
+// The c2p pass of the two-pass stage (one (work item, row) unit per
+// workgroup iteration).  With dt_partial it also writes each workgroup's CFL
+// minimum there, as the dt pass does (grid <= DT_MAX_GRID), for
+// launch_dt_finish.
 hipError_t launch_prim(int system, bool stats, bool learned, const StageArgs& a, int grid, int threads,
-                       hipStream_t s);
+                       hipStream_t s, double* dt_partial = nullptr);
 // Polar columns per work item (`threads`) a form supports, its largest
 // radial rows per item, and the workgroup size it launches with.
 bool stage_variant_supported(int threads, int form);

@@ -225,10 +225,14 @@ def test_bomb_jet_one_step_vs_oracle():
     assert max(err.values()) <= ONE_STEP, err
 
 
-def test_negative_density_is_reported_with_position():
+@pytest.mark.parametrize("mode", [knc.PER_BLOCK, knc.BATCHED])
+def test_negative_density_is_reported_with_position(mode):
+    """A planted zone fails the time step's c2p (State::time_step meets it
+    first): the dt pass per block, or the first stage's c2p pass, which
+    computes dt in the batched runs."""
     s = knc.setups.jet_in_cloud(nr=128, nq=32, nblocks=2, corr=2.0)
     s.blocks[1].conserved[5, 7, 0] = -1.0
-    ctx = knc.context_from_setup(s)
+    ctx = knc.context_from_setup(s, launch_mode=mode)
     with pytest.raises(knc.HydroError) as e:
         ctx.advance(1)
     assert e.value.kind == "NegativeDensity"
@@ -238,6 +242,31 @@ def test_negative_density_is_reported_with_position():
         sim.advance(sim.time_step(), 1)
     assert (eo.value.block, eo.value.i, eo.value.j) == (1, 5, 7)
     assert e.value.r == eo.value.r and e.value.theta == eo.value.theta
+
+
+@pytest.mark.parametrize("name,rk,guess", [("bomb_jet", 2, knc.GUESS_PLAIN), ("bomb_jet", 2, knc.GUESS_PREVIOUS),
+                                            ("bomb_jet", 2, knc.GUESS_LEARNED_FOLD), ("bomb_jet", 3, knc.GUESS_PLAIN),
+                                            ("euler_blast", 1, knc.GUESS_PLAIN), ("euler_blast", 3, knc.GUESS_PLAIN)])
+def test_dt_in_the_first_stage_equals_the_dt_pass(name, rk, guess):
+    """Batched runs compute each fold's dt in the first stage's c2p pass;
+    per-block runs in a separate pass over the state (k_dt).  Same dt at
+    every fold, same state bit for bit, over five folds of 4 steps of the
+    kinetic bomb + jet in 8 uneven blocks and of the Newtonian blast, for
+    the guesses and RK orders listed."""
+    from knc import train_guess
+    s = (knc.setups.bomb_jet(nr=512, nq=128, nblocks=8, nshell=8) if name == "bomb_jet"
+         else knc.setups.euler_blast())
+    s.hydro.runge_kutta_order = rk
+    ctxs = [knc.context_from_setup(s, launch_mode=m, c2p_guess=guess) for m in (knc.PER_BLOCK, knc.BATCHED)]
+    if guess == knc.GUESS_LEARNED_FOLD:
+        net = train_guess.load()
+        for c in ctxs:
+            net.attach(c)
+    for _ in range(5):
+        dts = [c.advance(4) for c in ctxs]
+        assert dts[0] == dts[1]
+    err = rel_l1(gpu_state(ctxs[1]), gpu_state(ctxs[0]))
+    assert max(err.values()) == 0.0, err
 
 
 @pytest.mark.parametrize("mode,passes", [("per_block", 1), ("per_block", 2), ("batched", 1), ("batched", 2),
