@@ -269,6 +269,31 @@ def test_dt_in_the_first_stage_equals_the_dt_pass(name, rk, guess):
     assert max(err.values()) == 0.0, err
 
 
+@pytest.mark.parametrize("guess", [knc.GUESS_LEARNED_FOLD, knc.GUESS_LEARNED_STEP])
+def test_learned_guess_with_dt_in_the_first_stage(guess):
+    """A batched run's first stage with the learned guess: the guess pass,
+    then the c2p pass that also reduces the CFL minimum; per-block runs use
+    the guess pass, the c2p pass and a separate dt pass.  In the two-pass
+    form every zone is recovered once per stage either way, so the Newton
+    histograms agree bin for bin (the same guesses), and dt and state bit for
+    bit."""
+    from knc import train_guess
+    s = knc.setups.bomb_jet(nr=512, nq=128, nblocks=8, nshell=8)
+    net = train_guess.load()
+    ctxs = [knc.context_from_setup(s, launch_mode=m, c2p_guess=guess, collect_stats=True, stage_passes=2)
+            for m in (knc.PER_BLOCK, knc.BATCHED)]
+    for c in ctxs:
+        net.attach(c)
+    for _ in range(3):
+        dts = [c.advance(5) for c in ctxs]
+        assert dts[0] == dts[1]
+    err = rel_l1(gpu_state(ctxs[1]), gpu_state(ctxs[0]))
+    assert max(err.values()) == 0.0, err
+    a, b = (c.c2p_stats() for c in ctxs)
+    assert np.array_equal(a["hist"], b["hist"]) and a["hist"].sum() > 0
+    assert a["fallbacks"] == b["fallbacks"]
+
+
 @pytest.mark.parametrize("mode,passes", [("per_block", 1), ("per_block", 2), ("batched", 1), ("batched", 2),
                                          ("batched", 3)])
 def test_first_failing_stage_is_reported(mode, passes):
