@@ -264,14 +264,25 @@ __device__ inline Flux5 scaled(const Cons& F, double Fs, double c, double A)
 
 // Stage kernel.  NT threads, one polar column each; a work item is rows
 // [i0,i1) x columns [j0,j1) of one block (j1 - j0 <= NT).  When the item does
-// not span every column, its halo columns j0-2, j0-1 and j1-2 .. j1+1 are
-// recovered once per item for all its rows, and the fluxes through its right
-// polar face j1, into LDS before the march; each zone row then costs two
-// workgroup barriers (primitive exchange, polar flux exchange).  The
-// primitive rows alternate between two LDS buffers, so a row's writes need
-// no barrier of their own: the buffer was last read two rows earlier.
-constexpr int MAX_ROWS = 64;
-constexpr int NHALO = 6;   // j0-2, j0-1 | j1-2, j1-1, j1, j1+1
+// not span every column, its halo columns j0-2, j0-1, j0 and j1-2 .. j1+1 are
+// recovered once per item for all its rows, with the fluxes through its
+// right polar face j1 and the left states of its left polar face j0, into
+// LDS before the march.  Step k of the march recovers row k, forms the
+// radial face below row k-1, the polar slopes and face states of row k-1
+// and the polar fluxes and update of row k-2, with one workgroup barrier:
+//   * row k's primitives go to an LDS row for row k's slopes in step k+1;
+//   * row k-1's slopes use its neighbours' primitives published in step
+//     k-1; each column keeps its lower face's right state P - g/2 for step
+//     k+1 and publishes its upper face's left state P + g/2 to an LDS row;
+//   * row k-2's polar fluxes combine the left states published in step k-1
+//     with the right states kept from it, so each zone's slope and face
+//     states are formed once;
+//   * the barrier publishes the fluxes of row k-2 (and what steps k+1 and
+//     k+2 will read); each LDS array alternates between two rows, whose
+//     previous contents were read before the barrier of step k-1.
+constexpr int MAX_ROWS = 32;
+constexpr int NHALO_F = 7;   // j0-2, j0-1, j0 | j1-2, j1-1, j1, j1+1
+constexpr int NHALO = 6;     // the warp-specialised form: j0-2, j0-1 | j1-2, j1-1, j1, j1+1
 
 template <int SYS, bool STATS, bool GUESS, bool PRIMS>
 __device__ inline PS column_primitive(const StageArgs& a, const BlockDesc& B, int row, int col, unsigned int* lds_hist)
@@ -280,16 +291,25 @@ __device__ inline PS column_primitive(const StageArgs& a, const BlockDesc& B, in
     return raw_primitive<SYS, STATS, GUESS>(a, src, load_raw<GUESS && !PRIMS, PRIMS>(src, col), col, lds_hist);
 }
 
+__device__ inline void lds_put5(double* d, const PS& q)
+{
+    d[0] = q.P.rho; d[1] = q.P.ur; d[2] = q.P.uq; d[3] = q.P.p; d[4] = q.s;
+}
+
+__device__ inline PS lds_get5(const double* d) { return PS{{d[0], d[1], d[2], d[3]}, d[4]}; }
+
 // PRIMS: the primitives come from the planes written by k_prim (the two-pass
 // stage); otherwise the stage recovers them itself (fused stage).
 template <int SYS, int RS, bool STATS, int NT, bool GUESS, bool PRIMS>
 __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
 {
-    constexpr int WP = NT + 3;                 // primitive row: 2 left halo, NT own, 1 right halo
+    constexpr int WP = NT + 3;                 // primitive row: j0-2, j0-1 | NT own | j1
     __shared__ double ldsPP[2][5 * WP];      // primitive rows, alternating by row parity
-    __shared__ double ldsF[5 * NT];
-    __shared__ double ldsH[NHALO * MAX_ROWS * 5];
+    __shared__ double ldsPL[2][5 * NT];      // left states of each column's upper polar face, by row parity
+    __shared__ double ldsF[2][5 * NT];       // polar fluxes of each column's lower face, by step parity
+    __shared__ double ldsH[NHALO_F * MAX_ROWS * 5];
     __shared__ double ldsR[MAX_ROWS * 5];      // flux through face j1 per row
+    __shared__ double ldsL[MAX_ROWS * 5];      // left state of face j0 (column j0-1's upper face) per row
     __shared__ unsigned int lds_hist[HIST_BINS];
     const int tid = threadIdx.x;
     if (STATS) {
@@ -329,31 +349,25 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
         const bool right_halo = j1 < nq;
 
         if (left_halo || right_halo) {
-            // Halo primitives of this item's rows, then the right-face fluxes.
-            __syncthreads();
+            // Halo primitives of this item's rows, then the right-face fluxes
+            // and the left-face left states.
+            __syncthreads();   // the previous item's reads of ldsH, ldsR and ldsL are done
             // uniform trip count: every lane reaches raw_primitive together
-            for (int z0 = 0; z0 < NHALO * nrow; z0 += NT) {
+            for (int z0 = 0; z0 < NHALO_F * nrow; z0 += NT) {
                 const int z = z0 + tid;
-                const int zz = z < NHALO * nrow ? z : 0;
-                const int h = zz % NHALO, rr = zz / NHALO;
-                const int col = h < 2 ? j0 - 2 + h : j1 - 4 + h;
-                const bool need = z < NHALO * nrow && (h < 2 ? left_halo : right_halo) && col >= 0 && col < nq;
+                const int zz = z < NHALO_F * nrow ? z : 0;
+                const int h = zz % NHALO_F, rr = zz / NHALO_F;
+                const int col = h < 3 ? j0 - 2 + h : j1 - 5 + h;
+                const bool need = z < NHALO_F * nrow && (h < 3 ? left_halo : right_halo) && col >= 0 && col < nq;
                 const PS q0 = column_primitive<SYS, STATS, GUESS, PRIMS>(a, B, it.i0 + rr, need ? col : j, lds_hist);
-                if (z < NHALO * nrow) {
-                    const PS q = need ? q0 : zero;
-                    double* d = ldsH + (h * MAX_ROWS + rr) * 5;
-                    d[0] = q.P.rho; d[1] = q.P.ur; d[2] = q.P.uq; d[3] = q.P.p; d[4] = q.s;
-                }
+                if (z < NHALO_F * nrow) lds_put5(ldsH + (h * MAX_ROWS + rr) * 5, need ? q0 : zero);
             }
             __syncthreads();
-            if (right_halo) {
-                for (int rr = tid; rr < nrow; rr += NT) {
+            for (int rr = tid; rr < nrow; rr += NT) {
+                if (right_halo) {
                     PS q[4];
 #pragma unroll
-                    for (int h = 0; h < 4; ++h) {
-                        const double* d = ldsH + ((2 + h) * MAX_ROWS + rr) * 5;
-                        q[h] = PS{{d[0], d[1], d[2], d[3]}, d[4]};
-                    }
+                    for (int h = 0; h < 4; ++h) q[h] = lds_get5(ldsH + ((3 + h) * MAX_ROWS + rr) * 5);
                     // face j1 between columns j1-1 (q[1]) and j1 (q[2])
                     PS gl = zero, gr = zero;
                     if (j1 - 1 >= 1) gl = plm5(th, q[0], q[1], q[2]);
@@ -365,11 +379,20 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     double* d = ldsR + rr * 5;
                     d[0] = f.d; d[1] = f.sr; d[2] = f.sq; d[3] = f.tau; d[4] = f.s;
                 }
+                if (left_halo) {
+                    // column j0-1's slope and its upper face's left state
+                    PS q[3];
+#pragma unroll
+                    for (int h = 0; h < 3; ++h) q[h] = lds_get5(ldsH + (h * MAX_ROWS + rr) * 5);
+                    PS g = zero;
+                    if (j0 - 1 >= 1 && j0 < nq) g = plm5(th, q[0], q[1], q[2]);
+                    lds_put5(ldsL + rr * 5, add_half(q[1], g));
+                }
             }
-            // the first row exchange below begins with a barrier
+            // the march's barriers publish ldsR and ldsL before their first reads
         }
 
-        PS Pm2, Pm1, gm2;
+        PS Pm2, Pm1, gm2, PRm;
         Flux5 Fm;
         RowSrc src = resolve_row<PRIMS>(a, B, it.i0 - 2);
         Raw5 cur = load_raw<GUESS && !PRIMS, PRIMS>(src, j), nxt;
@@ -420,47 +443,58 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                 }
                 gm2 = g;
             }
-            if (update) {
-                // Zone row r = k-2: polar slopes and fluxes (scheme.rs:107-128).
-                const int r = k - 2, rr = r - it.i0;
-                const PS& P = Pm2;
-                // Row r's buffer was last read by row r-2, before the flux
-                // barrier of row r-1 that every thread has since passed.
-                double* ldsP = ldsPP[r & 1];
-                lds_put(ldsP, WP, tid + 2, P);
-                if (tid < 2 && left_halo) {
-                    const double* d = ldsH + (tid * MAX_ROWS + rr) * 5;
-                    lds_put(ldsP, WP, tid, PS{{d[0], d[1], d[2], d[3]}, d[4]});
-                }
-                if (tid == NT - 1 && right_halo) {
-                    const double* d = ldsH + (4 * MAX_ROWS + rr) * 5;
-                    lds_put(ldsP, WP, NT + 2, PS{{d[0], d[1], d[2], d[3]}, d[4]});
-                }
-                __syncthreads();
-                const int lp = tid + 2;
+            // Row k-1: polar slope (scheme.rs:107-110, zero at the first and
+            // last columns) and the states of its two polar faces.  Its
+            // neighbours' primitives were published in step k-1.
+            PS PRk = zero;
+            if (k >= it.i0 + 1 && k <= it.i1) {
+                const double* L = ldsPP[(k - 1) & 1];
                 PS gy = zero;
-                if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(ldsP, WP, lp - 1), P, lds_get(ldsP, WP, lp + 1));
-                Flux5 Fy{0.0, 0.0, 0.0, 0.0, 0.0};
+                if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(L, WP, tid + 1), Pm1, lds_get(L, WP, tid + 3));
+                lds_put(ldsPL[(k - 1) & 1], NT, tid, add_half(Pm1, gy));
+                PRk = sub_half(Pm1, gy);
+            }
+            // Row r = k-2: flux through the lower polar face of column j
+            // (scheme.rs:118-128), between column j-1's upper-face left state
+            // (published in step k-1) and this column's lower-face right state.
+            Flux5 Fy{0.0, 0.0, 0.0, 0.0, 0.0};
+            if (update) {
                 if (act && j >= 1) {
-                    const PS Pl = lds_get(ldsP, WP, lp - 1);
-                    PS gl = zero;
-                    if (j >= 2) gl = plm5(th, lds_get(ldsP, WP, lp - 2), Pl, P);
+                    // thread 0 with j >= 1 has a left halo (j0 > 0)
+                    const PS Pl = tid == 0 ? lds_get5(ldsL + (k - 2 - it.i0) * 5)
+                                           : lds_get(ldsPL[(k - 2) & 1], NT, tid - 1);
                     Cons G;
                     double Gs;
-                    riemann<SYS, RS, 1>(add_half(Pl, gl), sub_half(P, gy), a.zp, &G, &Gs);
+                    riemann<SYS, RS, 1>(Pl, PRm, a.zp, &G, &Gs);
                     Fy = scaled(G, Gs, c, aqr);
                 }
-                ldsF[tid] = Fy.d;
-                ldsF[NT + tid] = Fy.sr;
-                ldsF[2 * NT + tid] = Fy.sq;
-                ldsF[3 * NT + tid] = Fy.tau;
-                ldsF[4 * NT + tid] = Fy.s;
-                __syncthreads();
+                double* F = ldsF[k & 1];
+                F[tid] = Fy.d;
+                F[NT + tid] = Fy.sr;
+                F[2 * NT + tid] = Fy.sq;
+                F[3 * NT + tid] = Fy.tau;
+                F[4 * NT + tid] = Fy.s;
+            }
+            // Row k's primitives, for row k's slopes in step k+1, with the
+            // item's halo columns j0-2, j0-1 and j1 of that row.
+            if (k >= it.i0 && k < it.i1) {
+                const int rr = k - it.i0;
+                double* Lp = ldsPP[k & 1];
+                lds_put(Lp, WP, tid + 2, Pk);
+                if (tid < 2 && left_halo) lds_put(Lp, WP, tid, lds_get5(ldsH + (tid * MAX_ROWS + rr) * 5));
+                if (tid == NT - 1 && right_halo)
+                    lds_put(Lp, WP, NT + 2, lds_get5(ldsH + (5 * MAX_ROWS + rr) * 5));
+            }
+            __syncthreads();
+            if (update) {
+                const int r = k - 2, rr = r - it.i0;
+                const PS& P = Pm2;
                 Flux5 Fn{0.0, 0.0, 0.0, 0.0, 0.0};
                 if (j + 1 < nq) {
                     if (j + 1 < j1) {
-                        Fn = Flux5{ldsF[tid + 1], ldsF[NT + tid + 1], ldsF[2 * NT + tid + 1], ldsF[3 * NT + tid + 1],
-                                   ldsF[4 * NT + tid + 1]};
+                        const double* F = ldsF[k & 1];
+                        Fn = Flux5{F[tid + 1], F[NT + tid + 1], F[2 * NT + tid + 1], F[3 * NT + tid + 1],
+                                   F[4 * NT + tid + 1]};
                     } else {
                         const double* d = ldsR + rr * 5;
                         Fn = Flux5{d[0], d[1], d[2], d[3], d[4]};
@@ -503,6 +537,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
             Fm = Fk;
             Pm2 = Pm1;
             Pm1 = Pk;
+            PRm = PRk;
             if (!PRIMS) {
                 rm2 = rm1;
                 rm1 = cur;
