@@ -5,14 +5,18 @@
 // polar column; it marches outward in r.  Each step it recovers the
 // primitives of one new row (c2p), forms the radial PLM slope and the radial
 // Riemann flux of the face behind it (everything radial stays in the
-// thread's registers), and then finishes the zone row two behind: polar PLM
-// and polar Riemann fluxes through LDS, geometric sources, flux divergence,
-// the update U + dt L(U) and, on the last RK stage, the weighted average
-// with U^n (src/state.rs:179-212).  Each zone's c2p runs once per work item
-// plus two halo rows on either side; nothing but U is read from or written to
-// HBM.
+// thread's registers), the polar slope and face states of the row behind,
+// and finishes the zone row two behind: polar Riemann fluxes from face
+// states exchanged through LDS, geometric sources, flux divergence, the
+// update U + dt L(U) and, on the last RK stage, the weighted average with
+// U^n (src/state.rs:179-212), with one workgroup barrier per row.  Each
+// zone's c2p runs once per work item plus two halo rows on either side;
+// nothing but U (and the per-zone geometry) is read from HBM, and only U is
+// written.
 //
-// k_dt: the CFL reduction of State::time_step (src/state.rs:143-162): c2p,
+// k_prim: the c2p pass of the two-pass stage form, which with DT also forms
+// the CFL minima of the run's first stage.  k_dt: the CFL reduction of
+// State::time_step (src/state.rs:143-162) as a pass of its own: c2p,
 // dl / (max_signal_speed * c), wavefront min by cross-lane shuffles, one
 // partial minimum per workgroup, reduced by k_dt_finish.
 #include <hip/hip_runtime.h>
