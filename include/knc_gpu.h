@@ -440,8 +440,11 @@ int knc_autotune(knc_context* ctx, int32_t trials, int32_t force, knc_tune_resul
 int knc_get_c2p_stats(knc_context* ctx, knc_c2p_stats* out);
 int knc_reset_c2p_stats(knc_context* ctx);
 int knc_synchronize(knc_context* ctx);
-/* Average device time of the last advance call's stage launches (ms), from
- * HIP events on the context's stream; 0 if not measured. */
+/* Average device time of the last advance call's stages (ms), from HIP
+ * events on the context's stream around all of its stage launches (the
+ * guess and c2p passes before a stage included, and in batched runs the
+ * CFL reduction that rides on the first stage's c2p pass); 0 if not
+ * measured. */
 int knc_last_stage_time(knc_context* ctx, double* ms_per_stage, int32_t* launches);
 int knc_set_timing(knc_context* ctx, int32_t enable);
 
