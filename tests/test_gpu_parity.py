@@ -329,10 +329,12 @@ def test_later_stage_failure_is_reported_at_its_zone():
 
 
 @pytest.mark.parametrize("nq,threads,passes", [(256, 64, 1), (100, 64, 1), (65, 64, 1), (130, 128, 1),
-                                                (100, 64, 2), (130, 128, 2)])
+                                                (100, 64, 2), (130, 128, 2), (300, 256, 1), (300, 256, 2),
+                                                (700, 256, 1), (700, 512, 1), (1000, 512, 2)])
 def test_polar_tiles_match_untiled_and_oracle(nq, threads, passes):
-    """Work items narrower than the mesh (halo columns recovered per item)
-    give the same bits as one item spanning every column, and as the oracle."""
+    """Work items narrower than the mesh (halo columns recovered per item,
+    a partial last tile) give the same bits as one item spanning every
+    column (or 512-column items), and as the oracle."""
     s = knc.setups.bomb_jet(nr=96, nq=nq, nblocks=3, nshell=4)
     tiled = knc.context_from_setup(s, launch_mode=knc.BATCHED, stage_threads=threads, rows_per_item=7,
                                    stage_passes=passes)
