@@ -305,7 +305,7 @@ __device__ inline PS lds_get5(const double* d) { return PS{{d[0], d[1], d[2], d[
 // PRIMS: the primitives come from the planes written by k_prim (the two-pass
 // stage); otherwise the stage recovers them itself (fused stage).
 template <int SYS, int RS, bool STATS, int NT, bool GUESS, bool PRIMS>
-__global__ void __launch_bounds__(NT) k_stage(StageArgs a)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_stage(StageArgs a)
 {
     constexpr int WP = NT + 3;                 // primitive row: j0-2, j0-1 | NT own | j1
     __shared__ double ldsPP[2][5 * WP];      // primitive rows, alternating by row parity
@@ -396,17 +396,15 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
             // the march's barriers publish ldsR and ldsL before their first reads
         }
 
-        PS Pm2, Pm1, gm2, PRm;
-        Flux5 Fm;
-        RowSrc src = resolve_row<PRIMS>(a, B, it.i0 - 2);
-        Raw5 cur = load_raw<GUESS && !PRIMS, PRIMS>(src, j), nxt;
-        // The fused stage keeps the conserved rows k-1 and k-2 it loaded for
-        // the c2p: row k-2 is also the update operand U of this step.
-        Raw5 rm1, rm2;
-
-        for (int k = it.i0 - 2; k < it.i1 + 2; ++k) {
+        // One step of the march.  The rows and fluxes it carries to the next
+        // step are passed by role (the step's Pm2, Pm1, new Pk, ...), so the
+        // loop below, two steps per iteration, swaps the roles of the values
+        // carried from one step to the next instead of copying them.
+        auto march_step = [&](const int k, const PS& Pm2, const PS& Pm1, PS& Pk, PS& gm2, const Flux5& Fm, Flux5& Fk,
+                              const PS& PRm, PS& PRk, const Raw5& cur, Raw5& nxt, const Raw5& rm2, const RowSrc& src,
+                              RowSrc& nsrc) {
             // Prefetch the next row and this step's update operands.
-            RowSrc nsrc = src;
+            nsrc = src;
             if (k + 1 < it.i1 + 2) {
                 nsrc = resolve_row<PRIMS>(a, B, k + 1);
                 nxt = load_raw<GUESS && !PRIMS, PRIMS>(nsrc, j);
@@ -430,12 +428,12 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     for (int f = 0; f < 5; ++f) u0.v[f] = gld(U0, o + f * np);
                 }
             }
-            const PS Pk = raw_primitive<SYS, STATS, GUESS>(a, src, cur, j, lds_hist);
+            Pk = raw_primitive<SYS, STATS, GUESS>(a, src, cur, j, lds_hist);
             if (GUESS && !PRIMS && a.write_guess && act && k >= it.i0 && k < it.i1) {
                 // this item owns row k: keep its pressure as the next stage's guess
                 gstf(guess_plane(B, a.guess_out), (int64_t)k * nq + j, guess_log2(Pk.P.p, cur.v[0] * cur.rdv));
             }
-            Flux5 Fk;
+            Fk = Flux5{0.0, 0.0, 0.0, 0.0, 0.0};
             if (k >= it.i0) {
                 const PS g = plm5(th, Pm2, Pm1, Pk);                 // gx[k-1]
                 if (k >= it.i0 + 1) {
@@ -450,7 +448,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
             // Row k-1: polar slope (scheme.rs:107-110, zero at the first and
             // last columns) and the states of its two polar faces.  Its
             // neighbours' primitives were published in step k-1.
-            PS PRk = zero;
+            PRk = zero;
             if (k >= it.i0 + 1 && k <= it.i1) {
                 const double* L = ldsPP[(k - 1) & 1];
                 PS gy = zero;
@@ -538,16 +536,49 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a)
                     gst(o, 4 * np, v4);
                 }
             }
-            Fm = Fk;
-            Pm2 = Pm1;
-            Pm1 = Pk;
-            PRm = PRk;
-            if (!PRIMS) {
-                rm2 = rm1;
-                rm1 = cur;
+        };
+        PS X0, X1, X2, gm2, R0, R1;
+        Flux5 F0, F1;
+        RowSrc S0 = resolve_row<PRIMS>(a, B, it.i0 - 2), S1 = S0;
+        // raw rows: the update operand (row k-2), row k-1, the c2p row k and
+        // the prefetched row k+1; the fused stage keeps the rows it loaded
+        // for the c2p, row k-2 being also the update operand U of step k
+        Raw5 RA, RB, C0 = load_raw<GUESS && !PRIMS, PRIMS>(S0, j), C1;
+        int k = it.i0 - 2;
+        if constexpr (RS == RS_HLLC) {
+            // HLLC's star states leave no registers for a second step in
+            // flight: one step per iteration, the carried values copied
+            for (; k < it.i1 + 2; ++k) {
+                march_step(k, X0, X1, X2, gm2, F0, F1, R0, R1, C0, C1, RA, S0, S1);
+                F0 = F1;
+                X0 = X1;
+                X1 = X2;
+                R0 = R1;
+                if (!PRIMS) {
+                    RA = RB;
+                    RB = C0;
+                }
+                C0 = C1;
+                S0 = S1;
             }
-            cur = nxt;
-            src = nsrc;
+        } else {
+            for (; k + 1 < it.i1 + 2; k += 2) {
+                march_step(k, X0, X1, X2, gm2, F0, F1, R0, R1, C0, C1, RA, S0, S1);
+                march_step(k + 1, X1, X2, X0, gm2, F1, F0, R1, R0, C1, RA, RB, S1, S0);
+                // back to the first step's roles: rows k and k+1 as Pm2 and Pm1,
+                // raw rows k, k+1, k+2 as update operand, row k-1 and c2p row
+                X1 = X0;
+                X0 = X2;
+                if (!PRIMS) {
+                    RB = C1;
+                    const Raw5 t = C0;
+                    C0 = RA;
+                    RA = t;
+                } else {
+                    C0 = RA;
+                }
+            }
+            if (k < it.i1 + 2) march_step(k, X0, X1, X2, gm2, F0, F1, R0, R1, C0, C1, RA, S0, S1);
         }
     }
     if (STATS) {
