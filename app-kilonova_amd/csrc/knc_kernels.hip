@@ -397,9 +397,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
         }
 
         // One step of the march.  The rows and fluxes it carries to the next
-        // step are passed by role (the step's Pm2, Pm1, new Pk, ...), so the
-        // loop below, two steps per iteration, swaps the roles of the values
-        // carried from one step to the next instead of copying them.
+        // step are passed by role (the step's Pm2, Pm1, new Pk, ...), so that
+        // the HLLE loop below, two steps per iteration, swaps the roles of
+        // the values carried from one step to the next instead of copying
+        // them.
         auto march_step = [&](const int k, const PS& Pm2, const PS& Pm1, PS& Pk, PS& gm2, const Flux5& Fm, Flux5& Fk,
                               const PS& PRm, PS& PRk, const Raw5& cur, Raw5& nxt, const Raw5& rm2, const RowSrc& src,
                               RowSrc& nsrc) {
