@@ -64,6 +64,28 @@ __device__ inline double in_vgpr(double x)
     return x;
 }
 
+// Issue priority of a stage wave, set once per row of the march.  A CU holds
+// two stage workgroups, and its SIMDs issue for the older of two ready
+// waves: left to that, the workgroup dispatched first runs ahead and the
+// other finishes alone, one wave per SIMD, for the last fifth of the stage.
+// Instead the two take turns: the priority bit alternates on the 100 MHz
+// wall clock every 2^11 ticks (20 us), in opposite phase for the first and
+// second half of the grid (blockIdx and blockIdx + gridDim/2 share a CU when
+// the grid fills two workgroups per CU), and a wave in the first half of its
+// work item ranks above one in the second half.  Priorities order issue
+// only; no result depends on them.
+__device__ inline void march_priority(int done, int total, unsigned int phase)
+{
+    const bool turn = (((unsigned int)(wall_clock64() >> 11)) ^ phase) & 1u;
+    if (2 * done < total) {
+        if (turn) __builtin_amdgcn_s_setprio(3);
+        else __builtin_amdgcn_s_setprio(2);
+    } else {
+        if (turn) __builtin_amdgcn_s_setprio(1);
+        else __builtin_amdgcn_s_setprio(0);
+    }
+}
+
 // c2p of one zone (BlockState::try_to_primitive of U/dV, src/state.rs:76-94,
 // and the scalar concentration s = M/dV/Gamma of src/scheme.rs:35, each
 // quotient correctly rounded as the reference divides).  u is the
@@ -334,6 +356,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
     const bool avg = a.avg_slot >= 0;
     const double w1 = -bw + 1.0;
     const PS zero{{0.0, 0.0, 0.0, 0.0}, 0.0};
+    const unsigned int phase = 2u * blockIdx.x >= gridDim.x ? 1u : 0u;
 
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const WorkItem it = a.items[w];
@@ -404,6 +427,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
         auto march_step = [&](const int k, const PS& Pm2, const PS& Pm1, PS& Pk, PS& gm2, const Flux5& Fm, Flux5& Fk,
                               const PS& PRm, PS& PRk, const Raw5& cur, Raw5& nxt, const Raw5& rm2, const RowSrc& src,
                               RowSrc& nsrc) {
+            march_priority(k - (it.i0 - 2), nrow + 4, phase);
             // Prefetch the next row and this step's update operands.
             nsrc = src;
             if (k + 1 < it.i1 + 2) {
