@@ -73,8 +73,15 @@ __device__ inline double sqrt_rn_core(double x)
     return fma(d, h, g);
 }
 
-// The range in which sqrt_rn_core is the correctly rounded root.
-__device__ inline bool sqrt_rn_core_exact(double x) { return x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023; }
+// The high 32 bits of x.  The range tests of the guards below compare them
+// with 32-bit literals: a compare of x itself with an FP64 constant reads the
+// constant from a pair of scalar registers, which the stage kernel spills.
+__host__ __device__ inline uint32_t hi_word(double x) { return (uint32_t)(__builtin_bit_cast(uint64_t, x) >> 32); }
+
+// The range in which sqrt_rn_core is the correctly rounded root,
+// [2^-767, DBL_MAX]: the high words 0x10000000 .. 0x7FEFFFFF (zeros,
+// subnormals, negatives, infinities and NaN fall outside).
+__device__ inline bool sqrt_rn_core_exact(double x) { return hi_word(x) - 0x10000000u <= 0x6FEFFFFFu; }
 
 __device__ inline double sqrt_rn(double x)
 {
@@ -84,12 +91,15 @@ __device__ inline double sqrt_rn(double x)
 }
 
 // sqrt_rn of a Lorentz factor's square 1 + u^2, which is at least 1 unless
-// it is NaN or +inf: one compare selects the compiler's square root for
-// those two.
+// it is NaN or +inf: one compare (high word at least 0x7FF00000: +inf, NaN,
+// or a negative x, which 1 + u^2 is not) selects the compiler's square root
+// for those two.
+__device__ inline bool sqrt_rn_ge1_outside(double x) { return hi_word(x) >= 0x7FF00000u; }
+
 __device__ inline double sqrt_rn_ge1(double x)
 {
     double g = sqrt_rn_core(x);
-    if (__builtin_expect(!(x <= 0x1.fffffffffffffp+1023), 0)) g = __builtin_sqrt(x);
+    if (__builtin_expect(sqrt_rn_ge1_outside(x), 0)) g = __builtin_sqrt(x);
     return g;
 }
 
@@ -100,15 +110,18 @@ __device__ inline double sqrt_rn_ge1(double x)
 // for d > 0.  Three operations instead of a division when y is shared or
 // tabulated.  The theorem needs every intermediate in the normal range:
 // for d in [2^-400, 2^400] (cell volumes, checked where their tables are
-// built) that holds when |q| is in [2^-500, 2^500], and lanes outside it
-// (tiny scalar tails, zeros excepted, infinities, NaN) divide.  Equal to the
-// host's x / d bit for bit (test_fp64_arithmetic_rounds_like_host).
+// built) that holds when |q| is in [2^-500, 2^500), the high words of |q|
+// 0x20B00000 .. 0x5F2FFFFF, and lanes outside it (tiny scalar tails, zeros
+// excepted, infinities, NaN) divide.  Equal to the host's x / d bit for bit
+// (test_fp64_arithmetic_rounds_like_host).
+__host__ __device__ inline bool div_rcp_outside(double r) { return (hi_word(r) & 0x7FFFFFFFu) - 0x20B00000u >= 0x3E800000u; }
+
 __host__ __device__ inline double div_rcp(double x, double d, double y)
 {
     const double q = x * y;
     const double t = fma(q, d, -x);
     double r = fma(-t, y, q);
-    if (__builtin_expect(!(fabs(r) >= 0x1p-500 && fabs(r) <= 0x1p500) && x != 0.0, 0)) r = x / d;
+    if (__builtin_expect(div_rcp_outside(r) && x != 0.0, 0)) r = x / d;
     return r;
 }
 
@@ -128,7 +141,7 @@ __device__ inline double sqrt_rn_fast(double x, bool* slow)
 
 __device__ inline double sqrt_rn_ge1_fast(double x, bool* slow)
 {
-    *slow = *slow || !(x <= 0x1.fffffffffffffp+1023);
+    *slow = *slow || sqrt_rn_ge1_outside(x);
     return sqrt_rn_core(x);
 }
 
@@ -137,7 +150,7 @@ __device__ inline double div_rcp_fast(double x, double d, double y, bool* slow)
     const double q = x * y;
     const double t = fma(q, d, -x);
     const double r = fma(-t, y, q);
-    *slow = *slow || (!(fabs(r) >= 0x1p-500 && fabs(r) <= 0x1p500) && x != 0.0);
+    *slow = *slow || (div_rcp_outside(r) && x != 0.0);
     return r;
 }
 

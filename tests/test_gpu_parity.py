@@ -64,6 +64,17 @@ def test_fp64_arithmetic_rounds_like_host():
     hard = _near_midpoint_squares(rng, 1000)
     a[n // 2 + 64: n // 2 + 64 + hard.size] = hard
     b = rng.standard_normal(n) * np.exp(rng.uniform(-300, 300, n) / 10)
+    # quotients at and next to the edges of the three-operation division's
+    # range, |a/b| = 2^-500 and 2^500, for divisors with and without a
+    # significand
+    qe = np.array([2.0 ** -500, 2.0 ** 500])
+    qe = np.concatenate([qe, np.nextafter(qe, 0), np.nextafter(qe, np.inf)])
+    qb = np.array([1.0, 3.0, 0.75, 2.0 ** -300, 5.0 * 2.0 ** 299])
+    qa = (qe[:, None] * qb[None, :]).ravel()
+    qb = np.broadcast_to(qb, (qe.size, qb.size)).ravel()
+    k1 = n // 2 + 4096
+    a[k1: k1 + 2 * qa.size] = np.concatenate([qa, -qa])
+    b[k1: k1 + 2 * qa.size] = np.concatenate([qb, qb])
     # significands of the divisor next to 2 and to 1, and of the dividend next to 2
     m, k0 = 1 << 16, 3 * n // 4
     e = rng.integers(-60, 60, (3, m))
