@@ -70,10 +70,12 @@ __device__ inline double in_vgpr(double x)
 // other finishes alone, one wave per SIMD, for the last fifth of the stage.
 // Instead the two take turns: the priority bit alternates on the 100 MHz
 // wall clock every 2^11 ticks (20 us), in opposite phase for the first and
-// second half of the grid (blockIdx and blockIdx + gridDim/2 share a CU when
-// the grid fills two workgroups per CU), and a wave in the first half of its
-// work item ranks above one in the second half.  Priorities order issue
-// only; no result depends on them.
+// second half of the grid (on MI355X the dispatcher places blockIdx and
+// blockIdx + gridDim/2 on one CU when the grid fills two workgroups per CU,
+// as a per-workgroup timing of the stage showed; where it does not, the
+// hardware's age order decides within each half of the work item), and a
+// wave in the first half of its work item ranks above one in the second
+// half.  Priorities order issue only; no result depends on them.
 __device__ inline void march_priority(int done, int total, unsigned int phase)
 {
     const bool turn = (((unsigned int)(wall_clock64() >> 11)) ^ phase) & 1u;
