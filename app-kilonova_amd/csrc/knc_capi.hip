@@ -1908,4 +1908,30 @@ int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c,
     return KNC_OK;
 }
 
+int knc_model_rows_check(knc_context* c, int32_t side, double time, double* rows, double* rows_one_thread)
+{
+    if (!c || !rows || !rows_one_thread) return fail(KNC_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (side != KNC_SIDE_INNER && side != KNC_SIDE_OUTER) return fail(KNC_ERR_INVALID_ARGUMENT, "bad side");
+    int rc;
+    if ((rc = require_blocks(c))) return rc;
+    if (!c->bmodel[side].on) return fail(KNC_ERR_INVALID_ARGUMENT, "no device boundary model on this side");
+    const size_t n = (size_t)10 * c->nq;
+    DevBuf d1, d2;
+    ZONE_ALLOC(d1, n * 8);
+    ZONE_ALLOC(d2, n * 8);
+    const GhostRows g = ghost_rows(c->bmodel[side].mesh, side, c->blocks.begin()->first, c->blocks.rbegin()->first);
+    const int tag0 = (side ? GHOST_OUTER_ORDER : GHOST_INNER_ORDER) << 16;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g, c->d_qc, c->th.dcost, time, 0, tag0,
+                              (double*)d1.p, c->d_err, c->stream));
+    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g, c->d_qc, c->th.dcost, time, 0, tag0,
+                              (double*)d2.p, c->d_err, c->stream, true));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(rows, d1.p, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rows_one_thread, d2.p, n * 8, hipMemcpyDeviceToHost));
+    // zones the model cannot evaluate are left in the rows as zeros; the
+    // check leaves no entries in the context's error log
+    return reset_errors(c);
+}
+
 }  // extern "C"

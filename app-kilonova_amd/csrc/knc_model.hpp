@@ -243,25 +243,49 @@ __device__ inline double pressure_difference_rk4(double r, double z, double dz)
 }
 }  // namespace gal
 
-__device__ inline bool halo_kilonova_at(const double* p, double r, double q, double t, Prim* P, double* X)
+// The disk part of a halo_kilonova zone: the altitude z, the step dz of the
+// pressure integration and the density.
+struct HaloDisk {
+    double z, dz, d;
+};
+
+// At most this many RK4 steps of the pressure integration (1000 up to z,
+// and one more where the accumulated altitude rounds below z).
+constexpr int HALO_MAX_STEPS = 2000;
+
+// halo_kilonova up to the pressure integration: 0 for a zone in the shell
+// (*P and *X set), -1 below the midplane (the reference panics), 1 for the
+// disk (*h set; the pressure is 1e-3 plus the RK4 steps at the altitudes
+// 0, dz, 2 dz, ... accumulated in that order while below z).
+__device__ inline int halo_kilonova_begin(const double* p, double r, double q, double t, Prim* P, double* X,
+                                          HaloDisk* h)
 {
     const double z = r * cos(q) + p[0];
     const Shell s = shell_at(p[1], p[2], p[3], p[4], t);
     if (in_shell(s, r)) {
         *P = shell_primitive(s, p[4], r, 1e-3);
         *X = 1.0;
-        return true;
+        return 0;
     }
-    if (!(z > 0.0)) return false;
-    const double d = gal::thin_disk_density(p[5], z);
+    if (!(z > 0.0)) return -1;
+    h->z = z;
+    h->d = gal::thin_disk_density(p[5], z);
     // vertical_pressure_profile(r, z, z * 1e-3, 1e-3).last()
-    const double dz = z * 1e-3;
+    h->dz = z * 1e-3;
+    return 1;
+}
+
+__device__ inline bool halo_kilonova_at(const double* p, double r, double q, double t, Prim* P, double* X)
+{
+    HaloDisk h;
+    const int c = halo_kilonova_begin(p, r, q, t, P, X, &h);
+    if (c <= 0) return c == 0;
     double zz = 0.0, pr = 1e-3;
-    for (int n = 0; zz < z && n < 2000; ++n) {
-        pr += gal::pressure_difference_rk4(r, zz, dz);
-        zz += dz;
+    for (int n = 0; zz < h.z && n < HALO_MAX_STEPS; ++n) {
+        pr += gal::pressure_difference_rk4(r, zz, h.dz);
+        zz += h.dz;
     }
-    *P = Prim{d, 0.0, 0.0, pr};
+    *P = Prim{h.d, 0.0, 0.0, pr};
     *X = 0.0;
     return true;
 }

@@ -61,60 +61,115 @@ __global__ void k_block_from_model(ModelDev m, ZoneParams zp, int nr, int nq, co
 // to the mesh edge, then try_to_primitive of U/dV and s = M/dV/Gamma, each
 // quotient divided as the reference divides, written as the boundary
 // row set (5 planes x 2nq, radial order).  A c2p failure is logged like a
-// stage failure, under the ghost block's tag.
+// stage failure, under the ghost block's tag.  model_row_zone is one zone
+// of the row set k = i nq + j from the model's primitive P and scalar X
+// (ok false: the model cannot be evaluated there).
+template <int SYS>
+__device__ inline void model_row_zone(bool ok, Prim P, double X, const ZoneParams& zp, int nq, const GhostRows& g,
+                                      const double* dcost, int seq, int tag0, double* out, ErrLog* err, int k)
+{
+    const int n2 = 2 * nq;
+    const int i = k / nq, j = k % nq;
+    const Cons c = p2c<SYS>(P, zp);
+    const double dv = g.volA[i] * dcost[j];
+    const double ud = c.d * dv, usr = c.sr * dv, usq = c.sq * dv, utau = c.tau * dv;
+    const double um = ud * X;
+    const Cons u{ud / dv, usr / dv, usq / dv, utau / dv};
+    Prim Q{0.0, 0.0, 0.0, 0.0};
+    double rW = 1.0, value = u.d;
+    int code = ok ? E_OK : KNC_ERR_MODEL;
+    if (code == E_OK) {
+        if (SYS == SYS_EULER) {
+            code = c2p_euler(u, zp, &Q);
+        } else if (u.d < 0.0) {
+            code = E_NEGATIVE_DENSITY;
+        } else if (u.tau < 0.0) {
+            code = E_NEGATIVE_ENERGY_DENSITY;
+            value = u.tau;
+        } else {
+            const double S2 = u.sr * u.sr + u.sq * u.sq;
+            double p = 0.0;
+            int it = 0;
+            code = c2p_srhd_newton(u.d, S2, u.tau, zp, 0.0, &p, &it);
+            if (code == E_OK) {
+                code = c2p_srhd_finish(u, S2, p, &Q, &rW);
+                if (code == E_NEGATIVE_PRESSURE) code = E_OK;
+            } else {
+                value = u.tau;
+            }
+        }
+    }
+    double s = 0.0;
+    if (code != E_OK) {
+        record_error(err, seq, tag0 | i, j, code, value, u);
+        Q = Prim{0.0, 0.0, 0.0, 0.0};
+    } else {
+        s = um / dv;
+        if (SYS == SYS_SRHD) s = s / sqrt_rn_ge1(1.0 + Q.ur * Q.ur + Q.uq * Q.uq);
+    }
+    out[k] = Q.rho;
+    out[n2 + k] = Q.ur;
+    out[2 * n2 + k] = Q.uq;
+    out[3 * n2 + k] = Q.p;
+    out[4 * n2 + k] = s;
+}
+
 template <int SYS>
 __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, const double* qc, const double* dcost,
                              double t, int seq, int tag0, double* out, ErrLog* err)
 {
-    const int n2 = 2 * nq;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
-        const int i = k / nq, j = k % nq;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 2 * nq; k += gridDim.x * blockDim.x) {
         Prim P{0.0, 0.0, 0.0, 0.0};
         double X = 0.0;
-        const bool ok = model_at(m, g.rc[i], qc[j], t, &P, &X);
-        const Cons c = p2c<SYS>(P, zp);
-        const double dv = g.volA[i] * dcost[j];
-        const double ud = c.d * dv, usr = c.sr * dv, usq = c.sq * dv, utau = c.tau * dv;
-        const double um = ud * X;
-        const Cons u{ud / dv, usr / dv, usq / dv, utau / dv};
-        Prim Q{0.0, 0.0, 0.0, 0.0};
-        double rW = 1.0, value = u.d;
-        int code = ok ? E_OK : KNC_ERR_MODEL;
-        if (code == E_OK) {
-            if (SYS == SYS_EULER) {
-                code = c2p_euler(u, zp, &Q);
-            } else if (u.d < 0.0) {
-                code = E_NEGATIVE_DENSITY;
-            } else if (u.tau < 0.0) {
-                code = E_NEGATIVE_ENERGY_DENSITY;
-                value = u.tau;
-            } else {
-                const double S2 = u.sr * u.sr + u.sq * u.sq;
-                double p = 0.0;
-                int it = 0;
-                code = c2p_srhd_newton(u.d, S2, u.tau, zp, 0.0, &p, &it);
-                if (code == E_OK) {
-                    code = c2p_srhd_finish(u, S2, p, &Q, &rW);
-                    if (code == E_NEGATIVE_PRESSURE) code = E_OK;
-                } else {
-                    value = u.tau;
-                }
-            }
-        }
-        double s = 0.0;
-        if (code != E_OK) {
-            record_error(err, seq, tag0 | i, j, code, value, u);
-            Q = Prim{0.0, 0.0, 0.0, 0.0};
-        } else {
-            s = um / dv;
-            if (SYS == SYS_SRHD) s = s / sqrt_rn_ge1(1.0 + Q.ur * Q.ur + Q.uq * Q.uq);
-        }
-        out[k] = Q.rho;
-        out[n2 + k] = Q.ur;
-        out[2 * n2 + k] = Q.uq;
-        out[3 * n2 + k] = Q.p;
-        out[4 * n2 + k] = s;
+        const bool ok = model_at(m, g.rc[k / nq], qc[k % nq], t, &P, &X);
+        model_row_zone<SYS>(ok, P, X, zp, nq, g, dcost, seq, tag0, out, err, k);
     }
+}
+
+// The ghost rows of halo_kilonova, one workgroup per zone.  The zone's
+// pressure is 1e-3 plus some thousand RK4 steps of the vertical gravity,
+// each a dozen pow of FP64: as one thread's loop (k_model_rows) the rows
+// took milliseconds per stage.  The altitudes of the steps are accumulated
+// in order by one thread (the reference's zz += dz), the steps evaluated
+// by all threads, and their sum taken in order by one thread, so the
+// pressure is the loop's bit for bit.  The grid is exactly 2 nq workgroups.
+template <int SYS>
+__global__ void __launch_bounds__(256) k_model_rows_halo(ModelDev m, ZoneParams zp, int nq, GhostRows g,
+                                                        const double* qc, const double* dcost, double t, int seq,
+                                                        int tag0, double* out, ErrLog* err)
+{
+    __shared__ double alt[HALO_MAX_STEPS];
+    __shared__ double step[HALO_MAX_STEPS];
+    __shared__ int nsteps;
+    const int k = blockIdx.x;
+    const double r = g.rc[k / nq];
+    Prim P{0.0, 0.0, 0.0, 0.0};
+    double X = 0.0;
+    HaloDisk h;
+    // the same inputs in every thread: c is uniform over the workgroup
+    const int c = halo_kilonova_begin(m.p, r, qc[k % nq], t, &P, &X, &h);
+    if (c == 1) {
+        if (threadIdx.x == 0) {
+            double zz = 0.0;
+            int n = 0;
+            for (; zz < h.z && n < HALO_MAX_STEPS; ++n) {
+                alt[n] = zz;
+                zz += h.dz;
+            }
+            nsteps = n;
+        }
+        __syncthreads();
+        const int ns = nsteps;
+        for (int n = threadIdx.x; n < ns; n += blockDim.x) step[n] = gal::pressure_difference_rk4(r, alt[n], h.dz);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double pr = 1e-3;
+            for (int n = 0; n < ns; ++n) pr += step[n];
+            P = Prim{h.d, 0.0, 0.0, pr};
+            X = 0.0;
+        }
+    }
+    if (threadIdx.x == 0) model_row_zone<SYS>(c >= 0, P, X, zp, nq, g, dcost, seq, tag0, out, err, k);
 }
 
 int grid_of(int64_t n, int block)
@@ -149,8 +204,19 @@ hipError_t launch_block_from_model(int system, const ModelDev& m, const ZonePara
 
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
                              const double* qc, const double* dcost, double t, int seq,
-                             int tag0, double* out, ErrLog* err, hipStream_t s)
+                             int tag0, double* out, ErrLog* err, hipStream_t s, bool one_thread_per_zone)
 {
+    if (m.kind == MODEL_HALO_KILONOVA && !one_thread_per_zone) {
+        if (nq < 1) return hipSuccess;
+        const dim3 g(2 * nq), b(256);
+        if (system == SYS_SRHD)
+            hipLaunchKernelGGL(k_model_rows_halo<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq, tag0,
+                               out, err);
+        else
+            hipLaunchKernelGGL(k_model_rows_halo<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq,
+                               tag0, out, err);
+        return hipGetLastError();
+    }
     const dim3 g(grid_of(2 * (int64_t)nq, 256)), b(256);
     if (system == SYS_SRHD)
         hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq, tag0, out,

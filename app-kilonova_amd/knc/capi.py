@@ -224,6 +224,7 @@ def lib():
     L.knc_hydro_max_signal_speed.argtypes = [ctypes.POINTER(Hydro), I64, P, P]
     L.knc_hydro_plm_gradient.argtypes = [D, I64, P, P, P, P]
     L.knc_fp64_probe.argtypes = [I64, P, P, P, P, P, P, P]
+    L.knc_model_rows_check.argtypes = [P, I32, D, P, P]
     L.knc_fp64_fma_rate.argtypes = [I32, I32, I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]
     L.knc_mesh_block_vertices.argtypes = [ctypes.POINTER(Mesh), I32, P, P]
     L.knc_model_eval.argtypes = [P, ctypes.POINTER(Model), I64, P, P, D, P, P]
@@ -312,6 +313,16 @@ class Context:
             return
         m, _keep = make_model(kind, params, table)
         _check(lib().knc_set_boundary_model(self._h, int(side), ctypes.byref(mesh), ctypes.byref(m)))
+
+    def model_rows_check(self, side, time):
+        """The ghost rows of `side`'s device boundary model at `time`, as the
+        stage builds them and with one thread per zone: two (5, 2, nq)
+        arrays (rho, u_r, u_theta, p, s by the two rows and the columns)."""
+        nq = next(iter(self.shapes.values()))[1]
+        a = np.zeros((5, 2, nq))
+        b = np.zeros((5, 2, nq))
+        _check(lib().knc_model_rows_check(self._h, int(side), float(time), _p(a), _p(b)))
+        return a, b
 
     def set_birth_model(self, mesh, kind=None, params=(), table=None):
         """Blocks born by excision built from the initial model on the device

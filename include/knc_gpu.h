@@ -518,6 +518,16 @@ int knc_fp64_fma_rate(int chains, int waves_per_simd, int iters, double* fma_per
 int knc_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sqrt_abs_a,
                    double* mul_add, double* quot_abs_b);
 
+/* The ghost rows of one side's device boundary model (knc_set_boundary_model)
+ * at `time` for the context's current block set, each written as 5 planes of
+ * 2 nq doubles (rho, u_r, u_theta, p, s; the row next to the mesh edge
+ * second on the inner side, first on the outer), twice: as the stage builds
+ * them (rows) and with one thread per zone (rows_one_thread).  A check with
+ * no counterpart in the reference: the library builds halo_kilonova's rows
+ * with a workgroup per zone, and the two must agree bit for bit.  Leaves the
+ * context's error log empty. */
+int knc_model_rows_check(knc_context* c, int32_t side, double time, double* rows, double* rows_one_thread);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,6 +197,30 @@ def test_halo_kilonova_setup_matches_oracle():
     _run_pair(p, 1, 5, False, one_step=1e-11)
 
 
+@pytest.mark.parametrize("outer_radius", ["5e19", "9e19"])
+def test_halo_ghost_rows_of_the_workgroup_kernel_equal_one_thread_per_zone(outer_radius):
+    """halo_kilonova's ghost rows are built with one workgroup per zone (the
+    RK4 steps of the pressure integration spread over its threads, their
+    altitudes and their sum taken in order); they equal the rows of the
+    one-thread-per-zone kernel bit for bit, on both sides, at the start
+    time and later, full size (256 polar zones).  With the outer radius at
+    9e19 the outer rows reach below the midplane near theta = pi: those
+    zones are zeros in both, and the check leaves no error behind."""
+    p = presets.parse(presets.apply_overrides(SETUP_HALO, [f"mesh.outer_radius={outer_radius}"]))
+    ctx = presets.context_from_preset(p, launch_mode=knc.BATCHED)
+    t0 = ctx.time
+    for side in (capi.INNER, capi.OUTER):
+        for t in (t0, t0 + 3e9):
+            a, b = ctx.model_rows_check(side, t)
+            assert a.shape == (5, 2, 256)
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (side, t)
+            live = a[3] > 0.0
+            assert live.sum() >= (512 if outer_radius == "5e19" else 256)
+    if outer_radius == "5e19":
+        ctx.advance(2)   # no error was left in the log
+    ctx.close()
+
+
 def test_model_ghost_row_failure_is_reported_at_the_oracles_zone():
     """halo_kilonova with the outer radius at 9e19: the blocks lie inside
     the disk's domain but the outer ghost rows (mesh block 8, r = 1.11e20)
