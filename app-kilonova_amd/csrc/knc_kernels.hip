@@ -572,9 +572,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
         // for the c2p, row k-2 being also the update operand U of step k
         Raw5 RA, RB, C0 = load_raw<GUESS && !PRIMS, PRIMS>(S0, j), C1;
         int k = it.i0 - 2;
-        if constexpr (RS == RS_HLLC) {
+        if constexpr (RS == RS_HLLC && (GUESS || PRIMS)) {
             // HLLC's star states leave no registers for a second step in
-            // flight: one step per iteration, the carried values copied
+            // flight beside a guess plane or the two-pass form's operands:
+            // one step per iteration, the carried values copied
             for (; k < it.i1 + 2; ++k) {
                 march_step(k, X0, X1, X2, gm2, F0, F1, R0, R1, C0, C1, RA, S0, S1);
                 F0 = F1;
