@@ -196,13 +196,20 @@ constexpr double B_S = 1.790e21;
 constexpr double A_G = 1.461e22;
 constexpr double B_G = 7.035e23;
 
+// x^1.5 and x^2.5 of a positive x, the reference's powf(1.5) and
+// powf(2.5) of the galactic model, as x sqrt(x) and (x x) sqrt(x): within
+// two roundings of the power, like the device pow, at a tenth of its
+// instructions.  The ghost rows evaluate them some 20 000 times per zone.
+__device__ inline double pow_1p5(double x) { return x * sqrt_rn(x); }
+__device__ inline double pow_2p5(double x) { return (x * x) * sqrt_rn(x); }
+
 // GalacticModel::density(r, z).thin_disk.  powi(n) of the reference is
-// repeated multiplication (x*x, x*(x*x), (x*x)*(x*x)); powf is pow.
+// repeated multiplication (x*x, x*(x*x), (x*x)*(x*x)).
 __device__ inline double thin_disk_density(double r, double z)
 {
     const double zz = z * z + B_S * B_S;
     const double zb = sqrt(zz);
-    const double zb3 = pow(zz, 1.5);
+    const double zb3 = zz * zb;   // pow(zz, 1.5), with zb = sqrt(zz)
     const double z2 = z * z, b2 = B_S * B_S;
     const double z4 = z2 * z2, b4 = b2 * b2, a3 = A_S * (A_S * A_S);
     const double za = zb + A_S;
@@ -211,10 +218,10 @@ __device__ inline double thin_disk_density(double r, double z)
                                                3.0 * A_S * A_S * B_S * B_S) +
                         zb3 * (2.0 * z * z - B_S * B_S) + A_S * B_S * B_S * z * z - A_S * B_S * B_S * r * r -
                         3.0 * A_S * b4 - a3 * B_S * B_S) /
-                       (zb3 * pow(za * za + r * r, 2.5)));
+                       (zb3 * pow_2p5(za * za + r * r)));
     const double as = A_S + zb;
     const double w = r * r + as * as;
-    const double dr = M_S / 4.0 / KNC_PI * ((2.0 * w - 3.0 * r * r) / pow(w, 2.5));
+    const double dr = M_S / 4.0 / KNC_PI * ((2.0 * w - 3.0 * r * r) / pow_2p5(w));
     return dz + dr;
 }
 
@@ -222,13 +229,13 @@ __device__ inline double thin_disk_density(double r, double z)
 // bulge mass m_b, as the reference does)
 __device__ inline double g_field_z(double r, double z)
 {
-    const double bulge = -G * M_B * z / pow(r * r + z * z + A_B * A_B, 1.5);
+    const double bulge = -G * M_B * z / pow_1p5(r * r + z * z + A_B * A_B);
     const double zs = sqrt(z * z + B_S * B_S);
     const double as = A_S + zs;
-    const double thin = -G * M_B * z * (A_S + zs) / pow(r * r + as * as, 1.5) / zs;
+    const double thin = -G * M_B * z * (A_S + zs) / pow_1p5(r * r + as * as) / zs;
     const double zg = sqrt(z * z + B_G * B_G);
     const double ag = A_G + zg;
-    const double thick = -G * M_B * z * (A_G + zg) / pow(r * r + ag * ag, 3.0 / 2.0) / zg;
+    const double thick = -G * M_B * z * (A_G + zg) / pow_1p5(r * r + ag * ag) / zg;
     const double halo = -V_H * V_H * z / (r * r + z * z + A_H * A_H);
     return bulge + thin + thick + halo;
 }
