@@ -874,10 +874,22 @@ int excise(knc_context* c, double t, bool apply, bool* change)
     return KNC_OK;
 }
 
-// Room for the ghost-row sets of every stage of a fold.
+// Room for the ghost-row sets of every stage of a fold, and for their
+// descriptors.
 int ensure_boundary_sets(knc_context* c, int fold)
 {
-    const size_t need = (size_t)fold * c->h.runge_kutta_order * 2 * 10 * c->nq;
+    const size_t nsets = (size_t)fold * c->h.runge_kutta_order * 2;
+    const size_t need = nsets * 10 * c->nq;
+    if (nsets > c->cap_rowsets) {
+        if (c->d_rowsets) HIP_TRY(hipFree(c->d_rowsets));
+        if (c->h_rowsets) HIP_TRY(hipHostFree(c->h_rowsets));
+        c->d_rowsets = nullptr;
+        c->h_rowsets = nullptr;
+        c->cap_rowsets = 0;
+        HIP_TRY(hipMalloc(&c->d_rowsets, nsets * sizeof(RowSet)));
+        HIP_TRY(hipHostMalloc(&c->h_rowsets, nsets * sizeof(RowSet), hipHostMallocDefault));
+        c->cap_rowsets = nsets;
+    }
     if (need <= c->cap_bseq) return KNC_OK;
     drop_graphs(c);
     if (c->d_bseq) HIP_TRY(hipFree(c->d_bseq));
@@ -932,22 +944,31 @@ int flush_steps(knc_context* c, int first, int n)
     }
     if (c->bmodel[0].on || c->bmodel[1].on) {
         // device-model sides, written after the host sets they share the
-        // buffer with; the block set is fixed within a segment
+        // buffer with; the block set is fixed within a segment.  Every row
+        // set of the segment is known here (the stage times are), so each
+        // side's sets are built in one launch.  The descriptors of step s
+        // sit at entries [2 order s, 2 order (s + 1)) of the pinned buffer,
+        // which earlier segments of the fold do not reuse.
         const int order = c->h.runge_kutta_order;
         const int32_t inner = c->blocks.begin()->first, outer = c->blocks.rbegin()->first;
-        GhostRows g[2];
-        for (int side = 0; side < 2; ++side)
-            if (c->bmodel[side].on) g[side] = ghost_rows(c->bmodel[side].mesh, side, inner, outer);
         for (int s = first; s < first + n; ++s)
             for (int st = 0; st < order; ++st)
                 for (int side = 0; side < 2; ++side) {
-                    if (!c->bmodel[side].on) continue;
-                    double* out = c->d_bseq + (((size_t)s * order + st) * 2 + side) * 10 * c->nq;
-                    const int tag0 = (side ? GHOST_OUTER_ORDER : GHOST_INNER_ORDER) << 16;
-                    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g[side], c->d_qc,
-                                              c->th.dcost, c->stage_times[(size_t)s * order + st],
-                                              stage_seq(c, s - first + 1, st, true), tag0, out, c->d_err, c->stream));
+                    const size_t e = ((size_t)s * order + st) * 2 + side;
+                    c->h_rowsets[e] = RowSet{c->stage_times[(size_t)s * order + st],
+                                             stage_seq(c, s - first + 1, st, true), 0, (int64_t)(e * 10 * c->nq)};
                 }
+        const size_t e0 = (size_t)first * order * 2, ne = (size_t)n * order * 2;
+        HIP_TRY(hipMemcpyAsync(c->d_rowsets + e0, c->h_rowsets + e0, ne * sizeof(RowSet), hipMemcpyHostToDevice,
+                               c->stream));
+        for (int side = 0; side < 2; ++side) {
+            if (!c->bmodel[side].on) continue;
+            const GhostRows g = ghost_rows(c->bmodel[side].mesh, side, inner, outer);
+            const int tag0 = (side ? GHOST_OUTER_ORDER : GHOST_INNER_ORDER) << 16;
+            HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g, c->d_qc, c->th.dcost,
+                                      c->d_rowsets + e0 + side, n * order, 2, tag0, c->d_bseq, c->d_err,
+                                      c->stream));
+        }
     }
     return enqueue_steps(c, n, first);
 }
@@ -1263,8 +1284,9 @@ void knc_context_destroy(knc_context* c)
     free_model(c->bmodel[1]);
     free_model(c->birth);
     void* ptrs[] = {c->d_theta, c->d_bnd, c->d_blocks, c->d_items, c->d_nitems, c->d_err, c->d_stats,
-                    c->d_dt_partial, c->d_dt, c->d_mlp, c->d_bseq, c->d_qc};
+                    c->d_dt_partial, c->d_dt, c->d_mlp, c->d_bseq, c->d_qc, c->d_rowsets};
     if (c->h_bseq) (void)hipHostFree(c->h_bseq);
+    if (c->h_rowsets) (void)hipHostFree(c->h_rowsets);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1916,16 +1938,19 @@ int knc_model_rows_check(knc_context* c, int32_t side, double time, double* rows
     if ((rc = require_blocks(c))) return rc;
     if (!c->bmodel[side].on) return fail(KNC_ERR_INVALID_ARGUMENT, "no device boundary model on this side");
     const size_t n = (size_t)10 * c->nq;
-    DevBuf d1, d2;
+    DevBuf d1, d2, ds;
     ZONE_ALLOC(d1, n * 8);
     ZONE_ALLOC(d2, n * 8);
+    ZONE_ALLOC(ds, sizeof(RowSet));
+    const RowSet set{time, 0, 0, 0};
+    HIP_TRY(hipMemcpy(ds.p, &set, sizeof set, hipMemcpyHostToDevice));
     const GhostRows g = ghost_rows(c->bmodel[side].mesh, side, c->blocks.begin()->first, c->blocks.rbegin()->first);
     const int tag0 = (side ? GHOST_OUTER_ORDER : GHOST_INNER_ORDER) << 16;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g, c->d_qc, c->th.dcost, time, 0, tag0,
-                              (double*)d1.p, c->d_err, c->stream));
-    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g, c->d_qc, c->th.dcost, time, 0, tag0,
-                              (double*)d2.p, c->d_err, c->stream, true));
+    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g, c->d_qc, c->th.dcost,
+                              (const RowSet*)ds.p, 1, 1, tag0, (double*)d1.p, c->d_err, c->stream));
+    HIP_TRY(launch_model_rows(c->h.system, c->bmodel[side].dev, c->zp, c->nq, g, c->d_qc, c->th.dcost,
+                              (const RowSet*)ds.p, 1, 1, tag0, (double*)d2.p, c->d_err, c->stream, true));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(rows, d1.p, n * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(rows_one_thread, d2.p, n * 8, hipMemcpyDeviceToHost));

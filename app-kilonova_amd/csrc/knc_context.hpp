@@ -113,6 +113,11 @@ struct knc_context {
     double* h_bseq = nullptr;   // pinned
     double* d_bseq = nullptr;
     size_t cap_bseq = 0;        // doubles
+    // the device-model sides' row sets of a fold, one per (step, stage,
+    // side) in the order of the sets in d_bseq
+    knc::RowSet* h_rowsets = nullptr;   // pinned
+    knc::RowSet* d_rowsets = nullptr;
+    size_t cap_rowsets = 0;
     bool trial = false;         // autotuner trial steps: fixed rows only
 
     // initial models evaluated on the device: per side for model boundaries,

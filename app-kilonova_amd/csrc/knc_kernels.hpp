@@ -117,13 +117,23 @@ hipError_t launch_model_eval(const ModelDev& m, int64_t n, const double* r, cons
 hipError_t launch_block_from_model(int system, const ModelDev& m, const ZoneParams& zp, int nr, int nq,
                                    const double* rc, const double* volA, const double* qc, const double* dcost,
                                    double t, double* planes, int* fail, hipStream_t s);
-// The ghost rows of one side (5 planes x 2 nq) from a device model.  With
+// One ghost-row set of a side: the stage time, the error-log sequence of
+// its stage and where its 5 planes x 2 nq go (in doubles from the base).
+struct RowSet {
+    double t;
+    int32_t seq;
+    int32_t pad;
+    int64_t offset;
+};
+
+// The ghost rows of one side from a device model, for nsets row sets at
+// sets[0], sets[stride], ... (device memory), in one launch.  With
 // one_thread_per_zone the zone-per-thread kernel runs for every model (the
 // library otherwise takes the workgroup-per-zone kernel for halo_kilonova);
 // knc_model_rows_check compares the two.
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
-                             const double* qc, const double* dcost, double t, int seq,
-                             int tag0, double* out, ErrLog* err, hipStream_t s, bool one_thread_per_zone = false);
+                             const double* qc, const double* dcost, const RowSet* sets, int nsets, int stride,
+                             int tag0, double* base, ErrLog* err, hipStream_t s, bool one_thread_per_zone = false);
 hipError_t launch_fma_probe(int chains, int iters, int grid, double* out, hipStream_t s);
 hipError_t launch_fp64_probe(int64_t n, const double* a, const double* b, const double* c, double* div, double* sq,
                              double* mad, double* quot, hipStream_t s);

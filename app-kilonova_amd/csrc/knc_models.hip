@@ -116,13 +116,16 @@ __device__ inline void model_row_zone(bool ok, Prim P, double X, const ZoneParam
 
 template <int SYS>
 __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, const double* qc, const double* dcost,
-                             double t, int seq, int tag0, double* out, ErrLog* err)
+                             const RowSet* sets, int nsets, int stride, int tag0, double* base, ErrLog* err)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 2 * nq; k += gridDim.x * blockDim.x) {
+    const int64_t n2 = 2 * (int64_t)nq, n = nsets * n2;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        const RowSet rs = sets[(x / n2) * stride];
+        const int k = (int)(x % n2);
         Prim P{0.0, 0.0, 0.0, 0.0};
         double X = 0.0;
-        const bool ok = model_at(m, g.rc[k / nq], qc[k % nq], t, &P, &X);
-        model_row_zone<SYS>(ok, P, X, zp, nq, g, dcost, seq, tag0, out, err, k);
+        const bool ok = model_at(m, g.rc[k / nq], qc[k % nq], rs.t, &P, &X);
+        model_row_zone<SYS>(ok, P, X, zp, nq, g, dcost, rs.seq, tag0, base + rs.offset, err, k);
     }
 }
 
@@ -132,16 +135,19 @@ __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, con
 // took milliseconds per stage.  The altitudes of the steps are accumulated
 // in order by one thread (the reference's zz += dz), the steps evaluated
 // by all threads, and their sum taken in order by one thread, so the
-// pressure is the loop's bit for bit.  The grid is exactly 2 nq workgroups.
+// pressure is the loop's bit for bit.  The grid is exactly nsets x 2 nq
+// workgroups.
 template <int SYS>
 __global__ void __launch_bounds__(256) k_model_rows_halo(ModelDev m, ZoneParams zp, int nq, GhostRows g,
-                                                        const double* qc, const double* dcost, double t, int seq,
-                                                        int tag0, double* out, ErrLog* err)
+                                                        const double* qc, const double* dcost, const RowSet* sets,
+                                                        int stride, int tag0, double* base, ErrLog* err)
 {
     __shared__ double alt[HALO_MAX_STEPS];
     __shared__ double step[HALO_MAX_STEPS];
     __shared__ int nsteps;
-    const int k = blockIdx.x;
+    const int k = blockIdx.x % (2 * nq);
+    const RowSet rs = sets[(blockIdx.x / (2 * nq)) * stride];
+    const double t = rs.t;
     const double r = g.rc[k / nq];
     Prim P{0.0, 0.0, 0.0, 0.0};
     double X = 0.0;
@@ -169,7 +175,7 @@ __global__ void __launch_bounds__(256) k_model_rows_halo(ModelDev m, ZoneParams 
             X = 0.0;
         }
     }
-    if (threadIdx.x == 0) model_row_zone<SYS>(c >= 0, P, X, zp, nq, g, dcost, seq, tag0, out, err, k);
+    if (threadIdx.x == 0) model_row_zone<SYS>(c >= 0, P, X, zp, nq, g, dcost, rs.seq, tag0, base + rs.offset, err, k);
 }
 
 int grid_of(int64_t n, int block)
@@ -203,27 +209,27 @@ hipError_t launch_block_from_model(int system, const ModelDev& m, const ZonePara
 }
 
 hipError_t launch_model_rows(int system, const ModelDev& m, const ZoneParams& zp, int nq, const GhostRows& rows,
-                             const double* qc, const double* dcost, double t, int seq,
-                             int tag0, double* out, ErrLog* err, hipStream_t s, bool one_thread_per_zone)
+                             const double* qc, const double* dcost, const RowSet* sets, int nsets, int stride,
+                             int tag0, double* base, ErrLog* err, hipStream_t s, bool one_thread_per_zone)
 {
+    if (nq < 1 || nsets < 1) return hipSuccess;
     if (m.kind == MODEL_HALO_KILONOVA && !one_thread_per_zone) {
-        if (nq < 1) return hipSuccess;
-        const dim3 g(2 * nq), b(256);
+        const dim3 g(nsets * 2 * nq), b(256);
         if (system == SYS_SRHD)
-            hipLaunchKernelGGL(k_model_rows_halo<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq, tag0,
-                               out, err);
+            hipLaunchKernelGGL(k_model_rows_halo<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, sets, stride,
+                               tag0, base, err);
         else
-            hipLaunchKernelGGL(k_model_rows_halo<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq,
-                               tag0, out, err);
+            hipLaunchKernelGGL(k_model_rows_halo<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, sets, stride,
+                               tag0, base, err);
         return hipGetLastError();
     }
-    const dim3 g(grid_of(2 * (int64_t)nq, 256)), b(256);
+    const dim3 g(grid_of((int64_t)nsets * 2 * nq, 256)), b(256);
     if (system == SYS_SRHD)
-        hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq, tag0, out,
-                           err);
+        hipLaunchKernelGGL(k_model_rows<SYS_SRHD>, g, b, 0, s, m, zp, nq, rows, qc, dcost, sets, nsets, stride, tag0,
+                           base, err);
     else
-        hipLaunchKernelGGL(k_model_rows<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, t, seq, tag0,
-                           out, err);
+        hipLaunchKernelGGL(k_model_rows<SYS_EULER>, g, b, 0, s, m, zp, nq, rows, qc, dcost, sets, nsets, stride,
+                           tag0, base, err);
     return hipGetLastError();
 }
 
