@@ -131,8 +131,9 @@ __global__ void k_model_rows(ModelDev m, ZoneParams zp, int nq, GhostRows g, con
 
 // The ghost rows of halo_kilonova, one workgroup per zone.  The zone's
 // pressure is 1e-3 plus some thousand RK4 steps of the vertical gravity,
-// each a dozen pow of FP64: as one thread's loop (k_model_rows) the rows
-// took milliseconds per stage.  The altitudes of the steps are accumulated
+// each three evaluations of the disk's density and field (fractional
+// powers, square roots and divisions of FP64): as one thread's loop
+// (k_model_rows) the rows took milliseconds per stage.  The altitudes of the steps are accumulated
 // in order by one thread (the reference's zz += dz), the steps evaluated
 // by all threads, and their sum taken in order by one thread, so the
 // pressure is the loop's bit for bit.  The grid is exactly nsets x 2 nq
