@@ -202,9 +202,10 @@ bool guess_read(const knc_context* c)
 {
     return learned(c) || (c->o.c2p_guess == KNC_GUESS_PREVIOUS && c->h.system == KNC_SRHD);
 }
-// The stage form (STAGE_FUSED, STAGE_TWO_PASS, STAGE_SPECIALISED).
+// The stage form (STAGE_FUSED, STAGE_TWO_PASS, STAGE_SPECIALISED, STAGE_ROWS).
 int stage_form(const knc_context* c) { return c->o.stage_passes == 0 ? STAGE_FUSED : c->o.stage_passes; }
-bool two_pass(const knc_context* c) { return stage_form(c) == STAGE_TWO_PASS; }
+// Forms whose flux pass reads the primitive planes of a c2p pass.
+bool two_pass(const knc_context* c) { return stage_form(c) == STAGE_TWO_PASS || stage_form(c) == STAGE_ROWS; }
 // A side whose ghost rows are refreshed at every stage time (host callback
 // or device model).
 bool model_side(const knc_context* c, int side) { return c->bsrc[side] || c->bmodel[side].on; }
@@ -538,8 +539,12 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment
         a.items = c->d_items;
         a.nitems_ptr = c->d_nitems;
         if ((rc = pre_pass(a, c->nitems))) return rc;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, with_dt ? (int)STAGE_TWO_PASS : stage_form(c),
-                             a, with_dt ? c->persistent_grid_two_pass : c->persistent_grid, c->threads, c->stream));
+        // the dt reduction needs a c2p pass: a fused form's first stage of
+        // the run takes the two-pass form; the row-unit form is two-pass
+        const int form = with_dt && stage_form(c) != STAGE_ROWS ? (int)STAGE_TWO_PASS : stage_form(c);
+        const int grid = form == STAGE_ROWS ? row_pass_grid(c, c->nitems)
+                                            : with_dt ? c->persistent_grid_two_pass : c->persistent_grid;
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, form, a, grid, c->threads, c->stream));
         c->last_launches += 1;
         return KNC_OK;
     }
@@ -554,7 +559,8 @@ int launch_one_stage(knc_context* c, StageArgs a, bool first_stage, bool segment
         a.items = c->d_items + bi.first;
         a.nitems_ptr = nullptr;
         a.nitems = bi.second;
-        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, stage_form(c), a, bi.second, c->threads,
+        const int grid = stage_form(c) == STAGE_ROWS ? row_pass_grid(c, bi.second) : bi.second;
+        HIP_TRY(launch_stage(c->h.system, c->h.riemann_solver, stats, lg, stage_form(c), a, grid, c->threads,
                              c->stream));
         c->last_launches += 1;
     }
@@ -1147,7 +1153,7 @@ int autotune(knc_context* c, int trials, bool force, knc_tune_result* out)
     int ncand = 0;
     const int widths[] = {64, 128, 256, 512};
     const int rows[] = {4, 8, 16, 32, 64};
-    for (int passes = 1; passes <= 3 && rc == KNC_OK; ++passes)
+    for (int passes = 1; passes <= 4 && rc == KNC_OK; ++passes)
         for (int t : widths) {
             if (rc) break;
             if (!stage_variant_supported(t, passes)) continue;
@@ -1231,9 +1237,9 @@ int knc_context_create(const knc_hydro* hydro, const knc_options* options, knc_c
         delete c;
         return fail(KNC_ERR_INVALID_ARGUMENT, "unknown c2p guess mode");
     }
-    if (c->o.stage_passes < 0 || c->o.stage_passes > 3) {
+    if (c->o.stage_passes < 0 || c->o.stage_passes > 4) {
         delete c;
-        return fail(KNC_ERR_INVALID_ARGUMENT, "stage_passes must be 0, 1, 2 or 3");
+        return fail(KNC_ERR_INVALID_ARGUMENT, "stage_passes must be 0, 1, 2, 3 or 4");
     }
     c->zp = zone_params(c->h);
     int ndev = 0;

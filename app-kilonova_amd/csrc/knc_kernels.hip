@@ -617,6 +617,156 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
 }
 
 // ---------------------------------------------------------------------------
+// Stage form 4: the flux pass of the two-pass stage with one (work item,
+// row) unit per workgroup pass instead of a march down the item's rows.  A
+// unit reads the primitives of rows r-2 .. r+2 that k_prim wrote, forms the
+// radial slopes of rows r-1, r and r+1 and both radial faces of row r (each
+// face is formed by the two rows that share it, from the same operands, so
+// with the same bits), the polar slopes and faces of row r through LDS, and
+// the update.  Items of a few rows (the reference's 4-row blocks) march a
+// few steps each and leave most of the GPU idle; their rows as units fill
+// it.  The expressions and their order are k_stage's.
+// ---------------------------------------------------------------------------
+__device__ inline PS row_prim(const RowSrc& src, int col)
+{
+    const Raw5 x = load_raw<false, true>(src, col);
+    return PS{{x.v[0], x.v[1], x.v[2], x.v[3]}, x.v[4]};
+}
+
+template <int SYS, int RS, int NT>
+__global__ void __launch_bounds__(NT) k_stage_rows(StageArgs a)
+{
+    constexpr int WP = NT + 4;                 // row r: j0-2, j0-1 | NT own | j1, j1+1
+    __shared__ double ldsP[5 * WP];
+    __shared__ double ldsPL[5 * NT];           // left state of each column's upper polar face
+    __shared__ double ldsF[5 * NT];            // polar flux through each column's lower face
+    const int tid = threadIdx.x;
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double bw = a.bweight;
+    const double th = a.zp.plm_theta;
+    const double c = a.zp.c;
+    const bool avg = a.avg_slot >= 0;
+    const double w1 = -bw + 1.0;
+    const PS zero{{0.0, 0.0, 0.0, 0.0}, 0.0};
+    const int nunits = (a.nitems_ptr ? *a.nitems_ptr : a.nitems) * a.rows;
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const WorkItem it = a.items[u / a.rows];
+        const int r = it.i0 + u % a.rows;
+        if (r >= it.i1) continue;   // uniform over the workgroup
+        const BlockDesc B = a.blocks[it.block];
+        const int nq = B.nq;
+        const int j0 = it.j0, j1 = it.j1, n = j1 - j0;
+        const int64_t np = (int64_t)B.nr * nq;
+        const bool act = tid < n;
+        const int j = act ? j0 + tid : j1 - 1;
+        PS P[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) P[m] = row_prim(resolve_row<true>(a, B, r - 2 + m), j);
+        // Radial slopes (scheme.rs:99-105) and the faces below and above row
+        // r (scheme.rs:81-97), as the march forms radial face k-1 in step k.
+        const PS g0 = plm5(th, P[0], P[1], P[2]);
+        const PS g1 = plm5(th, P[1], P[2], P[3]);
+        const PS g2 = plm5(th, P[2], P[3], P[4]);
+        Cons F;
+        double Fs;
+        riemann<SYS, RS, 0>(add_half(P[1], g0), sub_half(P[2], g1), a.zp, &F, &Fs);
+        const Flux5 Fm = scaled(F, Fs, c, gld(zg_ar(B), (int64_t)r * nq + j));
+        riemann<SYS, RS, 0>(add_half(P[2], g1), sub_half(P[3], g2), a.zp, &F, &Fs);
+        const Flux5 Fk = scaled(F, Fs, c, gld(zg_ar(B), (int64_t)(r + 1) * nq + j));
+        // Row r and the item's halo columns j0-2, j0-1, j1, j1+1 into LDS.
+        __syncthreads();   // the previous unit's reads of the LDS rows are done
+        if (act) lds_put(ldsP, WP, tid + 2, P[2]);
+        if (tid < 4) {
+            const int col = tid < 2 ? j0 - 2 + tid : j1 + tid - 2;
+            if (col >= 0 && col < nq) lds_put(ldsP, WP, tid < 2 ? tid : n + tid, row_prim(block_row<true>(a, B, r), col));
+        }
+        __syncthreads();
+        // Polar slope of column j (scheme.rs:107-110, zero at the first and
+        // last columns) and the states of its two polar faces.
+        PS gy = zero;
+        if (j >= 1 && j + 1 < nq) gy = plm5(th, lds_get(ldsP, WP, tid + 1), P[2], lds_get(ldsP, WP, tid + 3));
+        lds_put(ldsPL, NT, tid, add_half(P[2], gy));
+        const PS PR = sub_half(P[2], gy);
+        __syncthreads();
+        // Flux through the lower polar face of column j (scheme.rs:118-128).
+        Flux5 Fy{0.0, 0.0, 0.0, 0.0, 0.0};
+        if (act && j >= 1) {
+            PS Pl;
+            if (tid == 0) {
+                // column j0-1 of the left halo: its slope and its upper face's left state
+                PS gl = zero;
+                if (j0 - 1 >= 1 && j0 < nq) gl = plm5(th, lds_get(ldsP, WP, 0), lds_get(ldsP, WP, 1), P[2]);
+                Pl = add_half(lds_get(ldsP, WP, 1), gl);
+            } else {
+                Pl = lds_get(ldsPL, NT, tid - 1);
+            }
+            Cons G;
+            double Gs;
+            riemann<SYS, RS, 1>(Pl, PR, a.zp, &G, &Gs);
+            Fy = scaled(G, Gs, c, gld(zg_aq(B), (int64_t)r * (nq + 1) + j));
+        }
+        ldsF[tid] = Fy.d;
+        ldsF[NT + tid] = Fy.sr;
+        ldsF[2 * NT + tid] = Fy.sq;
+        ldsF[3 * NT + tid] = Fy.tau;
+        ldsF[4 * NT + tid] = Fy.s;
+        __syncthreads();
+        if (act) {
+            Flux5 Fn{0.0, 0.0, 0.0, 0.0, 0.0};
+            if (j + 1 < nq) {
+                if (j + 1 < j1) {
+                    Fn = Flux5{ldsF[tid + 1], ldsF[NT + tid + 1], ldsF[2 * NT + tid + 1], ldsF[3 * NT + tid + 1],
+                               ldsF[4 * NT + tid + 1]};
+                } else {
+                    // face j1, between this column and column j1 of the right halo
+                    const PS Q = lds_get(ldsP, WP, n + 2);
+                    PS gr = zero;
+                    if (j1 + 1 < nq) gr = plm5(th, P[2], Q, lds_get(ldsP, WP, n + 3));
+                    Cons G;
+                    double Gs;
+                    riemann<SYS, RS, 1>(add_half(P[2], gy), sub_half(Q, gr), a.zp, &G, &Gs);
+                    Fn = scaled(G, Gs, c, gld(zg_aq(B), (int64_t)r * (nq + 1) + j1));
+                }
+            }
+            // Sources x dV and the update (scheme.rs:130-158).
+            const double dv = gld(rg_volA(B), r) * gld(a.th.dcost, j);
+            double ssr, ssq;
+            sources<SYS>(P[2].P, gld(rg_rinv(B), r), gld(a.th.cot, j), a.zp, &ssr, &ssq);
+            const double sc0 = 0.0 * c * dv;
+            const double sc1 = ssr * c * dv;
+            const double sc2 = ssq * c * dv;
+            const double sc3 = 0.0 * c * dv;
+            const double du0 = (sc0 - (Fk.d - Fm.d) - (Fn.d - Fy.d)) * dt;
+            const double du1 = (sc1 - (Fk.sr - Fm.sr) - (Fn.sr - Fy.sr)) * dt;
+            const double du2 = (sc2 - (Fk.sq - Fm.sq) - (Fn.sq - Fy.sq)) * dt;
+            const double du3 = (sc3 - (Fk.tau - Fm.tau) - (Fn.tau - Fy.tau)) * dt;
+            const double ds = ((Fk.s - Fm.s) + (Fn.s - Fy.s)) * -dt;
+            const int64_t o = (int64_t)r * nq + j;
+            const double* Uin = slot_of(B, a.in_slot);
+            double v0 = gld(Uin, o) + du0;
+            double v1 = gld(Uin, o + np) + du1;
+            double v2 = gld(Uin, o + 2 * np) + du2;
+            double v3 = gld(Uin, o + 3 * np) + du3;
+            double v4 = gld(Uin, o + 4 * np) + ds;
+            if (avg) {
+                const double* U0 = slot_of(B, a.avg_slot);
+                v0 = v0 * w1 + gld(U0, o) * bw;
+                v1 = v1 * w1 + gld(U0, o + np) * bw;
+                v2 = v2 * w1 + gld(U0, o + 2 * np) * bw;
+                v3 = v3 * w1 + gld(U0, o + 3 * np) * bw;
+                v4 = v4 * w1 + gld(U0, o + 4 * np) * bw;
+            }
+            double* out = slot_of(B, a.out_slot) + o;
+            gst(out, 0, v0);
+            gst(out, np, v1);
+            gst(out, 2 * np, v2);
+            gst(out, 3 * np, v3);
+            gst(out, 4 * np, v4);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Warp-specialised stage (stage form 3): the same stage as k_stage, with the
 // work of a work item's column split between two kinds of wavefront.
 //   * flux waves (threads [0, NT), one polar column each) march like k_stage
@@ -1321,11 +1471,41 @@ static int occupancy_ws(int threads)
     return e == hipSuccess ? n : 0;
 }
 
+template <int SYS, int RS>
+static hipError_t launch_stage_rows(const StageArgs& a, int grid, int threads, hipStream_t s)
+{
+    switch (threads) {
+    case 64: hipLaunchKernelGGL((k_stage_rows<SYS, RS, 64>), dim3(grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((k_stage_rows<SYS, RS, 128>), dim3(grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_stage_rows<SYS, RS, 256>), dim3(grid), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_stage_rows<SYS, RS, 512>), dim3(grid), dim3(512), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int SYS, int RS>
+static int occupancy_rows(int threads)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (threads) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_rows<SYS, RS, 64>, 64, 0); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_rows<SYS, RS, 128>, 128, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_rows<SYS, RS, 256>, 256, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_rows<SYS, RS, 512>, 512, 0); break;
+    default: break;
+    }
+    return e == hipSuccess ? n : 0;
+}
+
 // Stage forms: 1 fused, 2 the flux pass of the two-pass stage, 3 the
-// warp-specialised fused stage.
+// warp-specialised fused stage, 4 the row-unit flux pass of the two-pass
+// stage.
 template <int SYS, int RS, bool ST, bool G>
 static hipError_t launch_stage_v(const StageArgs& a, int grid, int threads, int form, hipStream_t s)
 {
+    if (form == STAGE_ROWS) return launch_stage_rows<SYS, RS>(a, grid, threads, s);
     if (form == STAGE_TWO_PASS) return launch_stage_p<SYS, RS, false, false, true>(a, grid, threads, s);
     if (form == STAGE_SPECIALISED) return launch_stage_ws<SYS, RS, ST, G>(a, grid, threads, s);
     return launch_stage_p<SYS, RS, ST, G, false>(a, grid, threads, s);
@@ -1334,6 +1514,7 @@ static hipError_t launch_stage_v(const StageArgs& a, int grid, int threads, int 
 template <int SYS, int RS, bool ST, bool G>
 static int occupancy_v(int threads, int form)
 {
+    if (form == STAGE_ROWS) return occupancy_rows<SYS, RS>(threads);
     if (form == STAGE_TWO_PASS) return occupancy_p<SYS, RS, false, false, true>(threads);
     if (form == STAGE_SPECIALISED) return occupancy_ws<SYS, RS, ST, G>(threads);
     return occupancy_p<SYS, RS, ST, G, false>(threads);

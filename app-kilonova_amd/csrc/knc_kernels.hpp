@@ -69,8 +69,10 @@ struct GhostRows {
 
 // One stage in one of its forms (knc_options::stage_passes): fused (c2p
 // inside the stage kernel), the flux pass of the two-pass stage (which reads
-// the primitive planes k_prim wrote), or the warp-specialised fused stage.
-enum { STAGE_FUSED = 1, STAGE_TWO_PASS = 2, STAGE_SPECIALISED = 3 };
+// the primitive planes k_prim wrote), the warp-specialised fused stage, or
+// the two-pass stage with a flux pass of one (work item, row) unit per
+// workgroup pass (k_stage_rows; launched with the row-pass grid).
+enum { STAGE_FUSED = 1, STAGE_TWO_PASS = 2, STAGE_SPECIALISED = 3, STAGE_ROWS = 4 };
 hipError_t launch_stage(int system, int rs, bool stats, bool learned, int form, const StageArgs& a, int grid,
                         int threads, hipStream_t s);
 // The c2p pass of the two-pass stage (one (work item, row) unit per

@@ -115,6 +115,9 @@ typedef struct knc_options {
                                 3: fused stage with the c2p and the flux work on
                                 separate wavefronts of each workgroup (128 or 256
                                 polar columns per work item, at most 32 rows);
+                                4: the c2p pass, then a flux pass over (work
+                                item, row) units, all rows at once (for meshes
+                                of few-row blocks);
                                 0 = 1 (knc_autotune may choose any) */
 } knc_options;
 

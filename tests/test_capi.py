@@ -89,7 +89,7 @@ def test_invalid_options_are_rejected_before_any_device_call():
     h = knc.relativistic_hydro()
     out = ctypes.c_void_p()
     for bad in (capi.Options(1, 5, 0, 0, 0, 1, 0, 0),       # unknown c2p guess mode
-                capi.Options(1, 0, 0, 0, 0, 1, 0, 4)):      # stage_passes must be 0, 1, 2 or 3
+                capi.Options(1, 0, 0, 0, 0, 1, 0, 5)):      # stage_passes must be 0 to 4
         rc = knc.lib().knc_context_create(ctypes.byref(h), ctypes.byref(bad), ctypes.byref(out))
         assert rc == 10 and out.value is None
 

@@ -1,7 +1,9 @@
 """The other stage forms against the fused stage (stage_passes = 1), bit for
 bit: the two-pass stage (2: a c2p pass, then the flux pass over the
-primitive planes) and the warp-specialised stage (3: c2p and flux work on
-separate wavefronts, primitive rows passed through an LDS ring).  SRHD and
+primitive planes), the warp-specialised stage (3: c2p and flux work on
+separate wavefronts, primitive rows passed through an LDS ring) and the
+row-unit two-pass stage (4: the flux pass over (work item, row) units, each
+radial face formed by both rows that share it).  SRHD and
 Euler, HLLE and HLLC, RK1/2/3, tiles of 64 to 512 columns including a
 partial last tile, per-block and batched launches, the guess modes,
 excision births and removals, and an error reported at the same zone."""
@@ -32,7 +34,8 @@ def _pair(setup, steps, fold=5, excision=False, form=3, **options):
 
 
 @pytest.mark.parametrize("form,threads,rows", [(3, 256, 32), (3, 256, 8), (3, 128, 16), (3, 128, 4),
-                                               (2, 256, 32), (2, 512, 16), (2, 128, 7), (2, 64, 32)])
+                                               (2, 256, 32), (2, 512, 16), (2, 128, 7), (2, 64, 32),
+                                               (4, 256, 32), (4, 256, 4), (4, 128, 7), (4, 64, 16), (4, 512, 8)])
 def test_bomb_jet_uneven_blocks_batched(form, threads, rows):
     print(f"form {form} vs fused: bomb_jet 512x256, {threads} columns x {rows} rows", flush=True)
     s = knc.setups.bomb_jet(nr=512, nq=256, nblocks=8, nshell=8)
@@ -41,7 +44,8 @@ def test_bomb_jet_uneven_blocks_batched(form, threads, rows):
 
 
 @pytest.mark.parametrize("form,nq,threads", [(3, 192, 256), (3, 96, 128), (3, 1, 128),
-                                             (2, 192, 256), (2, 96, 64), (2, 1, 128), (2, 320, 512)])
+                                             (2, 192, 256), (2, 96, 64), (2, 1, 128), (2, 320, 512),
+                                             (4, 192, 256), (4, 96, 64), (4, 1, 128), (4, 320, 512)])
 def test_partial_tiles_and_one_dimensional(form, nq, threads):
     print(f"form {form} vs fused: jet_in_cloud 256x{nq}, {threads} columns", flush=True)
     s = knc.setups.jet_in_cloud(nr=256, nq=nq, nblocks=4, corr=4.0)
@@ -49,7 +53,7 @@ def test_partial_tiles_and_one_dimensional(form, nq, threads):
     assert _same(ref, ws)
 
 
-@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("form", [2, 3, 4])
 @pytest.mark.parametrize("solver", [knc.HLLE, knc.HLLC])
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_euler_blast_solvers_and_orders(form, solver, order):
@@ -60,7 +64,7 @@ def test_euler_blast_solvers_and_orders(form, solver, order):
     assert _same(ref, ws)
 
 
-@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("form", [2, 3, 4])
 @pytest.mark.parametrize("solver", [knc.HLLE, knc.HLLC])
 def test_srhd_solvers_rk3(form, solver):
     s = knc.setups.jet_in_cloud(nr=256, nq=128, nblocks=4, corr=4.0, riemann=solver)
@@ -69,7 +73,7 @@ def test_srhd_solvers_rk3(form, solver):
     assert _same(ref, ws)
 
 
-@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("form", [2, 3, 4])
 @pytest.mark.parametrize("guess", [knc.GUESS_PREVIOUS, knc.GUESS_LEARNED_FOLD])
 def test_guess_modes_with_excision(form, guess):
     print(f"form {form} vs fused: wind_shock cycling blocks, guess mode {guess}", flush=True)
@@ -97,7 +101,7 @@ def test_error_reported_at_the_same_zone():
     b = s.blocks[1]
     b.conserved[17, 40, 0] = -1.0
     errs = []
-    for form in (1, 2, 3):
+    for form in (1, 2, 3, 4):
         ctx = knc.context_from_setup(s, launch_mode=knc.BATCHED, stage_passes=form)
         with pytest.raises(knc.HydroError) as e:
             ctx.advance_steps(1e-12, 1)
@@ -105,7 +109,7 @@ def test_error_reported_at_the_same_zone():
     # the first stage fails at the planted zone; the second (dt beyond the
     # CFL limit at the inner jet) would fail at block 0, but the first
     # failing stage is the one reported
-    assert errs[0] == errs[1] == errs[2] and errs[0][:4] == (1, b.index, 17, 40)
+    assert errs[0] == errs[1] == errs[2] == errs[3] and errs[0][:4] == (1, b.index, 17, 40)
 
 
 def test_autotuner_may_choose_the_specialised_stage():
@@ -113,8 +117,25 @@ def test_autotuner_may_choose_the_specialised_stage():
     ctx = knc.context_from_setup(s, launch_mode=knc.BATCHED)
     tuned = ctx.autotune(force=True)
     print("autotuned:", tuned, flush=True)
-    assert tuned["stage_passes"] in (1, 2, 3)
+    assert tuned["stage_passes"] in (1, 2, 3, 4)
     ref = knc.context_from_setup(s, launch_mode=knc.BATCHED)
     for _ in range(2):
         assert ref.advance(5) == ctx.advance(5)
     assert _same(ref, ctx)
+
+
+@pytest.mark.parametrize("mode", [knc.BATCHED, knc.PER_BLOCK])
+def test_row_units_on_the_reference_setup_with_model_boundaries(mode):
+    """The reference's jet_in_cloud setup (4-row blocks, device model ghost
+    rows on both sides, moving excision) with 128 polar zones: the row-unit
+    form equals the fused stage bit for bit, across the excision events of
+    60 steps."""
+    from knc import presets
+    p = presets.parse(presets.apply_overrides(presets.SETUPS["jet_in_cloud"], ["mesh.num_polar_zones=128"]))
+    ref = presets.context_from_preset(p, launch_mode=mode)
+    rows = presets.context_from_preset(p, launch_mode=mode, stage_passes=4)
+    for _ in range(6):
+        assert ref.advance(10) == rows.advance(10)
+    assert ref.block_indexes() == rows.block_indexes()
+    assert ref.time == rows.time
+    assert _same(ref, rows)
