@@ -623,9 +623,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
 // radial slopes of rows r-1, r and r+1 and both radial faces of row r (each
 // face is formed by the two rows that share it, from the same operands, so
 // with the same bits), the polar slopes and faces of row r through LDS, and
-// the update.  Items of a few rows (the reference's 4-row blocks) march a
-// few steps each and leave most of the GPU idle; their rows as units fill
-// it.  The expressions and their order are k_stage's.
+// the update.  It reads five primitive rows per zone and solves a third
+// Riemann problem per zone, so the march is faster wherever it fills the
+// GPU; on meshes of few zones and in per-block launches the rows as units
+// fill it, and the autotuner chooses between the two.  The expressions and
+// their order are k_stage's.
 // ---------------------------------------------------------------------------
 __device__ inline PS row_prim(const RowSrc& src, int col)
 {
