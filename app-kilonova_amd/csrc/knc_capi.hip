@@ -312,6 +312,7 @@ void drop_graphs(knc_context* c)
 {
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     c->graphs.clear();
+    c->graph_keys_seen.clear();
 }
 
 // Polar table (layout in knc_device.hpp), with separate sin and cos calls.
@@ -772,21 +773,24 @@ int require_blocks(knc_context* c)
 
 // Enqueues n RK steps from slot c->cur: replayed from a hipGraph captured
 // per (starting slot, n) in batched mode, launched one by one otherwise.
+// A run is captured on the second use of its key since the block set last
+// changed: the first runs directly, so that runs cut short by excision
+// events, which may never repeat, do not pay for a capture.
 int enqueue_steps(knc_context* c, int n, int first)
 {
     if (n <= 0) return KNC_OK;
     const int start_slot = c->cur;
     c->segment_slot = start_slot;
-    const bool graph = c->o.launch_mode == KNC_LAUNCH_BATCHED && c->o.use_graph && n > 1;
+    bool graph = c->o.launch_mode == KNC_LAUNCH_BATCHED && c->o.use_graph && n > 1;
+    // the ghost-row pointers of model boundaries depend on the fold step
+    const auto key = std::make_tuple(start_slot, n, any_model_side(c) ? first : 0, (int)c->dt_in_first_stage);
+    if (graph && !c->graphs.count(key) && c->graph_keys_seen.insert(key).second) graph = false;
     if (!graph) {
         int rc;
         for (int s = 0; s < n; ++s)
             if ((rc = enqueue_rk_step(c, s + 1, first + s, s == 0))) return rc;
         return KNC_OK;
     }
-    // the ghost-row pointers of model boundaries depend on the fold step
-    const bool model = any_model_side(c);
-    auto key = std::make_tuple(start_slot, n, model ? first : 0, (int)c->dt_in_first_stage);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         hipGraph_t g;

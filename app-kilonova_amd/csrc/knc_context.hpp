@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <set>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -100,6 +101,8 @@ struct knc_context {
     // graphs keyed by (starting slot, steps, first fold step for model
     // boundaries, whether the run's first stage carries the CFL reduction)
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+    // keys run once, directly, since the graphs were last dropped
+    std::set<std::tuple<int, int, int, int>> graph_keys_seen;
 
     // learned c2p guess: device copy of the network's operand fragments
     void* d_mlp = nullptr;
